@@ -1,0 +1,705 @@
+// C-ABI of libgvom_hip.so (declared in include/gvom_abi.h): handle lifetime, device
+// memory, the ring-buffer bookkeeping of the reference class Gvom
+// (scripts/gvom.py:21-101, 198-216, 225-308) and the launch sequences of the scan and
+// combine stages.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gvom_kernels.h"
+
+hipError_t launch_scan_excl_u32(uint32_t *data, int64_t n, int32_t *total, hipStream_t s);
+hipError_t launch_clear_slots(int32_t *slot_tables, const DevState *st, int S, const Geo &g, hipStream_t s);
+
+namespace {
+thread_local std::string g_create_error;
+
+int64_t bits_for(int64_t v) {  // smallest b with 2^b >= v
+    int64_t b = 0;
+    while (((int64_t)1 << b) < v) ++b;
+    return b;
+}
+}  // namespace
+
+struct gvom_handle {
+    gvom_params p;
+    Geo g;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string err;
+
+    DevState *st = nullptr;
+    int32_t *slot_tables = nullptr;
+    SlotRec *slot_recs = nullptr;
+    int64_t ncap = 0;
+    int32_t *comb_tables = nullptr;
+    CombRec *comb_recs = nullptr;
+    int64_t ccap = 0;
+    double *height = nullptr, *inferred = nullptr, *slope_x = nullptr, *slope_y = nullptr, *rough = nullptr,
+           *delta = nullptr;
+    int32_t *pos = nullptr, *neg = nullptr, *vis = nullptr, *density = nullptr, *col_count = nullptr;
+    int32_t *export_buf = nullptr;
+
+    int64_t ptcap = 0;
+    void *pts_in = nullptr;
+    void *world = nullptr;
+    uint8_t *valid = nullptr;
+    uint32_t *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr, *hist = nullptr, *blk = nullptr,
+             *seg_key = nullptr;
+    int32_t *seg_start = nullptr;
+    Moment *mom = nullptr;
+    ScanDesc *sd = nullptr;
+
+    // host mirror of the ring state (gvom.py buffer_index / last_buffer_index / slots)
+    int next_slot = 0, last_slot = 0, filled_mask = 0, parity = 0, have_last = 0;
+    double slot_origin[GVOM_MAX_BUFFER][3] = {};
+    double latest_ego[3] = {0, 0, 0};
+
+    // Stage timing: one set of 9 events per process call (0..4) and combine call (5..8);
+    // gvom_stage_times averages every stage over all sets recorded since timing was enabled.
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> set_used;   // bit k set when event k of that set was recorded
+    int cur_set = -1;
+    int ray_block = 256, ray_rpt = 1;
+};
+
+#define CKH(h, expr)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            (h)->err = std::string(#expr) + ": " + hipGetErrorString(e_);            \
+            return GVOM_EHIP;                                                        \
+        }                                                                            \
+    } while (0)
+
+static void free_all(gvom_handle *h) {
+    void *ptrs[] = {h->st, h->slot_tables, h->slot_recs, h->comb_tables, h->comb_recs, h->height, h->inferred,
+                    h->slope_x, h->slope_y, h->rough, h->delta, h->pos, h->neg, h->vis, h->density,
+                    h->col_count, h->export_buf, h->pts_in, h->world, h->valid, h->k0, h->v0, h->k1, h->v1,
+                    h->hist, h->blk, h->seg_key, h->seg_start, h->mom, h->sd};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &e : h->ev_pool)
+        if (e) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+static int fail_create(gvom_handle *h, const std::string &msg, int code) {
+    g_create_error = msg;
+    if (h) {
+        free_all(h);
+        delete h;
+    }
+    return code;
+}
+
+template <typename T>
+static hipError_t dalloc(T **p, int64_t count) {
+    if (count <= 0) count = 1;
+    return hipMalloc((void **)p, (size_t)count * sizeof(T));
+}
+
+extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
+    if (!params || !out) return fail_create(nullptr, "null argument", GVOM_EINVAL);
+    *out = nullptr;
+    const gvom_params &p = *params;
+    if (!(p.xy_resolution > 0 && std::isfinite(p.xy_resolution) && p.z_resolution > 0 &&
+          std::isfinite(p.z_resolution)))
+        return fail_create(nullptr, "resolutions must be positive and finite", GVOM_EINVAL);
+    if (p.xy_size < 4 || p.xy_size > 8192 || p.z_size < 4 || p.z_size > 256)
+        return fail_create(nullptr, "xy_size must be in [4, 8192] and z_size in [4, 256]", GVOM_EINVAL);
+    if (p.buffer_size < 1 || p.buffer_size > GVOM_MAX_BUFFER)
+        return fail_create(nullptr, "buffer_size must be in [1, 30]", GVOM_EINVAL);
+    if (p.xy_eigen_dist < 0 || p.xy_eigen_dist > 8 || p.z_eigen_dist < 0 || p.z_eigen_dist > 8)
+        return fail_create(nullptr, "eigen distances must be in [0, 8]", GVOM_EINVAL);
+    const int64_t XY = p.xy_size, Z = p.z_size, V = XY * XY * Z;
+    if (V * (int64_t)(p.buffer_size + 1) >= ((int64_t)1 << 31))
+        return fail_create(nullptr, "map too large: voxels * (buffer_size + 1) must be < 2^31", GVOM_EINVAL);
+
+    gvom_handle *h = new gvom_handle();
+    h->p = p;
+    Geo &g = h->g;
+    g.xy_res = p.xy_resolution;
+    g.z_res = p.z_resolution;
+    g.min_d2 = p.min_distance * p.min_distance;
+    g.XY = (int32_t)XY;
+    g.Z = (int32_t)Z;
+    g.V = V;
+    g.X = XY * XY;
+    g.B = p.buffer_size;
+    g.ex = p.xy_eigen_dist;
+    g.ez = p.z_eigen_dist;
+    g.XYp = (int32_t)(XY + 2 * g.ex);
+    g.Zp = (int32_t)(Z + 2 * g.ez);
+    int64_t vpad = (int64_t)g.XYp * g.XYp * g.Zp;
+    int64_t vmax = vpad > V ? vpad : V;
+    g.VB = (int32_t)(bits_for(vmax + 1) + 1);
+    if (g.VB > 27) return fail_create(h, "map too large for 32-bit voxel sort keys", GVOM_EINVAL);
+    g.pos_thr = p.positive_obstacle_threshold;
+    g.neg_thr = p.negative_obstacle_threshold;
+    g.slope_thr = p.slope_obstacle_threshold;
+    g.robot_h = p.robot_height;
+    g.robot_r = p.robot_radius;
+    g.g2l = p.ground_to_lidar_height;
+
+    hipError_t e;
+#define CKC(expr)                                                                              \
+    do {                                                                                       \
+        e = (expr);                                                                            \
+        if (e != hipSuccess) return fail_create(h, std::string(#expr) + ": " + hipGetErrorString(e), GVOM_EHIP); \
+    } while (0)
+    CKC(hipGetDevice(&h->device));
+    CKC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    CKC(dalloc(&h->st, 1));
+    CKC(hipMemset(h->st, 0, sizeof(DevState)));
+    CKC(dalloc(&h->slot_tables, V * g.B));
+    h->ccap = V;
+    CKC(dalloc(&h->comb_tables, 2 * V));
+    CKC(dalloc(&h->comb_recs, 2 * h->ccap));
+    const int64_t X = g.X;
+    CKC(dalloc(&h->height, X));
+    CKC(dalloc(&h->inferred, X));
+    CKC(dalloc(&h->slope_x, X));
+    CKC(dalloc(&h->slope_y, X));
+    CKC(dalloc(&h->rough, X));
+    CKC(dalloc(&h->delta, X));
+    CKC(dalloc(&h->pos, X));
+    CKC(dalloc(&h->neg, X));
+    CKC(dalloc(&h->vis, X));
+    CKC(dalloc(&h->density, X));
+    CKC(dalloc(&h->col_count, X + 1));
+    CKC(dalloc(&h->sd, GVOM_MAX_BATCH));
+#undef CKC
+    *out = h;
+    return GVOM_OK;
+}
+
+extern "C" void gvom_destroy(gvom_handle *h) {
+    if (!h) return;
+    {
+        std::lock_guard<std::mutex> lk(h->mu);
+        if (h->stream) (void)hipStreamSynchronize(h->stream);
+        free_all(h);
+    }
+    delete h;
+}
+
+extern "C" const char *gvom_last_error(const gvom_handle *h) {
+    return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" void *gvom_stream(gvom_handle *h) { return h ? (void *)h->stream : nullptr; }
+
+// Grow per-batch work buffers (points) and per-slot record capacity (cells per scan).
+static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) {
+    const Geo &g = h->g;
+    int64_t need_ncap = max_scan < g.V ? max_scan : g.V;
+    if (need_ncap < 1) need_ncap = 1;
+    if (need_ncap > h->ncap) {
+        int64_t nc = h->ncap * 2 > need_ncap ? h->ncap * 2 : need_ncap;
+        if (nc > g.V) nc = g.V;
+        SlotRec *nr = nullptr;
+        CKH(h, dalloc(&nr, nc * g.B));
+        if (h->slot_recs) {
+            for (int s = 0; s < g.B; ++s)
+                CKH(h, hipMemcpyAsync(nr + (int64_t)s * nc, h->slot_recs + (int64_t)s * h->ncap,
+                                      (size_t)h->ncap * sizeof(SlotRec), hipMemcpyDeviceToDevice, h->stream));
+            CKH(h, hipStreamSynchronize(h->stream));
+            CKH(h, hipFree(h->slot_recs));
+        }
+        h->slot_recs = nr;
+        h->ncap = nc;
+    }
+    // Always hold work buffers, also for an empty first scan: the segment pass writes its
+    // end sentinel even when there are no points.
+    if (batch_pts < 1) batch_pts = 1;
+    if (batch_pts > h->ptcap) {
+        int64_t pc = h->ptcap * 2 > batch_pts ? h->ptcap * 2 : batch_pts;
+        if (pc < 4096) pc = 4096;
+        if (pc >= ((int64_t)1 << 31) - 1) {
+            h->err = "too many points in one batch";
+            return GVOM_EINVAL;
+        }
+        CKH(h, hipStreamSynchronize(h->stream));
+        void *ptrs[] = {h->pts_in, h->world, h->valid, h->k0, h->v0, h->k1, h->v1, h->hist, h->blk,
+                        h->seg_key, h->seg_start, h->mom};
+        for (void *q : ptrs)
+            if (q) CKH(h, hipFree(q));
+        CKH(h, hipMalloc(&h->pts_in, (size_t)pc * 3 * sizeof(double)));
+        CKH(h, hipMalloc(&h->world, (size_t)pc * 3 * sizeof(double)));
+        CKH(h, dalloc(&h->valid, pc));
+        CKH(h, dalloc(&h->k0, pc));
+        CKH(h, dalloc(&h->v0, pc));
+        CKH(h, dalloc(&h->k1, pc));
+        CKH(h, dalloc(&h->v1, pc));
+        CKH(h, dalloc(&h->hist, radix_hist_entries(pc)));
+        CKH(h, dalloc(&h->blk, pc / 2048 + 2));
+        CKH(h, dalloc(&h->seg_key, pc));
+        CKH(h, dalloc(&h->seg_start, pc + 1));
+        CKH(h, dalloc(&h->mom, pc));
+        h->ptcap = pc;
+    }
+    return GVOM_OK;
+}
+
+extern "C" int gvom_reserve(gvom_handle *h, int64_t max_points, int64_t max_batch_points) {
+    if (!h) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (max_points < 0 || max_batch_points < 0) {
+        h->err = "negative size";
+        return GVOM_EINVAL;
+    }
+    if (max_batch_points < max_points) max_batch_points = max_points;
+    return ensure_capacity(h, max_points, max_batch_points);
+}
+
+static void host_origin(const Geo &g, const double ego[3], double o[3]) {
+    o[0] = std::floor((ego[0] / g.xy_res) - (double)g.XY / 2.0);
+    o[1] = std::floor((ego[1] / g.xy_res) - (double)g.XY / 2.0);
+    o[2] = std::floor((ego[2] / g.z_res) - (double)g.Z / 2.0);
+}
+
+// Timing events: process records 0..4 of a new set, combine records 5..8 of the current set
+// (or of a new one when that set already has a combine).
+static constexpr int EV_PER_SET = 9;
+static constexpr int MAX_SETS = 8192;
+
+static void stage_event(gvom_handle *h, int k) {
+    if (!h->timing) return;
+    if (k == 0 || (k == 5 && (h->cur_set < 0 || (h->set_used[h->cur_set] >> 5 & 1)))) {
+        if ((int)h->set_used.size() >= MAX_SETS) return;
+        h->set_used.push_back(0);
+        h->cur_set = (int)h->set_used.size() - 1;
+        while ((int)h->ev_pool.size() < EV_PER_SET * (h->cur_set + 1)) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            h->ev_pool.push_back(e);
+        }
+    }
+    if (h->cur_set < 0) return;
+    if (hipEventRecord(h->ev_pool[EV_PER_SET * h->cur_set + k], h->stream) == hipSuccess)
+        h->set_used[h->cur_set] |= 1 << k;
+}
+
+// The scan stage for S scans (gvom.py:105-220 for each).
+static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets, int S, int dtype,
+                        const double *ego, const void *T, int t_dtype, bool on_device) {
+    const Geo &g = h->g;
+    if (S < 1 || S > GVOM_MAX_BATCH || S > g.B) {
+        h->err = "number of scans must be in [1, min(32, buffer_size)]";
+        return GVOM_EINVAL;
+    }
+    if (dtype != GVOM_F32 && dtype != GVOM_F64) {
+        h->err = "point dtype must be GVOM_F32 or GVOM_F64";
+        return GVOM_EINVAL;
+    }
+    if (T && t_dtype != GVOM_F32 && t_dtype != GVOM_F64 && t_dtype != GVOM_NO_TRANSFORM) {
+        h->err = "transform dtype must be GVOM_F32, GVOM_F64 or GVOM_NO_TRANSFORM";
+        return GVOM_EINVAL;
+    }
+    if (offsets[0] != 0) {
+        h->err = "offsets[0] must be 0";
+        return GVOM_EINVAL;
+    }
+    int64_t max_scan = 0;
+    for (int s = 0; s < S; ++s) {
+        int64_t n = offsets[s + 1] - offsets[s];
+        if (n < 0) {
+            h->err = "offsets must be non-decreasing";
+            return GVOM_EINVAL;
+        }
+        if (n > max_scan) max_scan = n;
+        for (int k = 0; k < 3; ++k) {
+            const double res = k < 2 ? g.xy_res : g.z_res;
+            if (!std::isfinite(ego[3 * s + k]) || std::fabs(ego[3 * s + k] / res) >= 4194304.0) {
+                h->err = "ego position must be finite with |ego / resolution| < 2^22";
+                return GVOM_EINVAL;
+            }
+        }
+    }
+    const int64_t NT = offsets[S];
+    if (NT > 0 && !xyz) {
+        h->err = "null point array";
+        return GVOM_EINVAL;
+    }
+    int sbits = 0;
+    while ((1 << sbits) < S) ++sbits;
+    if (g.VB + sbits > 32) {
+        h->err = "batch too large for 32-bit sort keys on this map size";
+        return GVOM_EINVAL;
+    }
+    int rc = ensure_capacity(h, max_scan, NT);
+    if (rc) return rc;
+    if (!h->slot_recs || !h->pts_in || !h->world || !h->valid || !h->k0 || !h->v0 || !h->k1 || !h->v1 ||
+        !h->hist || !h->blk || !h->seg_key || !h->seg_start || !h->mom || h->ptcap < NT || h->ptcap < 1) {
+        h->err = "internal: work buffers not allocated";
+        return GVOM_ESTATE;
+    }
+
+    ScanDesc sdh[GVOM_MAX_BATCH];
+    std::memset(sdh, 0, sizeof(sdh));
+    for (int s = 0; s < S; ++s) {
+        ScanDesc &d = sdh[s];
+        d.t_mode = (T && t_dtype != GVOM_NO_TRANSFORM) ? t_dtype : GVOM_NO_TRANSFORM;
+        if (d.t_mode == GVOM_F32) {
+            const float *t = (const float *)T + 16 * s;
+            for (int k = 0; k < 12; ++k) d.T[k] = (double)t[k];
+        } else if (d.t_mode == GVOM_F64) {
+            const double *t = (const double *)T + 16 * s;
+            for (int k = 0; k < 12; ++k) d.T[k] = t[k];
+        }
+        d.ego[0] = ego[3 * s];
+        d.ego[1] = ego[3 * s + 1];
+        d.ego[2] = ego[3 * s + 2];
+        d.offset = offsets[s];
+        d.count = offsets[s + 1] - offsets[s];
+    }
+    hipStream_t st = h->stream;
+    const size_t esz = dtype == GVOM_F32 ? 4 : 8;
+    const void *pts = xyz;
+    if (!on_device && NT > 0) {
+        CKH(h, hipMemcpyAsync(h->pts_in, xyz, (size_t)NT * 3 * esz, hipMemcpyHostToDevice, st));
+        pts = h->pts_in;
+    }
+    CKH(h, hipMemcpyAsync(h->sd, sdh, sizeof(ScanDesc) * S, hipMemcpyHostToDevice, st));
+    stage_event(h, 0);
+    CKH(h, launch_clear_slots(h->slot_tables, h->st, S, g, st));
+    CKH(h, launch_prepare(dtype, pts, h->world, h->k0, h->v0, h->valid, h->sd, S, NT, g, h->st, st));
+    stage_event(h, 1);
+    CKH(h, launch_raycast(dtype, h->world, h->valid, h->sd, S, NT, g, h->st, h->slot_tables, h->ray_block,
+                          h->ray_rpt, st));
+    stage_event(h, 2);
+    uint32_t *sk = h->k0, *sv = h->v0;
+    CKH(h, radix_sort(h->k0, h->v0, h->k1, h->v1, NT, g.VB + sbits, h->hist, &sk, &sv, st));
+    CKH(h, launch_segments(sk, NT, h->seg_start, h->seg_key, h->blk, S, g, h->st, st));
+    stage_event(h, 3);
+    CKH(h, launch_moments(dtype, h->world, sv, h->seg_start, h->seg_key, NT, h->sd, S, g, h->st, h->slot_tables,
+                          h->slot_recs, h->ncap, h->mom, st));
+    CKH(h, launch_metrics(h->seg_key, h->mom, NT, S, g, h->st, h->slot_tables, h->slot_recs, h->ncap, st));
+    CKH(h, launch_commit_process(h->st, S, g.B, st));
+    stage_event(h, 4);
+
+    for (int s = 0; s < S; ++s) {
+        int slot = (h->next_slot + s) % g.B;
+        h->filled_mask |= 1 << slot;
+        host_origin(g, &ego[3 * s], h->slot_origin[slot]);
+    }
+    h->last_slot = (h->next_slot + S - 1) % g.B;
+    h->next_slot = (h->next_slot + S) % g.B;
+    h->latest_ego[0] = ego[3 * (S - 1)];
+    h->latest_ego[1] = ego[3 * (S - 1) + 1];
+    h->latest_ego[2] = ego[3 * (S - 1) + 2];
+    return GVOM_OK;
+}
+
+extern "C" int gvom_process_pointcloud(gvom_handle *h, const void *xyz, int64_t n, int32_t dtype,
+                                       const double ego[3], const void *T, int32_t t_dtype) {
+    if (!h || !ego || n < 0) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    int64_t off[2] = {0, n};
+    return process_impl(h, xyz, off, 1, dtype, ego, T, t_dtype, false);
+}
+
+extern "C" int gvom_process_pointcloud_device(gvom_handle *h, const void *xyz, int64_t n, int32_t dtype,
+                                              const double ego[3], const void *T, int32_t t_dtype) {
+    if (!h || !ego || n < 0) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    int64_t off[2] = {0, n};
+    return process_impl(h, xyz, off, 1, dtype, ego, T, t_dtype, true);
+}
+
+extern "C" int gvom_process_batch(gvom_handle *h, const void *xyz, const int64_t *offsets, int32_t n_scans,
+                                  int32_t dtype, const double *ego, const void *T, int32_t t_dtype,
+                                  int32_t on_device) {
+    if (!h || !offsets || !ego) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    return process_impl(h, xyz, offsets, n_scans, dtype, ego, T, t_dtype, on_device != 0);
+}
+
+// The combine stage (gvom.py:222-393).
+static int combine_impl(gvom_handle *h) {
+    const Geo &g = h->g;
+    if (!(h->filled_mask & (1 << h->last_slot))) return GVOM_NO_DATA;
+    hipStream_t st = h->stream;
+    const int cur = 1 - h->parity;
+    stage_event(h, 5);
+    CKH(h, launch_comb_tables(g, h->st, h->slot_tables, h->comb_tables, h->col_count, st));
+    stage_event(h, 6);
+    CKH(h, launch_scan_excl_u32((uint32_t *)h->col_count, g.X, &h->st->comb_total, st));
+    CKH(h, launch_comb_records(g, h->st, h->slot_tables, h->slot_recs, h->ncap, h->comb_tables, h->comb_recs,
+                               h->ccap, h->col_count, h->height, h->inferred, h->density, st));
+    stage_event(h, 7);
+    CKH(h, launch_maps2d(g, h->height, h->inferred, h->density, h->slope_x, h->slope_y, h->rough, h->delta,
+                         h->pos, h->neg, h->vis, st));
+    CKH(h, launch_commit_combine(h->st, st));
+    stage_event(h, 8);
+    h->parity = cur;
+    h->have_last = 1;
+    return GVOM_OK;
+}
+
+static void origin_metres(const gvom_handle *h, double out[3]) {
+    const double *o = h->slot_origin[h->last_slot];
+    out[0] = o[0] * h->g.xy_res;
+    out[1] = o[1] * h->g.xy_res;
+    out[2] = o[2] * h->g.z_res;
+}
+
+extern "C" int gvom_combine_maps(gvom_handle *h, double origin_m[3], int32_t *pos, int32_t *neg, double *rough,
+                                 int32_t *vis) {
+    if (!h) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    int rc = combine_impl(h);
+    if (rc) return rc;
+    const size_t X = (size_t)h->g.X;
+    hipStream_t st = h->stream;
+    if (pos) CKH(h, hipMemcpyAsync(pos, h->pos, X * 4, hipMemcpyDeviceToHost, st));
+    if (neg) CKH(h, hipMemcpyAsync(neg, h->neg, X * 4, hipMemcpyDeviceToHost, st));
+    if (rough) CKH(h, hipMemcpyAsync(rough, h->rough, X * 8, hipMemcpyDeviceToHost, st));
+    if (vis) CKH(h, hipMemcpyAsync(vis, h->vis, X * 4, hipMemcpyDeviceToHost, st));
+    CKH(h, hipStreamSynchronize(st));
+    if (origin_m) origin_metres(h, origin_m);
+    return GVOM_OK;
+}
+
+extern "C" int gvom_combine_maps_device(gvom_handle *h, double origin_m[3]) {
+    if (!h) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    int rc = combine_impl(h);
+    if (rc) return rc;
+    if (origin_m) origin_metres(h, origin_m);
+    return GVOM_OK;
+}
+
+extern "C" int gvom_device_maps(gvom_handle *h, int32_t **pos, int32_t **neg, double **rough, int32_t **vis) {
+    if (!h) return GVOM_EINVAL;
+    if (pos) *pos = h->pos;
+    if (neg) *neg = h->neg;
+    if (rough) *rough = h->rough;
+    if (vis) *vis = h->vis;
+    return GVOM_OK;
+}
+
+extern "C" int gvom_sync(gvom_handle *h) {
+    if (!h) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, hipStreamSynchronize(h->stream));
+    return GVOM_OK;
+}
+
+static int read_state(gvom_handle *h, DevState *out) {
+    CKH(h, hipMemcpyAsync(out, h->st, sizeof(DevState), hipMemcpyDeviceToHost, h->stream));
+    CKH(h, hipStreamSynchronize(h->stream));
+    return GVOM_OK;
+}
+
+static int export_table(gvom_handle *h, const int32_t *table, int32_t *out) {
+    if (!h->export_buf) CKH(h, dalloc(&h->export_buf, h->g.V));
+    CKH(h, launch_export_table(h->g, table, h->export_buf, h->stream));
+    CKH(h, hipMemcpyAsync(out, h->export_buf, (size_t)h->g.V * 4, hipMemcpyDeviceToHost, h->stream));
+    CKH(h, hipStreamSynchronize(h->stream));
+    return GVOM_OK;
+}
+
+extern "C" int gvom_get_slot(gvom_handle *h, int32_t slot, int32_t *index_map, int64_t cap, int32_t *hit,
+                             int32_t *total, float *min_h, double *metrics, double origin[3], int64_t *cells) {
+    if (!h) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (slot < 0 || slot >= h->g.B) {
+        h->err = "slot out of range";
+        return GVOM_EINVAL;
+    }
+    if (!(h->filled_mask & (1 << slot))) return GVOM_NO_DATA;
+    DevState ds;
+    int rc = read_state(h, &ds);
+    if (rc) return rc;
+    const int64_t C = ds.slot_cells[slot];
+    if (cells) *cells = C;
+    if (origin)
+        for (int k = 0; k < 3; ++k) origin[k] = ds.slot_origin[slot][k];
+    if (index_map) {
+        rc = export_table(h, h->slot_tables + (int64_t)slot * h->g.V, index_map);
+        if (rc) return rc;
+    }
+    if (hit || total || min_h || metrics) {
+        if (cap < C) {
+            h->err = "capacity smaller than the cell count";
+            return GVOM_ECAP;
+        }
+        std::vector<SlotRec> recs((size_t)(C > 0 ? C : 1));
+        if (C > 0) {
+            CKH(h, hipMemcpy(recs.data(), h->slot_recs + (int64_t)slot * h->ncap, (size_t)C * sizeof(SlotRec),
+                             hipMemcpyDeviceToHost));
+        }
+        for (int64_t i = 0; i < C; ++i) {
+            if (hit) hit[i] = recs[i].hit;
+            if (total) total[i] = recs[i].total;
+            if (min_h) min_h[i] = recs[i].min_h;
+            if (metrics)
+                for (int j = 0; j < 10; ++j) metrics[10 * i + j] = recs[i].m[j];
+        }
+    }
+    return GVOM_OK;
+}
+
+extern "C" int gvom_get_combined(gvom_handle *h, int32_t *index_map, int64_t cap, int32_t *hit, int32_t *total,
+                                 float *min_h, float *metrics, float *eigen, double origin[3], int64_t *cells) {
+    if (!h) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->have_last) return GVOM_NO_DATA;
+    DevState ds;
+    int rc = read_state(h, &ds);
+    if (rc) return rc;
+    const int par = h->parity;
+    const int64_t C = ds.comb_cells[par];
+    if (cells) *cells = C;
+    if (origin)
+        for (int k = 0; k < 3; ++k) origin[k] = ds.comb_origin[par][k];
+    if (index_map) {
+        rc = export_table(h, h->comb_tables + (int64_t)par * h->g.V, index_map);
+        if (rc) return rc;
+    }
+    if (hit || total || min_h || metrics || eigen) {
+        if (cap < C) {
+            h->err = "capacity smaller than the cell count";
+            return GVOM_ECAP;
+        }
+        std::vector<CombRec> recs((size_t)(C > 0 ? C : 1));
+        if (C > 0)
+            CKH(h, hipMemcpy(recs.data(), h->comb_recs + (int64_t)par * h->ccap, (size_t)C * sizeof(CombRec),
+                             hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < C; ++i) {
+            if (hit) hit[i] = recs[i].hit;
+            if (total) total[i] = recs[i].total;
+            if (min_h) min_h[i] = recs[i].min_h;
+            if (metrics)
+                for (int j = 0; j < 10; ++j) metrics[10 * i + j] = recs[i].m[j];
+            if (eigen)
+                for (int j = 0; j < 3; ++j) eigen[3 * i + j] = recs[i].ev[j];
+        }
+    }
+    return GVOM_OK;
+}
+
+extern "C" int gvom_get_maps(gvom_handle *h, double *height, double *inferred, double *slope_x, double *slope_y,
+                             double *rough, double *delta, int32_t *pos, int32_t *neg, int32_t *vis) {
+    if (!h) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->have_last) return GVOM_NO_DATA;
+    const size_t X = (size_t)h->g.X;
+    hipStream_t st = h->stream;
+    struct {
+        void *dst;
+        const void *src;
+        size_t bytes;
+    } cp[] = {{height, h->height, X * 8}, {inferred, h->inferred, X * 8}, {slope_x, h->slope_x, X * 8},
+              {slope_y, h->slope_y, X * 8}, {rough, h->rough, X * 8},     {delta, h->delta, X * 8},
+              {pos, h->pos, X * 4},         {neg, h->neg, X * 4},         {vis, h->vis, X * 4}};
+    for (auto &c : cp)
+        if (c.dst) CKH(h, hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, st));
+    CKH(h, hipStreamSynchronize(st));
+    return GVOM_OK;
+}
+
+extern "C" int gvom_ring_state(const gvom_handle *h, int32_t *buffer_index, int32_t *last_buffer_index,
+                               int32_t *filled_mask, int32_t *have_combined) {
+    if (!h) return GVOM_EINVAL;
+    if (buffer_index) *buffer_index = h->next_slot;
+    if (last_buffer_index) *last_buffer_index = h->last_slot;
+    if (filled_mask) *filled_mask = h->filled_mask;
+    if (have_combined) *have_combined = h->have_last;
+    return GVOM_OK;
+}
+
+extern "C" int gvom_debug_voxel_map(gvom_handle *h, float *out, int64_t cap, int64_t *rows) {
+    if (!h) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->have_last) return GVOM_NO_DATA;
+    DevState ds;
+    int rc = read_state(h, &ds);
+    if (rc) return rc;
+    const int par = h->parity;
+    const int64_t C = ds.comb_cells[par];
+    if (rows) *rows = C;
+    if (!out) return GVOM_OK;
+    if (cap < C) {
+        h->err = "capacity smaller than the cell count";
+        return GVOM_ECAP;
+    }
+    if (C == 0) return GVOM_OK;
+    float *dbuf = nullptr;
+    CKH(h, dalloc(&dbuf, C * 8));
+    hipError_t e = launch_debug_voxels(h->g, h->st, h->comb_tables, h->comb_recs, h->ccap, par, dbuf, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dbuf, (size_t)C * 8 * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(dbuf);
+    CKH(h, e);
+    return GVOM_OK;
+}
+
+static int debug_height_common(gvom_handle *h, float *out7, float *out3) {
+    if (!h) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->have_last) return GVOM_NO_DATA;
+    if (!out7 && !out3) return GVOM_OK;
+    const int64_t X = h->g.X;
+    float *dbuf = nullptr;
+    const int w = out7 ? 7 : 3;
+    CKH(h, dalloc(&dbuf, X * w));
+    hipError_t e = launch_debug_height(h->g, h->st, h->parity, h->height, h->rough, h->slope_x, h->slope_y, h->delta,
+                                       out7 ? dbuf : nullptr, out3 ? dbuf : nullptr, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out7 ? (void *)out7 : (void *)out3, dbuf, (size_t)X * w * sizeof(float),
+                           hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(dbuf);
+    CKH(h, e);
+    return GVOM_OK;
+}
+
+extern "C" int gvom_debug_height_map(gvom_handle *h, float *out) { return debug_height_common(h, out, nullptr); }
+
+extern "C" int gvom_debug_inferred_height_map(gvom_handle *h, float *out) {
+    return debug_height_common(h, nullptr, out);
+}
+
+extern "C" int gvom_set_timing(gvom_handle *h, int32_t enabled) {
+    if (!h) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->timing = enabled != 0;
+    h->set_used.clear();
+    h->cur_set = -1;
+    return GVOM_OK;
+}
+
+extern "C" int gvom_stage_times(gvom_handle *h, float *ms, int32_t n) {
+    if (!h || !ms) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, hipStreamSynchronize(h->stream));
+    static const int pairs[7][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {5, 6}, {6, 7}, {7, 8}};
+    for (int k = 0; k < n; ++k) {
+        ms[k] = 0.f;
+        if (k >= 7) continue;
+        double sum = 0.0;
+        int cnt = 0;
+        for (size_t q = 0; q < h->set_used.size(); ++q) {
+            int u = h->set_used[q];
+            if (!((u >> pairs[k][0]) & 1) || !((u >> pairs[k][1]) & 1)) continue;
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, h->ev_pool[EV_PER_SET * q + pairs[k][0]],
+                                    h->ev_pool[EV_PER_SET * q + pairs[k][1]]) == hipSuccess) {
+                sum += t;
+                ++cnt;
+            }
+        }
+        ms[k] = cnt ? (float)(sum / cnt) : 0.f;
+    }
+    return GVOM_OK;
+}

@@ -1,0 +1,582 @@
+// Combine stage of the scan->map path on gfx950 (reference Gvom.combine_maps,
+// scripts/gvom.py:222-393).
+//
+// Launch sequence (stream-ordered, no host synchronisation):
+//   tables    one wavefront per z-contiguous column: for each voxel, union of the
+//             filled slots' occupancy (slot order) and sum of their free passes, then
+//             the previous combined map with its <= 10-pass carry-over rule
+//             (gvom.py:1010-1063).  Writes OCC | source-mask or the pass count, and
+//             the column's occupied count.
+//   scan      exclusive scan of the column counts -> first cell id of each column,
+//             so ids follow the z-contiguous voxel order (a legal permutation of the
+//             reference's atomic order).
+//   records   one wavefront per column: assign ids, merge the sources' records in
+//             slot order then the previous map (gvom.py:889-980), eigenvalues
+//             (1424-1487), and the column passes: height (561-576), inferred height
+//             (580-590), positive-obstacle density window (530-555).
+//   maps2d    16x16 column tiles with a 15-cell halo in LDS: slope and roughness
+//             (718-805), guessed height delta (593-714), positive/negative obstacle
+//             and visibility maps (445-555).
+//   commit    ping-pong parity of the combined buffers.
+#include <hip/hip_runtime.h>
+
+#include "gvom_kernels.h"
+
+hipError_t launch_scan_excl_u32(uint32_t *data, int64_t n, int32_t *total, hipStream_t s);
+
+namespace {
+
+constexpr uint32_t MASK31 = 0x7FFFFFFFu;
+constexpr int LAST_BIT = 30;   // source-mask bit of the previous combined map
+
+__device__ __forceinline__ float py_maxf(float a, float b) { return (b > a) ? b : a; }
+__device__ __forceinline__ float py_minf(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double py_max(double a, double b) { return (b > a) ? b : a; }
+__device__ __forceinline__ double py_min(double a, double b) { return (b < a) ? b : a; }
+
+struct SrcCol {
+    const int32_t *col;   // source column base (nullptr when the shifted column is off-map)
+    int32_t dz;
+};
+
+__device__ __forceinline__ SrcCol source_column(const int32_t *table, const double co[3], const double so[3],
+                                                int64_t x, int64_t y, const Geo &g) {
+    SrcCol r;
+    int64_t dx = (int64_t)(co[0] - so[0]), dy = (int64_t)(co[1] - so[1]);
+    r.dz = (int32_t)(co[2] - so[2]);
+    int64_t ux = x + dx, uy = y + dy;
+    if (ux < 0 || ux >= g.XY || uy < 0 || uy >= g.XY) r.col = nullptr;
+    else r.col = table + (ux * g.XY + uy) * g.Z;
+    return r;
+}
+
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_comb_tables(Geo g, DevState *__restrict__ st,
+                                                     const int32_t *__restrict__ slot_tables,
+                                                     int32_t *__restrict__ comb_tables,
+                                                     int32_t *__restrict__ col_count) {
+    const int lane = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int parity = st->parity, cur = 1 - parity;
+    const int filled = st->filled_mask, have_last = st->have_last;
+    const double *co = st->slot_origin[st->last_slot];   // combined origin = newest slot's (gvom.py:229)
+    if (c == 0 && lane == 0) {
+        st->comb_origin[cur][0] = co[0];
+        st->comb_origin[cur][1] = co[1];
+        st->comb_origin[cur][2] = co[2];
+    }
+    if (c >= g.X) return;
+    const int64_t x = c / g.XY, y = c % g.XY;
+    SrcCol lsrc;
+    lsrc.col = nullptr;
+    lsrc.dz = 0;
+    if (have_last) lsrc = source_column(comb_tables + (int64_t)parity * g.V, co, st->comb_origin[parity], x, y, g);
+    int32_t *out = comb_tables + (int64_t)cur * g.V + c * g.Z;
+    uint32_t count = 0;
+    for (int z0 = 0; z0 < g.Z; z0 += 64) {
+        const int z = z0 + lane;
+        uint32_t mask = 0, P = 0;
+        if (z < g.Z) {
+            for (int s = 0; s < g.B; ++s) {
+                if (!(filled & (1 << s))) continue;
+                SrcCol sc = source_column(slot_tables + (int64_t)s * g.V, co, st->slot_origin[s], x, y, g);
+                if (!sc.col) continue;
+                int zs = z + sc.dz;
+                if (zs < 0 || zs >= g.Z) continue;
+                int32_t v = sc.col[zs];
+                if (v < 0) mask |= 1u << s;
+                else P += (uint32_t)v;
+            }
+            if (lsrc.col) {
+                int zl = z + lsrc.dz;
+                if (zl >= 0 && zl < g.Z) {
+                    int32_t v = lsrc.col[zl];
+                    if (mask) {
+                        if (v < 0) mask |= 1u << LAST_BIT;
+                    } else if (v < 0) {
+                        if (P <= 10u) mask = 1u << LAST_BIT;   // carry-over (gvom.py:1058)
+                    } else {
+                        P += (uint32_t)v;
+                    }
+                }
+            }
+            out[z] = mask ? (int32_t)(GVOM_OCC | mask) : (int32_t)P;
+        }
+        count += __popcll(__ballot(mask != 0));
+    }
+    if (lane == 0) col_count[c] = (int32_t)count;
+}
+
+// ------------------------------------------------------------------------------------
+// Record merge, __combine_metrics (gvom.py:889-980) with Numba's typing: the product of
+// two combined (f32) entries is f32, everything touching an f64 operand is f64.
+template <typename OT>
+__device__ __forceinline__ void merge_record(float cm[10], int32_t &hit, int32_t &tot, float &mh, const OT om[10],
+                                             int32_t ohit, int32_t otot, float omh) {
+    using MT = decltype((float)0 * (OT)0);
+    MT mx = (cm[0] * cm[9] + om[0] * om[9]) / (cm[9] + om[9]);
+    MT my = (cm[1] * cm[9] + om[1] * om[9]) / (cm[9] + om[9]);
+    MT mz = (cm[2] * cm[9] + om[2] * om[9]) / (cm[9] + om[9]);
+    float c3 = (float)((cm[9] * cm[3] + om[9] * om[3] + cm[9] * (cm[0] - mx) * (cm[0] - mx) +
+                        om[9] * (om[0] - mx) * (om[0] - mx)) / (cm[9] + om[9]));
+    float c4 = (float)((cm[9] * cm[4] + om[9] * om[4] + cm[9] * (cm[0] - mx) * (cm[1] - my) +
+                        om[9] * (om[0] - mx) * (om[1] - my)) / (cm[9] + om[9]));
+    float c5 = (float)((cm[9] * cm[5] + om[9] * om[5] + cm[9] * (cm[0] - mx) * (cm[2] - mz) +
+                        om[9] * (om[0] - mx) * (om[2] - mz)) / (cm[9] + om[9]));
+    float c6 = (float)((cm[9] * cm[6] + om[9] * om[6] + cm[9] * (cm[1] - my) * (cm[1] - my) +
+                        om[9] * (om[1] - my) * (om[1] - my)) / (cm[9] + om[9]));
+    float c7 = (float)((cm[9] * cm[7] + om[9] * om[7] + cm[9] * (cm[1] - my) * (cm[2] - mz) +
+                        om[9] * (om[1] - my) * (om[2] - mz)) / (cm[9] + om[9]));
+    float c8 = (float)((cm[9] * cm[8] + om[9] * om[8] + cm[9] * (cm[2] - mz) * (cm[2] - mz) +
+                        om[9] * (om[2] - mz) * (om[2] - mz)) / (cm[9] + om[9]));
+    cm[3] = c3; cm[4] = c4; cm[5] = c5; cm[6] = c6; cm[7] = c7; cm[8] = c8;
+    cm[0] = (float)mx; cm[1] = (float)my; cm[2] = (float)mz;
+    cm[9] = (float)(cm[9] + om[9]);
+    hit = (int32_t)((uint32_t)hit + (uint32_t)ohit);
+    tot = (int32_t)((uint32_t)tot + (uint32_t)otot);
+    mh = py_minf(mh, omh);
+}
+
+// __calculate_eigenvalues (gvom.py:1424-1487): f32 loads, f64 arithmetic, f32 stores,
+// descending order; the middle one is 3q minus the two stored extremes.
+__device__ __forceinline__ void eigen3(const float m[10], float ev[3]) {
+    const double PI = 3.141592653589793;
+    float xx = m[3], xy = m[4], xz = m[5], yy = m[6], yz = m[7], zz = m[8];
+    float p1 = xy * xy + xz * xz + yz * yz;
+    double q = (double)(xx + yy + zz) / 3.0;
+    if (p1 == 0) {
+        ev[0] = py_maxf(xx, py_maxf(yy, zz));
+        ev[2] = py_minf(xx, py_minf(yy, zz));
+        ev[1] = (float)(3.0 * q - (double)ev[0] - (double)ev[2]);
+    } else {
+        double p2 = ((double)xx - q) * ((double)xx - q) + ((double)yy - q) * ((double)yy - q) +
+                    ((double)zz - q) * ((double)zz - q) + 2.0 * (double)p1;
+        double p = sqrt(p2 / 6.0);
+        double B0 = ((double)xx - q) / p, B1 = (double)xy / p, B2 = (double)xz / p;
+        double B3 = ((double)yy - q) / p, B4 = (double)yz / p, B5 = ((double)zz - q) / p;
+        double r = B0 * (B3 * B5 - B4 * B4) - B1 * (B1 * B5 - B4 * B2) + B2 * (B1 * B4 - B3 * B2);
+        r = r / 2.0;
+        double phi;
+        if (r <= -1) phi = PI / 3.0;
+        else if (r >= 1) phi = 0.0;
+        else phi = acos(r) / 3.0;
+        ev[0] = (float)(q + 2.0 * p * cos(phi));
+        ev[2] = (float)(q + 2.0 * p * cos(phi + (2.0 * PI / 3.0)));
+        ev[1] = (float)(3.0 * q - (double)ev[0] - (double)ev[2]);
+    }
+}
+
+constexpr int MAX_ZCH = 4;   // z_size <= 256
+
+__global__ __launch_bounds__(256) void k_comb_records(Geo g, const DevState *__restrict__ st,
+                                                      const int32_t *__restrict__ slot_tables,
+                                                      const SlotRec *__restrict__ slot_recs, int64_t ncap,
+                                                      int32_t *__restrict__ comb_tables,
+                                                      CombRec *__restrict__ comb_recs, int64_t ccap,
+                                                      const int32_t *__restrict__ col_base,
+                                                      double *__restrict__ height, double *__restrict__ inferred,
+                                                      int32_t *__restrict__ density) {
+    const int lane = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (c >= g.X) return;
+    const int parity = st->parity, cur = 1 - parity;
+    const double *co = st->comb_origin[cur];
+    const int64_t x = c / g.XY, y = c % g.XY;
+    int32_t *ctab = comb_tables + (int64_t)cur * g.V + c * g.Z;
+    CombRec *crec = comb_recs + (int64_t)cur * ccap;
+    const CombRec *lrec = comb_recs + (int64_t)parity * ccap;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t run = (uint32_t)col_base[c];
+    int first_occ = -1, first_free = -1;
+    float mh_first = 1.0f;
+    int32_t chit[MAX_ZCH], ctot[MAX_ZCH];
+    bool cocc[MAX_ZCH];
+#pragma unroll
+    for (int k = 0; k < MAX_ZCH; ++k) {
+        const int z = k * 64 + lane;
+        chit[k] = 0;
+        ctot[k] = 0;
+        cocc[k] = false;
+        if (k * 64 >= g.Z) continue;
+        int32_t v = (z < g.Z) ? ctab[z] : 0;
+        const bool occ = v < 0;
+        const uint64_t ob = __ballot(occ);
+        const uint64_t fb = __ballot(!occ && v >= 1);
+        float mh = 1.0f;
+        if (occ) {
+            const uint32_t id = run + __popcll(ob & lt);
+            const uint32_t mask = (uint32_t)v & MASK31;
+            float cm[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            int32_t hit = 0, tot = 0;
+            // sources in slot order (gvom.py:280-292), then the previous map (295-298)
+            for (int s = 0; s < g.B; ++s) {
+                if (!(mask & (1u << s))) continue;
+                SrcCol sc = source_column(slot_tables + (int64_t)s * g.V, co, st->slot_origin[s], x, y, g);
+                const uint32_t sid = (uint32_t)sc.col[z + sc.dz] & MASK31;
+                const SlotRec &r = slot_recs[(int64_t)s * ncap + sid];
+                merge_record<double>(cm, hit, tot, mh, r.m, r.hit, r.total, r.min_h);
+            }
+            if (mask & (1u << LAST_BIT)) {
+                SrcCol sc = source_column(comb_tables + (int64_t)parity * g.V, co, st->comb_origin[parity], x, y, g);
+                const uint32_t lid = (uint32_t)sc.col[z + sc.dz] & MASK31;
+                const CombRec &r = lrec[lid];
+                merge_record<float>(cm, hit, tot, mh, r.m, r.hit, r.total, r.min_h);
+            }
+            CombRec out;
+            out.hit = hit;
+            out.total = tot;
+            out.min_h = mh;
+#pragma unroll
+            for (int j = 0; j < 10; ++j) out.m[j] = cm[j];
+            eigen3(cm, out.ev);
+            crec[id] = out;
+            ctab[z] = (int32_t)(GVOM_OCC | id);
+            chit[k] = hit;
+            ctot[k] = tot;
+            cocc[k] = true;
+        }
+        run += __popcll(ob);
+        if (first_occ < 0 && ob) {
+            const int l0 = __builtin_ctzll(ob);
+            first_occ = k * 64 + l0;
+            mh_first = __shfl(mh, l0);
+        }
+        if (first_free < 0 && fb) first_free = k * 64 + __builtin_ctzll(fb);
+    }
+    // __make_height_map (gvom.py:561-576): robot-radius default from the cell corner and
+    // the latest ego, overridden by the lowest occupied voxel.
+    double h = -1000.0;
+    {
+        const double *ego = st->latest_ego;
+        double xp = ((co[0] + (double)x) * g.xy_res) - ego[0];
+        double yp = ((co[1] + (double)y) * g.xy_res) - ego[1];
+        if (xp * xp + yp * yp <= g.robot_r * g.robot_r) h = ego[2] - g.g2l;
+        if (first_occ >= 0) h = (((double)mh_first + (double)first_occ) + co[2]) * g.z_res;
+    }
+    // positive-obstacle density window (gvom.py:530-555)
+    int32_t dens = 0;
+    {
+        double flo = floor(((h + g.pos_thr) / g.z_res) - co[2]);
+        double fhi = floor(((h + g.robot_h) / g.z_res) - co[2]);
+        if (flo + 1 >= 0 && flo + 1 < g.Z && fhi >= 0 && fhi < g.Z) {
+            const int lo = (int)flo + 1, hi = (int)fhi;
+            int64_t sh = 0, st_ = 0;
+#pragma unroll
+            for (int k = 0; k < MAX_ZCH; ++k) {
+                const int z = k * 64 + lane;
+                if (cocc[k] && z >= lo && z <= hi && chit[k] > 10) {
+                    sh += chit[k];
+                    st_ += ctot[k];
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                sh += __shfl_xor(sh, off);
+                st_ += __shfl_xor(st_, off);
+            }
+            double dsum = (double)sh, nsum = (double)st_;
+            if (nsum > 0.0) dsum /= nsum;
+            dens = (int32_t)(dsum * 100.0);
+        }
+    }
+    if (lane == 0) {
+        height[c] = h;
+        inferred[c] = first_free >= 0 ? ((double)first_free + co[2]) * g.z_res : -1000.0;
+        density[c] = dens;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+constexpr int TILE = 16;
+constexpr int HALO = 15;
+constexpr int LT = TILE + 2 * HALO;   // 46
+
+__global__ __launch_bounds__(256) void k_maps2d(Geo g, const double *__restrict__ height,
+                                                const double *__restrict__ inferred,
+                                                const int32_t *__restrict__ density,
+                                                double *__restrict__ slope_x, double *__restrict__ slope_y,
+                                                double *__restrict__ rough, double *__restrict__ delta,
+                                                int32_t *__restrict__ pos, int32_t *__restrict__ neg,
+                                                int32_t *__restrict__ vis) {
+    __shared__ double H[LT][LT];
+    const int XY = g.XY;
+    const int bx = blockIdx.y * TILE, by = blockIdx.x * TILE;   // tile origin (x rows, y fastest)
+    for (int k = threadIdx.x; k < LT * LT; k += 256) {
+        int lx = k / LT, ly = k % LT;
+        int gx = bx - HALO + lx, gy = by - HALO + ly;
+        H[lx][ly] = (gx >= 0 && gx < XY && gy >= 0 && gy < XY) ? height[(int64_t)gx * XY + gy] : -1000.0;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x / TILE, ty = threadIdx.x % TILE;
+    const int x0 = bx + tx, y0 = by + ty;
+    if (x0 >= XY || y0 >= XY) return;
+    const int64_t c0 = (int64_t)x0 * XY + y0;
+#define HAT(gx, gy) H[(gx) - bx + HALO][(gy) - by + HALO]
+
+    // __calculate_slope (gvom.py:718-805)
+    double sx = 0.0, sy = 0.0, rg = -1.0;
+    {
+        const int xlo = x0 - 1 < 0 ? 0 : x0 - 1, xhi = x0 + 2 > XY ? XY : x0 + 2;
+        const int ylo = y0 - 1 < 0 ? 0 : y0 - 1, yhi = y0 + 2 > XY ? XY : y0 + 2;
+        int ngood = 0;
+        for (int xx = xlo; xx < xhi; ++xx)
+            for (int yy = ylo; yy < yhi; ++yy)
+                if (HAT(xx, yy) > -1000) ++ngood;
+        if (ngood >= 3) {
+            // pts in window order (x outer, y inner) over observed cells, as gvom.py:736-749
+            double mx = 0.0, my = 0.0, mz = 0.0;
+            int i = 0;
+            for (int xx = xlo; xx < xhi; ++xx)
+                for (int yy = ylo; yy < yhi; ++yy) {
+                    double hh = HAT(xx, yy);
+                    if (hh > -1000) {
+                        mx += (double)xx * g.xy_res;
+                        my += (double)yy * g.xy_res;
+                        mz += hh;
+                        ++i;
+                    }
+                }
+            mx /= (double)i;
+            my /= (double)i;
+            mz /= (double)i;
+            double cxx = 0.0, cxy = 0.0, cxz = 0.0, cyy = 0.0, cyz = 0.0;
+            for (int xx = xlo; xx < xhi; ++xx)
+                for (int yy = ylo; yy < yhi; ++yy) {
+                    double hh = HAT(xx, yy);
+                    if (hh > -1000) {
+                        double px = (double)xx * g.xy_res, py = (double)yy * g.xy_res;
+                        cxx += (px - mx) * (px - mx);
+                        cxy += (px - mx) * (py - my);
+                        cxz += (px - mx) * (hh - mz);
+                        cyy += (py - my) * (py - my);
+                        cyz += (py - my) * (hh - mz);
+                    }
+                }
+            double det = cxx * cyy - cxy * cxy;
+            if (det != 0.0) {
+                double a0 = (cyy * cxz - cxy * cyz) / det;
+                double a1 = (cxx * cyz - cxy * cxz) / det;
+                double m = sqrt(a0 * a0 + a1 * a1 + 1.0);
+                a0 /= m;
+                a1 /= m;
+                double err = 0.0;
+                for (int xx = xlo; xx < xhi; ++xx)
+                    for (int yy = ylo; yy < yhi; ++yy) {
+                        double hh = HAT(xx, yy);
+                        if (hh > -1000) {
+                            double px = (double)xx * g.xy_res, py = (double)yy * g.xy_res;
+                            double e = (hh - mz) - (a0 * (px - mx) + a1 * (py - my));
+                            err += e * e;
+                        }
+                    }
+                err /= (double)ngood;
+                if (err > 0) err = log(err);
+                rg = err;
+                sx = atan2(a0, 1.0 / m);
+                sy = atan2(a1, 1.0 / m);
+            }
+        }
+    }
+
+    // __guess_height (gvom.py:593-714), its loop condition and the y_nh gating as written
+    double dh_out = 0.0;
+    const double h0 = HAT(x0, y0);
+    const double inf0 = inferred[c0];
+    if (!(h0 > -1000) && !(inf0 == -1000.0)) {
+        bool xpd = false, xnd = false, ypd = false, ynd = false;
+        int xp = x0, xn = x0, yp = y0, yn = y0;
+        double xph = -1000, xnh = -1000, yph = -1000, ynh = -1000;
+        int i = 0;
+        while (i < 15 && !(xnd && xnd && ypd && ynd)) {
+            xp += 1; xn -= 1; yp += 1; yn -= 1;
+            i += 1;
+            if (!xpd) {
+                if (xp < XY) {
+                    for (int dy = -i; dy < i; ++dy) {
+                        if (y0 + dy >= XY || y0 + dy < 0) continue;
+                        double v = HAT(xp, y0 + dy);
+                        if (v > -1000) { xph = v; xpd = true; break; }
+                    }
+                } else xpd = true;
+            }
+            if (!xnd) {
+                if (xn >= 0) {
+                    for (int dy = -i + 1; dy < i + 1; ++dy) {
+                        if (y0 + dy >= XY || y0 + dy < 0) continue;
+                        double v = HAT(xn, y0 + dy);
+                        if (v > -1000) { xnh = v; xnd = true; break; }
+                    }
+                } else xnd = true;
+            }
+            if (!ypd) {
+                if (yp < XY) {
+                    for (int dx = -i + 1; dx < i + 1; ++dx) {
+                        if (x0 + dx >= XY || x0 + dx < 0) continue;
+                        double v = HAT(x0 + dx, yp);
+                        if (v > -1000) { yph = v; ypd = true; break; }
+                    }
+                } else ypd = true;
+            }
+            if (!ynd) {
+                if (yn >= 0) {
+                    for (int dx = -i; dx < i; ++dx) {
+                        if (x0 + dx >= XY || x0 + dx < 0) continue;
+                        double v = HAT(x0 + dx, yn);
+                        if (v > -1000) { ynh = v; ynd = true; break; }
+                    }
+                } else ynd = true;
+            }
+        }
+        double mn = 1000.0, mxh = inf0;
+        if (xph > -1000) { mn = py_min(xph, mn); mxh = py_max(xph, mxh); }
+        if (xnh > -1000) { mn = py_min(xnh, mn); mxh = py_max(xnh, mxh); }
+        if (yph > -1000) { mn = py_min(yph, mn); mxh = py_max(yph, mxh); }
+        if (xnh > -1000) { mn = py_min(ynh, mn); mxh = py_max(ynh, mxh); }
+        double dh = mxh - mn;
+        if (dh > 0) dh_out = dh;
+    }
+#undef HAT
+    slope_x[c0] = sx;
+    slope_y[c0] = sy;
+    rough[c0] = rg;
+    delta[c0] = dh_out;
+    // __make_positive_obstacle_map slope test (gvom.py:524-526) over the density from the
+    // column pass; __make_negative_obstacle_map (506-512); __make_visability_map (445-452)
+    pos[c0] = (sqrt(sx * sx + sy * sy) >= g.slope_thr) ? 100 : density[c0];
+    neg[c0] = (dh_out > g.neg_thr) ? 100 : 0;
+    vis[c0] = (h0 > -1000) ? 1 : 0;
+}
+
+__global__ void k_commit_combine(DevState *st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int cur = 1 - st->parity;
+    st->comb_cells[cur] = st->comb_total;
+    st->parity = cur;
+    st->have_last = 1;
+}
+
+// ------------------------------------------------------------------------------------
+// Export: internal table -> reference layout (x fastest) and encoding.
+__global__ __launch_bounds__(256) void k_export_table(Geo g, const int32_t *__restrict__ table,
+                                                      int32_t *__restrict__ out) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.V) return;
+    int64_t x = v % g.XY, y = (v / g.XY) % g.XY, z = v / g.X;
+    int32_t t = table[(x * g.XY + y) * g.Z + z];
+    out[v] = t < 0 ? (int32_t)((uint32_t)t & MASK31) : (int32_t)(-1 - (int64_t)t);
+}
+
+// __make_voxel_pointcloud (gvom.py:482-503), rows in cell-id order.
+__global__ __launch_bounds__(256) void k_debug_voxels(Geo g, const DevState *__restrict__ st, int par,
+                                                      const int32_t *__restrict__ table,
+                                                      const CombRec *__restrict__ recs,
+                                                      float *__restrict__ out) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.V) return;
+    int32_t t = table[v];
+    if (t >= 0) return;
+    uint32_t id = (uint32_t)t & MASK31;
+    int64_t z = v % g.Z, c = v / g.Z, y = c % g.XY, x = c / g.XY;
+    const double *o = st->comb_origin[par];
+    const CombRec &r = recs[id];
+    float *row = out + (int64_t)id * 8;
+    row[0] = (float)(((double)x + o[0]) * g.xy_res);
+    row[1] = (float)(((double)y + o[1]) * g.xy_res);
+    row[2] = (float)(((double)z + o[2]) * g.z_res);
+    row[3] = (float)((double)r.hit / (double)r.total);
+    row[4] = (float)r.hit;
+    row[5] = r.ev[0] - r.ev[1];
+    row[6] = r.ev[1] - r.ev[2];
+    row[7] = r.ev[2];
+}
+
+// __make_height_map_pointcloud (gvom.py:456-468) and __make_infered_height_map_pointcloud
+// (471-479, fed with the guessed height delta, 439-440); row index x + y*xy.
+__global__ __launch_bounds__(256) void k_debug_height(Geo g, const DevState *__restrict__ st, int par,
+                                                      const double *__restrict__ height,
+                                                      const double *__restrict__ rough,
+                                                      const double *__restrict__ sx,
+                                                      const double *__restrict__ sy,
+                                                      const double *__restrict__ delta,
+                                                      float *__restrict__ out7, float *__restrict__ out3) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.X) return;
+    int64_t x = c / g.XY, y = c % g.XY;
+    int64_t row = x + y * g.XY;
+    const double *o = st->comb_origin[par];
+    float fx = (float)(((double)x + o[0]) * g.xy_res);
+    float fy = (float)(((double)y + o[1]) * g.xy_res);
+    if (out7) {
+        float *r = out7 + row * 7;
+        r[0] = fx;
+        r[1] = fy;
+        r[2] = (float)(height[c] - g.z_res);
+        r[3] = (float)rough[c];
+        r[4] = (float)sx[c];
+        r[5] = (float)sy[c];
+        r[6] = (float)sqrt(sx[c] * sx[c] + sy[c] * sy[c]);
+    }
+    if (out3) {
+        float *r = out3 + row * 3;
+        r[0] = fx;
+        r[1] = fy;
+        r[2] = (float)(delta[c] - g.z_res);
+    }
+}
+
+inline unsigned nblocks(int64_t n, int per) { return (unsigned)((n + per - 1) / per > 0 ? (n + per - 1) / per : 1); }
+
+}  // namespace
+
+hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_tables, int32_t *comb_tables,
+                              int32_t *col_count, hipStream_t s) {
+    k_comb_tables<<<nblocks(g.X, 4), 256, 0, s>>>(g, st, slot_tables, comb_tables, col_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_columns(int32_t *col_count, int64_t X, DevState *st, hipStream_t s) {
+    // reads the parity on device: the total goes to comb_cells[1 - parity]; the host mirror
+    // passes the right slot through the pointer it chooses.
+    (void)st;
+    return launch_scan_excl_u32((uint32_t *)col_count, X, nullptr, s);
+}
+
+hipError_t launch_comb_records(const Geo &g, DevState *st, const int32_t *slot_tables,
+                               const SlotRec *slot_recs, int64_t ncap, int32_t *comb_tables,
+                               CombRec *comb_recs, int64_t ccap, const int32_t *col_base, double *height,
+                               double *inferred, int32_t *density, hipStream_t s) {
+    k_comb_records<<<nblocks(g.X, 4), 256, 0, s>>>(g, st, slot_tables, slot_recs, ncap, comb_tables, comb_recs,
+                                                   ccap, col_base, height, inferred, density);
+    return hipGetLastError();
+}
+
+hipError_t launch_maps2d(const Geo &g, const double *height, const double *inferred, const int32_t *density,
+                         double *slope_x, double *slope_y, double *rough, double *delta, int32_t *pos,
+                         int32_t *neg, int32_t *vis, hipStream_t s) {
+    dim3 grid((unsigned)((g.XY + TILE - 1) / TILE), (unsigned)((g.XY + TILE - 1) / TILE));
+    k_maps2d<<<grid, 256, 0, s>>>(g, height, inferred, density, slope_x, slope_y, rough, delta, pos, neg, vis);
+    return hipGetLastError();
+}
+
+hipError_t launch_commit_combine(DevState *st, hipStream_t s) {
+    k_commit_combine<<<1, 64, 0, s>>>(st);
+    return hipGetLastError();
+}
+
+hipError_t launch_export_table(const Geo &g, const int32_t *table, int32_t *out_ref, hipStream_t s) {
+    k_export_table<<<nblocks(g.V, 256), 256, 0, s>>>(g, table, out_ref);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_voxels(const Geo &g, const DevState *st, const int32_t *comb_tables,
+                               const CombRec *comb_recs, int64_t ccap, int parity, float *out, hipStream_t s) {
+    k_debug_voxels<<<nblocks(g.V, 256), 256, 0, s>>>(g, st, parity, comb_tables + (int64_t)parity * g.V,
+                                                     comb_recs + (int64_t)parity * ccap, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_height(const Geo &g, const DevState *st, int parity, const double *height,
+                               const double *rough, const double *sx, const double *sy, const double *delta,
+                               float *out7, float *out3, hipStream_t s) {
+    k_debug_height<<<nblocks(g.X, 256), 256, 0, s>>>(g, st, parity, height, rough, sx, sy, delta, out7, out3);
+    return hipGetLastError();
+}

@@ -1,0 +1,58 @@
+// Host-side launchers of the gfx950 kernels (implemented next to the kernels).
+#pragma once
+
+#include "gvom_internal.h"
+
+// ---- fills --------------------------------------------------------------------------
+hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s);
+
+// ---- scan stage (gvom_scan.hip) ------------------------------------------------------
+// Transform, validity, own-voxel sort key per point (reference gvom.py:1122-1163).
+hipError_t launch_prepare(int dtype, const void *in, void *world, uint32_t *keys, uint32_t *vals,
+                          uint8_t *valid, const ScanDesc *sd, int S, int64_t NT, const Geo &g,
+                          DevState *st, hipStream_t s);
+// Exact-rule ray march counting passes into the slot tables (gvom.py:1174-1231).
+hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, const ScanDesc *sd,
+                          int S, int64_t NT, const Geo &g, const DevState *st, int32_t *slot_tables,
+                          int block, int rays_per_thread, hipStream_t s);
+// Stable LSD radix sort of (key, val) on the low `bits` key bits.  On return *kout/*vout
+// point at whichever buffer pair holds the result.
+hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits,
+                      uint32_t *hist, uint32_t **kout, uint32_t **vout, hipStream_t s);
+int64_t radix_hist_entries(int64_t n);
+// Segments of equal keys in the sorted array, per-scan bounds and slot cell counts.
+hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start, uint32_t *seg_key,
+                           uint32_t *blk, int S, const Geo &g, DevState *st, hipStream_t s);
+// Per-segment moments + slot table occupancy marks + hit/total/min-height records.
+hipError_t launch_moments(int dtype, const void *world, const uint32_t *svals, const int32_t *seg_start,
+                          const uint32_t *seg_key, int64_t seg_cap, const ScanDesc *sd, int S,
+                          const Geo &g, DevState *st, int32_t *slot_tables, SlotRec *slot_recs,
+                          int64_t ncap, Moment *mom, hipStream_t s);
+// Neighbourhood mean/covariance per occupied voxel (gvom.py:1250-1387).
+hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t seg_cap, int S,
+                          const Geo &g, const DevState *st, const int32_t *slot_tables,
+                          SlotRec *slot_recs, int64_t ncap, hipStream_t s);
+hipError_t launch_commit_process(DevState *st, int S, int B, hipStream_t s);
+
+// ---- combine stage (gvom_combine.hip) ----------------------------------------------------
+hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_tables,
+                              int32_t *comb_tables, int32_t *col_count, hipStream_t s);
+hipError_t launch_scan_columns(int32_t *col_count, int64_t X, DevState *st, hipStream_t s);
+hipError_t launch_comb_records(const Geo &g, DevState *st, const int32_t *slot_tables,
+                               const SlotRec *slot_recs, int64_t ncap, int32_t *comb_tables,
+                               CombRec *comb_recs, int64_t ccap, const int32_t *col_base,
+                               double *height, double *inferred, int32_t *density, hipStream_t s);
+hipError_t launch_maps2d(const Geo &g, const double *height, const double *inferred,
+                         const int32_t *density, double *slope_x, double *slope_y, double *rough,
+                         double *delta, int32_t *pos, int32_t *neg, int32_t *vis, hipStream_t s);
+hipError_t launch_commit_combine(DevState *st, hipStream_t s);
+
+// ---- export (gvom_combine.hip) -----------------------------------------------------------
+// Internal z-contiguous table -> reference layout and encoding.
+hipError_t launch_export_table(const Geo &g, const int32_t *table, int32_t *out_ref, hipStream_t s);
+hipError_t launch_debug_voxels(const Geo &g, const DevState *st, const int32_t *comb_tables,
+                               const CombRec *comb_recs, int64_t ccap, int parity, float *out,
+                               hipStream_t s);
+hipError_t launch_debug_height(const Geo &g, const DevState *st, int parity, const double *height,
+                               const double *rough, const double *sx, const double *sy,
+                               const double *delta, float *out7, float *out3, hipStream_t s);

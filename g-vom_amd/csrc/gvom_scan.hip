@@ -1,0 +1,759 @@
+// Scan stage of the scan->map path on gfx950: one Process_pointcloud call (or a batch
+// of them) turns points into a ring slot (reference scripts/gvom.py:105-220).
+//
+// Launch sequence for a batch of S scans over NT points (all stream-ordered, no host
+// synchronisation):
+//   fill      slot tables of the S target slots := 0
+//   prepare   transform, validity, own-voxel sort key per point
+//   raycast   one thread per ray, exact stepping rule, run-aggregated atomics
+//   sort      stable LSD radix sort of (key, point index)
+//   segments  runs of equal keys -> segment starts, per-scan bounds
+//   moments   per segment: n, sum r, sum r r^T, min local z; marks occupancy in the
+//             slot table and writes the hit/total/min-height record
+//   metrics   per occupied voxel: mean and population covariance over the
+//             neighbourhood, assembled from neighbouring segments' moments
+//   commit    ring bookkeeping
+//
+// Numerics follow the reference's Numba-CUDA typing; this file is compiled with
+// -ffp-contract=off so that no multiply-add is fused.
+#include <hip/hip_runtime.h>
+
+#include "gvom_kernels.h"
+
+namespace {
+
+__device__ __forceinline__ float py_maxf(float a, float b) { return (b > a) ? b : a; }
+__device__ __forceinline__ float py_minf(float a, float b) { return (b < a) ? b : a; }
+
+// Scan index of global point i (offsets ascending, S <= 32).
+__device__ __forceinline__ int scan_of(const ScanDesc *__restrict__ sd, int S, int64_t i) {
+    int lo = 0, hi = S - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (sd[mid].offset <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// gvom.py:138-141: origin = floor(ego / res - size / 2), f64.
+__device__ __forceinline__ void scan_origin(const ScanDesc &d, const Geo &g, double o[3]) {
+    o[0] = floor((d.ego[0] / g.xy_res) - (double)g.XY / 2.0);
+    o[1] = floor((d.ego[1] / g.xy_res) - (double)g.XY / 2.0);
+    o[2] = floor((d.ego[2] / g.z_res) - (double)g.Z / 2.0);
+}
+
+__device__ __forceinline__ int64_t lin_of(int64_t x, int64_t y, int64_t z, const Geo &g) {
+    return (x * g.XY + y) * g.Z + z;
+}
+
+// __transform_pointcloud (gvom.py:1122-1138): arithmetic in the promoted type of the
+// point and matrix dtypes, stored back in the point dtype.
+template <typename P>
+__device__ __forceinline__ void transform_point(const ScanDesc &d, P &x, P &y, P &z) {
+    if (d.t_mode == GVOM_NO_TRANSFORM) return;
+    if (sizeof(P) == 4 && d.t_mode == GVOM_F32) {
+        float t[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) t[k] = (float)d.T[k];
+        float fx = (float)x, fy = (float)y, fz = (float)z;
+        float r0 = fx * t[0] + fy * t[1] + fz * t[2] + t[3];
+        float r1 = fx * t[4] + fy * t[5] + fz * t[6] + t[7];
+        float r2 = fx * t[8] + fy * t[9] + fz * t[10] + t[11];
+        x = (P)r0; y = (P)r1; z = (P)r2;
+    } else {
+        double dx = (double)x, dy = (double)y, dz = (double)z;
+        const double *t = d.T;
+        double r0 = dx * t[0] + dy * t[1] + dz * t[2] + t[3];
+        double r1 = dx * t[4] + dy * t[5] + dz * t[6] + t[7];
+        double r2 = dx * t[8] + dy * t[9] + dz * t[10] + t[11];
+        x = (P)r0; y = (P)r1; z = (P)r2;
+    }
+}
+
+// Minimum-distance filter in the point dtype (gvom.py:1145-1149), plus the domain rule:
+// non-finite points and points with |p / res| >= 2^60 are dropped.
+template <typename P>
+__device__ __forceinline__ bool point_valid(P x, P y, P z, const Geo &g) {
+    double dx = (double)x, dy = (double)y, dz = (double)z;
+    if (!isfinite(dx) || !isfinite(dy) || !isfinite(dz)) return false;
+    const double lim = 1152921504606846976.0;  // 2^60
+    if (fabs(dx / g.xy_res) >= lim || fabs(dy / g.xy_res) >= lim || fabs(dz / g.z_res) >= lim) return false;
+    P d2 = x * x + y * y + z * z;
+    return !((double)d2 < g.min_d2);
+}
+
+// ------------------------------------------------------------------------------------
+template <typename P>
+__global__ __launch_bounds__(256) void k_prepare(const P *__restrict__ in, P *__restrict__ world,
+                                                 uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                 uint8_t *__restrict__ valid,
+                                                 const ScanDesc *__restrict__ sd, int S, int64_t NT,
+                                                 Geo g, DevState *__restrict__ st) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        // Slot origins for combine (gvom.py:208) and the ego the height map uses (gvom.py:110-112).
+        int next = st->next_slot;
+        for (int s = 0; s < S; ++s) {
+            double o[3];
+            scan_origin(sd[s], g, o);
+            int slot = (next + s) % g.B;
+            st->slot_origin[slot][0] = o[0];
+            st->slot_origin[slot][1] = o[1];
+            st->slot_origin[slot][2] = o[2];
+        }
+        st->latest_ego[0] = sd[S - 1].ego[0];
+        st->latest_ego[1] = sd[S - 1].ego[1];
+        st->latest_ego[2] = sd[S - 1].ego[2];
+    }
+    if (i >= NT) return;
+    int s = scan_of(sd, S, i);
+    const ScanDesc &d = sd[s];
+    P x = in[3 * i], y = in[3 * i + 1], z = in[3 * i + 2];
+    transform_point<P>(d, x, y, z);
+    world[3 * i] = x;
+    world[3 * i + 1] = y;
+    world[3 * i + 2] = z;
+    bool ok = point_valid<P>(x, y, z, g);
+    valid[i] = ok ? 1 : 0;
+    uint32_t key = GVOM_NOKEY;
+    if (ok) {
+        double o[3];
+        scan_origin(d, g, o);
+        double fx = floor(((double)x / g.xy_res) - o[0]);
+        double fy = floor(((double)y / g.xy_res) - o[1]);
+        double fz = floor(((double)z / g.z_res) - o[2]);
+        uint32_t sk = (uint32_t)s << g.VB;
+        if (fx >= 0 && fx < g.XY && fy >= 0 && fy < g.XY && fz >= 0 && fz < g.Z) {
+            key = sk | (uint32_t)lin_of((int64_t)fx, (int64_t)fy, (int64_t)fz, g);
+        } else if (fx >= -g.ex && fx < g.XY + g.ex && fy >= -g.ex && fy < g.XY + g.ex &&
+                   fz >= -g.ez && fz < g.Z + g.ez) {
+            // own voxel in the halo: contributes to in-map neighbours (gvom.py:1266-1291)
+            int64_t kx = (int64_t)fx + g.ex, ky = (int64_t)fy + g.ex, kz = (int64_t)fz + g.ez;
+            key = sk | (1u << (g.VB - 1)) | (uint32_t)((kx * g.XYp + ky) * g.Zp + kz);
+        }
+    }
+    keys[i] = key;
+    vals[i] = (uint32_t)i;
+}
+
+// ------------------------------------------------------------------------------------
+// Ray casting (__point_2_map, gvom.py:1174-1231).  State pt/end/slope in f32, length in
+// f64, dominant axis = argmax |slope| with ties to y over x and z over both, each step
+// adds +-1 to the dominant coordinate and slope/|slope_dom| to the others, the ray is
+// abandoned at its first out-of-map step.
+//
+// The loop is kept convergent across the wavefront (dead lanes carry a unique
+// sentinel key) so that at every step, lanes whose rays sit in the same voxel can be
+// merged: consecutive points of a spinning-lidar scan have nearly equal directions,
+// so near the sensor long runs of adjacent lanes hit the same voxel.  The head of
+// each run of equal keys issues one atomic add of the run length.
+//
+// Termination: the dominant coordinate moves by exactly 1 voxel per step while
+// |pt| < 2^23 (ego is restricted to |ego/res| < 2^22), and the ego starts inside the
+// map, so a ray leaves the map within max(xy, z) + 1 steps; maxit bounds the loop at
+// that number plus one.
+template <typename P>
+__global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
+                                                 const uint8_t *__restrict__ valid,
+                                                 const ScanDesc *__restrict__ sd, int S, int64_t NT,
+                                                 Geo g, const DevState *__restrict__ st,
+                                                 int32_t *__restrict__ tables, int maxit, int rpt) {
+    const int lane = threadIdx.x & 63;
+    const int next = st->next_slot;
+    for (int rep = 0; rep < rpt; ++rep) {
+        int64_t i = ((int64_t)blockIdx.x * rpt + rep) * blockDim.x + threadIdx.x;
+        bool alive = false;
+        float p0 = 0.f, p1 = 0.f, p2 = 0.f, inc0 = 0.f, inc1 = 0.f, inc2 = 0.f;
+        double length = 0.0, rl1 = 0.0, dlen = 0.0, o0 = 0.0, o1 = 0.0, o2 = 0.0;
+        uint32_t base = 0;
+        if (i < NT && valid[i]) {
+            int s = scan_of(sd, S, i);
+            const ScanDesc &d = sd[s];
+            double o[3];
+            scan_origin(d, g, o);
+            o0 = o[0]; o1 = o[1]; o2 = o[2];
+            base = (uint32_t)(((next + s) % g.B) * g.V);
+            float pt0 = (float)(d.ego[0] / g.xy_res);
+            float pt1 = (float)(d.ego[1] / g.xy_res);
+            float pt2 = (float)(d.ego[2] / g.z_res);
+            float e0 = (float)((double)world[3 * i] / g.xy_res);
+            float e1 = (float)((double)world[3 * i + 1] / g.xy_res);
+            float e2 = (float)((double)world[3 * i + 2] / g.z_res);
+            float s0 = e0 - pt0, s1 = e1 - pt1, s2 = e2 - pt2;
+            float rl = sqrtf(s0 * s0 + s1 * s1 + s2 * s2);
+            s0 = s0 / rl;
+            s1 = s1 / rl;
+            s2 = s2 / rl;
+            float a0 = fabsf(s0), a1 = fabsf(s1), a2 = fabsf(s2);
+            float smax = py_maxf(a0, py_maxf(a1, a2));
+            int si = 0;
+            if (smax == a1) si = 1;
+            if (smax == a2) si = 2;
+            float sd_ = si == 0 ? s0 : (si == 1 ? s1 : s2);
+            float ad = fabsf(sd_);
+            float dir = sd_ / ad;
+            // per-axis increments: +-1 on the dominant axis, slope/|slope_dom| elsewhere
+            inc0 = si == 0 ? dir : s0 / ad;
+            inc1 = si == 1 ? dir : s1 / ad;
+            inc2 = si == 2 ? dir : s2 / ad;
+            p0 = pt0; p1 = pt1; p2 = pt2;
+            rl1 = (double)rl - 1.0;
+            dlen = fabs(1.0 / (double)sd_);
+            alive = length < rl1;
+        }
+        for (int it = 0; it < maxit; ++it) {
+            if (__ballot(alive) == 0ull) break;
+            uint32_t key = 0xFFFFFFFFu - (uint32_t)lane;   // unique per dead lane
+            if (alive) {
+                p0 += inc0;
+                p1 += inc1;
+                p2 += inc2;
+                double fx = floor((double)p0 - o0);
+                double fy = floor((double)p1 - o1);
+                double fz = floor((double)p2 - o2);
+                if (fx < 0 || fx >= g.XY || fy < 0 || fy >= g.XY || fz < 0 || fz >= g.Z) {
+                    alive = false;
+                } else {
+                    key = base + (uint32_t)lin_of((int64_t)fx, (int64_t)fy, (int64_t)fz, g);
+                }
+            }
+            // merge runs of equal keys among adjacent lanes
+            uint32_t prev = __shfl_up(key, 1);
+            bool is_voxel = key < 0x80000000u;
+            bool cont = (lane > 0) && (prev == key) && is_voxel;
+            uint64_t same = __ballot(cont);
+            if (is_voxel && !cont) {
+                uint64_t rest = (lane == 63) ? 0ull : (same >> (lane + 1));
+                int len = 1 + __builtin_ctzll(~rest);
+                __hip_atomic_fetch_add(tables + key, len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (alive) {
+                length += dlen;
+                alive = length < rl1;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Radix sort: 8-bit digits, tiles of 2048 keys (256 threads x 8).
+constexpr int RS_TILE = 2048;
+
+__global__ __launch_bounds__(256) void k_rs_hist(const uint32_t *__restrict__ keys, int64_t n, int shift,
+                                                 uint32_t *__restrict__ hist, int nblk) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        int64_t i = base + k * 256 + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan in place of data[0..n), one block of 1024 threads; total -> *total.
+__global__ __launch_bounds__(1024) void k_scan_excl(uint32_t *__restrict__ data, int64_t n,
+                                                    int32_t *__restrict__ total) {
+    __shared__ uint32_t wsum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int64_t per = (n + 1023) / 1024;
+    int64_t b = t * per, e = b + per < n ? b + per : n;
+    uint32_t sum = 0;
+    for (int64_t i = b; i < e; ++i) sum += data[i];
+    // inclusive wave scan
+    uint32_t v = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t u = __shfl_up(v, off);
+        if (lane >= off) v += u;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    if (t < 16) {
+        uint32_t a = wsum[t];
+        uint32_t x = a;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            uint32_t u = __shfl_up(x, off, 16);
+            if (t >= off) x += u;
+        }
+        wsum[t] = x - a;  // exclusive over waves
+    }
+    __syncthreads();
+    uint32_t run = wsum[w] + v - sum;
+    for (int64_t i = b; i < e; ++i) {
+        uint32_t d = data[i];
+        data[i] = run;
+        run += d;
+    }
+    if (t == 1023 && total) *total = (int32_t)run;
+}
+
+__global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t *__restrict__ kin,
+                                                    const uint32_t *__restrict__ vin,
+                                                    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                    int64_t n, int shift, const uint32_t *__restrict__ offs,
+                                                    int nblk) {
+    __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t gbase[256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int k = threadIdx.x; k < 1024; k += 256) (&wcnt[0][0])[k] = 0;
+    gbase[threadIdx.x] = offs[(int64_t)threadIdx.x * nblk + blockIdx.x];
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE + w * 512;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t kk[8], vv[8], lo[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        int64_t i = base + r * 64 + lane;
+        bool in = i < n;
+        uint32_t key = in ? kin[i] : 0u;
+        kk[r] = key;
+        vv[r] = in ? vin[i] : 0u;
+        uint32_t d = (key >> shift) & 0xFFu;
+        uint64_t m = __ballot(in);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        uint32_t rank = __popcll(m & lt), cnt = __popcll(m);
+        uint32_t prior = wcnt[w][d];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        if (in && rank == cnt - 1) wcnt[w][d] = prior + cnt;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        lo[r] = prior + rank;
+    }
+    __syncthreads();
+    {
+        const int d = threadIdx.x;
+        uint32_t a = 0;
+#pragma unroll
+        for (int ww = 0; ww < 4; ++ww) {
+            uint32_t c = wcnt[ww][d];
+            wcnt[ww][d] = a;
+            a += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        int64_t i = base + r * 64 + lane;
+        if (i < n) {
+            uint32_t d = (kk[r] >> shift) & 0xFFu;
+            uint32_t pos = gbase[d] + wcnt[w][d] + lo[r];
+            kout[pos] = kk[r];
+            vout[pos] = vv[r];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Segments of the sorted key array.
+__device__ __forceinline__ bool is_head(const uint32_t *__restrict__ k, int64_t i) {
+    uint32_t key = k[i];
+    return key != GVOM_NOKEY && (i == 0 || k[i - 1] != key);
+}
+
+__global__ __launch_bounds__(256) void k_seg_count(const uint32_t *__restrict__ k, int64_t n,
+                                                   uint32_t *__restrict__ blk) {
+    __shared__ uint32_t c;
+    if (threadIdx.x == 0) c = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    uint32_t mine = 0;
+    for (int j = 0; j < 8; ++j) {
+        int64_t i = base + threadIdx.x * 8 + j;
+        if (i < n && is_head(k, i)) ++mine;
+    }
+    // wave reduce then one LDS atomic per wave
+    for (int off = 32; off > 0; off >>= 1) mine += __shfl_down(mine, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&c, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) blk[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *__restrict__ k, int64_t n,
+                                                     const uint32_t *__restrict__ blk_off,
+                                                     int32_t *__restrict__ seg_start,
+                                                     uint32_t *__restrict__ seg_key) {
+    __shared__ uint32_t wsum[4];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int64_t base = (int64_t)blockIdx.x * RS_TILE + t * 8;
+    uint32_t flags = 0, cnt = 0;
+    for (int j = 0; j < 8; ++j) {
+        int64_t i = base + j;
+        if (i < n && is_head(k, i)) {
+            flags |= 1u << j;
+            ++cnt;
+        }
+    }
+    uint32_t v = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t u = __shfl_up(v, off);
+        if (lane >= off) v += u;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    uint32_t wo = 0;
+    for (int ww = 0; ww < w; ++ww) wo += wsum[ww];
+    uint32_t r = blk_off[blockIdx.x] + wo + v - cnt;
+    for (int j = 0; j < 8; ++j) {
+        if (flags & (1u << j)) {
+            int64_t i = base + j;
+            seg_start[r] = (int32_t)i;
+            seg_key[r] = k[i];
+            ++r;
+        }
+    }
+}
+
+__device__ __forceinline__ int32_t lower_bound_u32(const uint32_t *__restrict__ a, int32_t n, uint32_t v) {
+    int32_t lo = 0, hi = n;
+    while (lo < hi) {
+        int32_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One block: nvalid, end sentinel of the last segment, per-scan [first, in-bounds end).
+__global__ __launch_bounds__(64) void k_seg_bounds(const uint32_t *__restrict__ k, int64_t n,
+                                                   int32_t *__restrict__ seg_start,
+                                                   const uint32_t *__restrict__ seg_key, int S, Geo g,
+                                                   DevState *__restrict__ st) {
+    const int t = threadIdx.x;
+    const int32_t nseg = st->nseg;
+    if (t == 0) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (k[mid] < GVOM_NOKEY) lo = mid + 1;
+            else hi = mid;
+        }
+        st->nvalid = (int32_t)lo;
+        seg_start[nseg] = (int32_t)lo;
+    }
+    if (t < S) {
+        uint32_t sk = (uint32_t)t << g.VB;
+        int32_t f = lower_bound_u32(seg_key, nseg, sk);
+        int32_t e = lower_bound_u32(seg_key, nseg, sk | (1u << (g.VB - 1)));
+        st->first_seg[t] = f;
+        st->inb_end[t] = e;
+        st->slot_cells[(st->next_slot + t) % g.B] = e - f;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Per-segment moments about the segment's own voxel corner.  For an in-map segment
+// this is also where the slot record is born: hit = points in the voxel
+// (gvom.py:1170), total = ray passes + hits (1171, 1229), min_h = min local z over
+// the voxel's points (1390-1421), and the table entry becomes OCC | id.
+template <typename P>
+__global__ __launch_bounds__(256) void k_moments(const P *__restrict__ world, const uint32_t *__restrict__ svals,
+                                                 const int32_t *__restrict__ seg_start,
+                                                 const uint32_t *__restrict__ seg_key,
+                                                 const ScanDesc *__restrict__ sd, Geo g,
+                                                 const DevState *__restrict__ st,
+                                                 int32_t *__restrict__ tables, SlotRec *__restrict__ recs,
+                                                 int64_t ncap, Moment *__restrict__ mom) {
+    int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= st->nseg) return;
+    const uint32_t key = seg_key[gi];
+    const int s = (int)(key >> g.VB);
+    const uint32_t vk = key & ((1u << g.VB) - 1u);
+    const bool halo = (vk >> (g.VB - 1)) & 1u;
+    int64_t wx, wy, wz;
+    if (!halo) {
+        int64_t l = vk;
+        wz = l % g.Z;
+        int64_t c = l / g.Z;
+        wy = c % g.XY;
+        wx = c / g.XY;
+    } else {
+        int64_t l = vk & ((1u << (g.VB - 1)) - 1u);
+        wz = l % g.Zp - g.ez;
+        int64_t c = l / g.Zp;
+        wy = c % g.XYp - g.ex;
+        wx = c / g.XYp - g.ex;
+    }
+    double o[3];
+    scan_origin(sd[s], g, o);
+    double S0 = 0, S1 = 0, S2 = 0, Qxx = 0, Qxy = 0, Qxz = 0, Qyy = 0, Qyz = 0, Qzz = 0;
+    float mh = 1.0f;
+    const int32_t a = seg_start[gi], b = seg_start[gi + 1];
+    for (int32_t j = a; j < b; ++j) {
+        const int64_t i = svals[j];
+        double r0 = (((double)world[3 * i] / g.xy_res) - o[0]) - (double)wx;
+        double r1 = (((double)world[3 * i + 1] / g.xy_res) - o[1]) - (double)wy;
+        double r2 = (((double)world[3 * i + 2] / g.z_res) - o[2]) - (double)wz;
+        S0 += r0; S1 += r1; S2 += r2;
+        Qxx += r0 * r0; Qxy += r0 * r1; Qxz += r0 * r2;
+        Qyy += r1 * r1; Qyz += r1 * r2; Qzz += r2 * r2;
+        mh = py_minf(mh, (float)r2);
+    }
+    Moment m;
+    m.S[0] = S0; m.S[1] = S1; m.S[2] = S2;
+    m.Q[0] = Qxx; m.Q[1] = Qxy; m.Q[2] = Qxz; m.Q[3] = Qyy; m.Q[4] = Qyz; m.Q[5] = Qzz;
+    m.n = b - a;
+    m.pad = 0;
+    mom[gi] = m;
+    if (!halo) {
+        const int slot = (st->next_slot + s) % g.B;
+        const int32_t id = (int32_t)(gi - st->first_seg[s]);
+        int32_t *tv = tables + (int64_t)slot * g.V + vk;
+        const int32_t passes = *tv;
+        *tv = (int32_t)(GVOM_OCC | (uint32_t)id);
+        SlotRec *r = recs + (int64_t)slot * ncap + id;
+        r->hit = b - a;
+        r->total = (int32_t)((uint32_t)passes + (uint32_t)(b - a));
+        r->min_h = mh;
+        r->pad = 0;
+    }
+}
+
+// Neighbourhood statistics of one occupied voxel u (gvom.py:1250-1387): every valid
+// point p whose own voxel w lies within (+-xy_eigen_dist, +-z_eigen_dist) of u adds
+// local = p/res - origin - u.  With r = p/res - origin - w and d = w - u, segment w
+// contributes sum(local) = S_w + n_w d and, about the mean m with a = d - m,
+// sum((local - m)(local - m)^T) = Q_w + S_w a^T + a S_w^T + n_w a a^T.
+__device__ __forceinline__ int64_t neighbour_segment(int64_t x, int64_t y, int64_t z, int s, const Geo &g,
+                                                     const int32_t *__restrict__ table, int32_t fs,
+                                                     int32_t inb_end, int32_t nseg,
+                                                     const uint32_t *__restrict__ seg_key) {
+    if (x >= 0 && x < g.XY && y >= 0 && y < g.XY && z >= 0 && z < g.Z) {
+        int32_t tv = table[(x * g.XY + y) * g.Z + z];
+        if (tv >= 0) return -1;
+        return fs + (int64_t)(tv & 0x7FFFFFFF);
+    }
+    // halo voxel: binary search among this scan's halo segments
+    int64_t kx = x + g.ex, ky = y + g.ex, kz = z + g.ez;
+    uint32_t key = ((uint32_t)s << g.VB) | (1u << (g.VB - 1)) | (uint32_t)((kx * g.XYp + ky) * g.Zp + kz);
+    int32_t lo = inb_end, hi = nseg;
+    while (lo < hi) {
+        int32_t mid = (lo + hi) >> 1;
+        if (seg_key[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < nseg && seg_key[lo] == key) ? lo : -1;
+}
+
+__global__ __launch_bounds__(256) void k_metrics(const uint32_t *__restrict__ seg_key,
+                                                 const Moment *__restrict__ mom, Geo g,
+                                                 const DevState *__restrict__ st,
+                                                 const int32_t *__restrict__ tables,
+                                                 SlotRec *__restrict__ recs, int64_t ncap) {
+    int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t nseg = st->nseg;
+    if (gi >= nseg) return;
+    const uint32_t key = seg_key[gi];
+    const uint32_t vk = key & ((1u << g.VB) - 1u);
+    if ((vk >> (g.VB - 1)) & 1u) return;   // halo segment: no record
+    const int s = (int)(key >> g.VB);
+    const int slot = (st->next_slot + s) % g.B;
+    const int32_t fs = st->first_seg[s], ie = st->inb_end[s];
+    const int32_t *table = tables + (int64_t)slot * g.V;
+    const int64_t uz = vk % g.Z, uc = vk / g.Z, uy = uc % g.XY, ux = uc / g.XY;
+
+    double n = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    for (int dx = -g.ex; dx <= g.ex; ++dx)
+        for (int dy = -g.ex; dy <= g.ex; ++dy)
+            for (int dz = -g.ez; dz <= g.ez; ++dz) {
+                int64_t w = neighbour_segment(ux + dx, uy + dy, uz + dz, s, g, table, fs, ie, nseg, seg_key);
+                if (w < 0) continue;
+                const Moment &M = mom[w];
+                double nw = (double)M.n;
+                m0 += M.S[0] + nw * (double)dx;
+                m1 += M.S[1] + nw * (double)dy;
+                m2 += M.S[2] + nw * (double)dz;
+                n += nw;
+            }
+    m0 = m0 / n;
+    m1 = m1 / n;
+    m2 = m2 / n;
+    double cxx = 0, cxy = 0, cxz = 0, cyy = 0, cyz = 0, czz = 0;
+    for (int dx = -g.ex; dx <= g.ex; ++dx)
+        for (int dy = -g.ex; dy <= g.ex; ++dy)
+            for (int dz = -g.ez; dz <= g.ez; ++dz) {
+                int64_t w = neighbour_segment(ux + dx, uy + dy, uz + dz, s, g, table, fs, ie, nseg, seg_key);
+                if (w < 0) continue;
+                const Moment &M = mom[w];
+                double nw = (double)M.n;
+                double a0 = (double)dx - m0, a1 = (double)dy - m1, a2 = (double)dz - m2;
+                cxx += M.Q[0] + 2.0 * M.S[0] * a0 + nw * a0 * a0;
+                cxy += M.Q[1] + M.S[0] * a1 + a0 * M.S[1] + nw * a0 * a1;
+                cxz += M.Q[2] + M.S[0] * a2 + a0 * M.S[2] + nw * a0 * a2;
+                cyy += M.Q[3] + 2.0 * M.S[1] * a1 + nw * a1 * a1;
+                cyz += M.Q[4] + M.S[1] * a2 + a1 * M.S[2] + nw * a1 * a2;
+                czz += M.Q[5] + 2.0 * M.S[2] * a2 + nw * a2 * a2;
+            }
+    SlotRec *r = recs + (int64_t)slot * ncap + (gi - fs);
+    r->m[0] = m0;
+    r->m[1] = m1;
+    r->m[2] = m2;
+    bool pos = n > 0;   // gvom.py:1383-1385
+    r->m[3] = pos ? cxx / n : 0.0;
+    r->m[4] = pos ? cxy / n : 0.0;
+    r->m[5] = pos ? cxz / n : 0.0;
+    r->m[6] = pos ? cyy / n : 0.0;
+    r->m[7] = pos ? cyz / n : 0.0;
+    r->m[8] = pos ? czz / n : 0.0;
+    r->m[9] = n;
+}
+
+__global__ void k_commit_process(DevState *st, int S, int B) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int next = st->next_slot;
+    int mask = st->filled_mask;
+    for (int s = 0; s < S; ++s) mask |= 1 << ((next + s) % B);
+    st->filled_mask = mask;
+    st->last_slot = (next + S - 1) % B;
+    st->next_slot = (next + S) % B;
+}
+
+// Zero the slot tables of the S slots this batch writes (slot choice read on device).
+// Each thread clears 4 consecutive words of one table.
+__global__ __launch_bounds__(256) void k_clear_slots(int32_t *__restrict__ tables, const DevState *__restrict__ st,
+                                                     int S, int B, int64_t V) {
+    const int64_t per = (V + 3) / 4;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= per * S) return;
+    const int s = (int)(i / per);
+    const int slot = (st->next_slot + s) % B;
+    int32_t *t = tables + (int64_t)slot * V;
+    const int64_t j = (i - (int64_t)s * per) * 4;
+    if ((V & 3) == 0) {
+        *reinterpret_cast<int4 *>(t + j) = make_int4(0, 0, 0, 0);
+    } else {
+        for (int k = 0; k < 4; ++k)
+            if (j + k < V) t[j + k] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t *__restrict__ p, int32_t v, int64_t n) {
+    int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i + 3 < n) {
+        *reinterpret_cast<int4 *>(p + i) = make_int4(v, v, v, v);
+    } else {
+        for (; i < n; ++i) p[i] = v;
+    }
+}
+
+inline unsigned nblocks(int64_t n, int per) { return (unsigned)((n + per - 1) / per > 0 ? (n + per - 1) / per : 1); }
+
+}  // namespace
+
+// ---- launchers ----------------------------------------------------------------------
+hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    bool aligned = ((uintptr_t)p % 16) == 0;
+    if (!aligned) return hipErrorInvalidValue;
+    k_fill_i32<<<nblocks((n + 3) / 4, 256), 256, 0, s>>>(p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_prepare(int dtype, const void *in, void *world, uint32_t *keys, uint32_t *vals,
+                          uint8_t *valid, const ScanDesc *sd, int S, int64_t NT, const Geo &g,
+                          DevState *st, hipStream_t s) {
+    unsigned nb = nblocks(NT, 256);
+    if (dtype == GVOM_F32)
+        k_prepare<float><<<nb, 256, 0, s>>>((const float *)in, (float *)world, keys, vals, valid, sd, S, NT, g, st);
+    else
+        k_prepare<double><<<nb, 256, 0, s>>>((const double *)in, (double *)world, keys, vals, valid, sd, S, NT, g, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, const ScanDesc *sd,
+                          int S, int64_t NT, const Geo &g, const DevState *st, int32_t *slot_tables,
+                          int block, int rpt, hipStream_t s) {
+    if (NT <= 0) return hipSuccess;
+    if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
+    if (rpt < 1) rpt = 1;
+    int maxit = (g.XY > g.Z ? g.XY : g.Z) + 2;
+    unsigned nb = nblocks(NT, block * rpt);
+    if (dtype == GVOM_F32)
+        k_raycast<float><<<nb, block, 0, s>>>((const float *)world, valid, sd, S, NT, g, st, slot_tables, maxit, rpt);
+    else
+        k_raycast<double><<<nb, block, 0, s>>>((const double *)world, valid, sd, S, NT, g, st, slot_tables, maxit, rpt);
+    return hipGetLastError();
+}
+
+int64_t radix_hist_entries(int64_t n) { return 256 * (int64_t)nblocks(n, RS_TILE) + 1024; }
+
+hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits,
+                      uint32_t *hist, uint32_t **kout, uint32_t **vout, hipStream_t s) {
+    *kout = k0;
+    *vout = v0;
+    if (n <= 0) return hipSuccess;
+    int nblk = (int)nblocks(n, RS_TILE);
+    uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
+    for (int shift = 0; shift < bits; shift += 8) {
+        k_rs_hist<<<nblk, 256, 0, s>>>(ka, n, shift, hist, nblk);
+        k_scan_excl<<<1, 1024, 0, s>>>(hist, (int64_t)256 * nblk, nullptr);
+        k_rs_scatter<<<nblk, 256, 0, s>>>(ka, va, kb, vb, n, shift, hist, nblk);
+        uint32_t *tk = ka; ka = kb; kb = tk;
+        uint32_t *tv = va; va = vb; vb = tv;
+    }
+    *kout = ka;
+    *vout = va;
+    return hipGetLastError();
+}
+
+hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start, uint32_t *seg_key,
+                           uint32_t *blk, int S, const Geo &g, DevState *st, hipStream_t s) {
+    int nblk = (int)nblocks(n, RS_TILE);
+    if (n > 0) {
+        k_seg_count<<<nblk, 256, 0, s>>>(skeys, n, blk);
+        k_scan_excl<<<1, 1024, 0, s>>>(blk, nblk, &st->nseg);
+        k_seg_compact<<<nblk, 256, 0, s>>>(skeys, n, blk, seg_start, seg_key);
+    } else {
+        k_scan_excl<<<1, 1024, 0, s>>>(blk, 0, &st->nseg);
+    }
+    k_seg_bounds<<<1, 64, 0, s>>>(skeys, n, seg_start, seg_key, S, g, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_moments(int dtype, const void *world, const uint32_t *svals, const int32_t *seg_start,
+                          const uint32_t *seg_key, int64_t seg_cap, const ScanDesc *sd, int S,
+                          const Geo &g, DevState *st, int32_t *slot_tables, SlotRec *slot_recs,
+                          int64_t ncap, Moment *mom, hipStream_t s) {
+    (void)S;
+    unsigned nb = nblocks(seg_cap, 256);
+    if (dtype == GVOM_F32)
+        k_moments<float><<<nb, 256, 0, s>>>((const float *)world, svals, seg_start, seg_key, sd, g, st,
+                                             slot_tables, slot_recs, ncap, mom);
+    else
+        k_moments<double><<<nb, 256, 0, s>>>((const double *)world, svals, seg_start, seg_key, sd, g, st,
+                                              slot_tables, slot_recs, ncap, mom);
+    return hipGetLastError();
+}
+
+hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t seg_cap, int S,
+                          const Geo &g, const DevState *st, const int32_t *slot_tables,
+                          SlotRec *slot_recs, int64_t ncap, hipStream_t s) {
+    (void)S;
+    k_metrics<<<nblocks(seg_cap, 256), 256, 0, s>>>(seg_key, mom, g, st, slot_tables, slot_recs, ncap);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_slots(int32_t *slot_tables, const DevState *st, int S, const Geo &g, hipStream_t s) {
+    k_clear_slots<<<nblocks((g.V + 3) / 4 * S, 256), 256, 0, s>>>(slot_tables, st, S, g.B, g.V);
+    return hipGetLastError();
+}
+
+hipError_t launch_commit_process(DevState *st, int S, int B, hipStream_t s) {
+    k_commit_process<<<1, 64, 0, s>>>(st, S, B);
+    return hipGetLastError();
+}
+
+// Shared with gvom_combine.hip for the column scan.
+hipError_t launch_scan_excl_u32(uint32_t *data, int64_t n, int32_t *total, hipStream_t s) {
+    k_scan_excl<<<1, 1024, 0, s>>>(data, n, total);
+    return hipGetLastError();
+}

@@ -1,0 +1,249 @@
+"""Drop-in `Gvom` for G-VOM's scan->map path on MI355X.
+
+Same constructor (14 positional arguments, gvom.py:21-22), the same methods
+`Process_pointcloud` (gvom.py:105; also exported as `process_pointcloud`, the name the
+reference README uses), `combine_maps` (gvom.py:222) and the three debug exporters
+(gvom.py:395-442), with the same return values.  All compute runs in
+libgvom_hip.so (hand-written gfx950 kernels) through its C-ABI; this class only
+converts arguments and allocates the host arrays it returns.
+
+Differences from the reference, all in cases the reference leaves undefined or
+racy:
+  - `ego_position` is copied, not aliased (gvom.py:111 stores the caller's object);
+  - points with a non-finite coordinate are dropped (the reference indexes with NaN);
+  - an empty scan, or a scan or map without occupied cells, gives the empty-state
+    result instead of a zero-block launch error;
+  - concurrent calls are serialised per instance (the reference advances
+    buffer_index without a lock, gvom.py:211-216).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+import gvom_native as nat
+
+
+def _as_points(pointcloud):
+    pts = np.asarray(pointcloud)
+    if pts.dtype not in (np.float32, np.float64):
+        pts = pts.astype(np.float64)
+    pts = np.ascontiguousarray(pts).reshape(-1, 3)
+    return pts, (nat.GVOM_F32 if pts.dtype == np.float32 else nat.GVOM_F64)
+
+
+def _as_transform(transform):
+    if transform is None:
+        return None, nat.GVOM_NO_TRANSFORM
+    T = np.asarray(transform)
+    if T.dtype not in (np.float32, np.float64):
+        T = T.astype(np.float64)
+    T = np.ascontiguousarray(T)
+    if T.shape != (4, 4):
+        raise ValueError(f"transform must be 4x4, got {T.shape}")
+    return T, (nat.GVOM_F32 if T.dtype == np.float32 else nat.GVOM_F64)
+
+
+class Gvom:
+    """Scan -> voxel map -> 2D cost maps (reference class: scripts/gvom.py:8)."""
+
+    def __init__(self, xy_resolution, z_resolution, xy_size, z_size, buffer_size, min_distance,
+                 positive_obstacle_threshold, negative_obstacle_threshold, slope_obsacle_threshold,
+                 robot_height, robot_radius, ground_to_lidar_height, xy_eigen_dist, z_eigen_dist):
+        self._lib = nat.load()
+        self.xy_resolution = xy_resolution
+        self.z_resolution = z_resolution
+        self.xy_size = int(xy_size)
+        self.z_size = int(z_size)
+        self.buffer_size = int(buffer_size)
+        self.min_distance = min_distance
+        self.positive_obstacle_threshold = positive_obstacle_threshold
+        self.negative_obstacle_threshold = negative_obstacle_threshold
+        self.slope_obsacle_threshold = slope_obsacle_threshold
+        self.robot_height = robot_height
+        self.robot_radius = robot_radius
+        self.ground_to_lidar_height = ground_to_lidar_height
+        self.xy_eigen_dist = int(xy_eigen_dist)
+        self.z_eigen_dist = int(z_eigen_dist)
+        self.metrics_count = 10
+        self.voxel_count = self.xy_size * self.xy_size * self.z_size
+        self.ego_position = [0, 0, 0]
+        p = nat.Params(float(xy_resolution), float(z_resolution), self.xy_size, self.z_size,
+                       self.buffer_size, 0, float(min_distance), float(positive_obstacle_threshold),
+                       float(negative_obstacle_threshold), float(slope_obsacle_threshold),
+                       float(robot_height), float(robot_radius), float(ground_to_lidar_height),
+                       self.xy_eigen_dist, self.z_eigen_dist)
+        h = ctypes.c_void_p()
+        nat.check(self._lib.gvom_create(ctypes.byref(p), ctypes.byref(h)), None)
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.gvom_destroy(h)
+            self._h = None
+
+    # -- gvom.py:105-220 -------------------------------------------------------------
+    def Process_pointcloud(self, pointcloud, ego_position, transform=None):
+        """Imports a pointcloud, processes it into a voxel map and adds it to the buffer."""
+        pts, dt = _as_points(pointcloud)
+        ego = np.ascontiguousarray(np.asarray(ego_position, dtype=np.float64).reshape(3))
+        T, tdt = _as_transform(transform)
+        self.ego_position = tuple(float(v) for v in ego)
+        nat.check(self._lib.gvom_process_pointcloud(self._h, nat.ptr(pts), pts.shape[0], dt, nat.ptr(ego),
+                                                    nat.ptr(T), tdt), self._h)
+
+    process_pointcloud = Process_pointcloud
+
+    def process_pointcloud_device(self, dev_points, n, ego_position, transform=None, dtype=np.float32):
+        """Process_pointcloud on points already in device memory (dev_points: a device
+        pointer as int / c_void_p / DeviceArray).  Does not synchronise."""
+        ptr = dev_points.ptr if isinstance(dev_points, nat.DeviceArray) else ctypes.c_void_p(int(dev_points))
+        ego = np.ascontiguousarray(np.asarray(ego_position, dtype=np.float64).reshape(3))
+        T, tdt = _as_transform(transform)
+        dt = nat.GVOM_F32 if np.dtype(dtype) == np.float32 else nat.GVOM_F64
+        self.ego_position = tuple(float(v) for v in ego)
+        nat.check(self._lib.gvom_process_pointcloud_device(self._h, ptr, int(n), dt, nat.ptr(ego), nat.ptr(T), tdt),
+                  self._h)
+
+    def reserve(self, max_points, max_batch_points=0):
+        nat.check(self._lib.gvom_reserve(self._h, int(max_points), int(max_batch_points)), self._h)
+
+    def process_batch(self, points, offsets, egos, transforms=None):
+        """Several scans in one launch sequence; same result as len(offsets)-1
+        consecutive Process_pointcloud calls.  points: flat (sum N, 3) array."""
+        pts, dt = _as_points(points)
+        off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+        ego = np.ascontiguousarray(np.asarray(egos, dtype=np.float64).reshape(-1, 3))
+        S = off.shape[0] - 1
+        if ego.shape[0] != S:
+            raise ValueError("one ego position per scan is required")
+        T, tdt = None, nat.GVOM_NO_TRANSFORM
+        if transforms is not None:
+            T = np.asarray(transforms)
+            if T.dtype not in (np.float32, np.float64):
+                T = T.astype(np.float64)
+            T = np.ascontiguousarray(T).reshape(S, 4, 4)
+            tdt = nat.GVOM_F32 if T.dtype == np.float32 else nat.GVOM_F64
+        self.ego_position = tuple(float(v) for v in ego[-1])
+        nat.check(self._lib.gvom_process_batch(self._h, nat.ptr(pts), nat.ptr(off), S, dt, nat.ptr(ego),
+                                               nat.ptr(T), tdt, 0), self._h)
+
+    # -- gvom.py:222-393 -------------------------------------------------------------
+    def combine_maps(self):
+        """Combines all maps in the buffer and processes them into 2D maps."""
+        X = self.xy_size
+        origin = np.zeros(3, np.float64)
+        pos = np.empty((X, X), np.int32)
+        neg = np.empty((X, X), np.int32)
+        rough = np.empty((X, X), np.float64)
+        vis = np.empty((X, X), np.int32)
+        rc = nat.check(self._lib.gvom_combine_maps(self._h, nat.ptr(origin), nat.ptr(pos), nat.ptr(neg),
+                                                   nat.ptr(rough), nat.ptr(vis)), self._h)
+        if rc == nat.GVOM_NO_DATA:
+            print("ERROR: No data in buffer")
+            return None
+        return (origin, pos, neg, rough, vis)
+
+    def combine_maps_device(self):
+        """combine_maps without host copies or synchronisation; returns the origin in
+        metres, or None when the buffer is empty."""
+        origin = np.zeros(3, np.float64)
+        rc = nat.check(self._lib.gvom_combine_maps_device(self._h, nat.ptr(origin)), self._h)
+        return None if rc == nat.GVOM_NO_DATA else origin
+
+    # -- gvom.py:395-442 -------------------------------------------------------------
+    def make_debug_voxel_map(self):
+        rows = ctypes.c_int64(0)
+        rc = nat.check(self._lib.gvom_debug_voxel_map(self._h, None, 0, ctypes.byref(rows)), self._h)
+        if rc == nat.GVOM_NO_DATA:
+            print("No data")
+            return None
+        out = np.zeros((rows.value, 8), np.float32)
+        nat.check(self._lib.gvom_debug_voxel_map(self._h, nat.ptr(out), rows.value, ctypes.byref(rows)), self._h)
+        return out
+
+    def make_debug_height_map(self):
+        out = np.zeros((self.xy_size * self.xy_size, 7), np.float32)
+        rc = nat.check(self._lib.gvom_debug_height_map(self._h, nat.ptr(out)), self._h)
+        if rc == nat.GVOM_NO_DATA:
+            print("No data")
+            return None
+        return out
+
+    def make_debug_inferred_height_map(self):
+        out = np.zeros((self.xy_size * self.xy_size, 3), np.float32)
+        rc = nat.check(self._lib.gvom_debug_inferred_height_map(self._h, nat.ptr(out)), self._h)
+        if rc == nat.GVOM_NO_DATA:
+            print("No data")
+            return None
+        return out
+
+    # -- state export (no reference method; the reference keeps these as members) ---
+    def ring_state(self):
+        vals = [ctypes.c_int32() for _ in range(4)]
+        nat.check(self._lib.gvom_ring_state(self._h, *[ctypes.byref(v) for v in vals]), self._h)
+        return dict(buffer_index=vals[0].value, last_buffer_index=vals[1].value,
+                    filled_mask=vals[2].value, have_combined=bool(vals[3].value))
+
+    def slot_state(self, slot):
+        """Slot `slot` in the reference's format: (index_map int32[V], hit, total, min_h,
+        metrics f64[C,10], origin f64[3]) or None when the slot is empty."""
+        cells = ctypes.c_int64(0)
+        origin = np.zeros(3)
+        rc = nat.check(self._lib.gvom_get_slot(self._h, slot, None, 0, None, None, None, None, nat.ptr(origin),
+                                               ctypes.byref(cells)), self._h)
+        if rc == nat.GVOM_NO_DATA:
+            return None
+        C = cells.value
+        idx = np.empty(self.voxel_count, np.int32)
+        hit = np.empty(C, np.int32)
+        tot = np.empty(C, np.int32)
+        mh = np.empty(C, np.float32)
+        met = np.empty((C, 10), np.float64)
+        nat.check(self._lib.gvom_get_slot(self._h, slot, nat.ptr(idx), C, nat.ptr(hit), nat.ptr(tot), nat.ptr(mh),
+                                          nat.ptr(met), nat.ptr(origin), ctypes.byref(cells)), self._h)
+        return dict(index_map=idx, hit=hit, total=tot, min_h=mh, metrics=met, origin=origin)
+
+    def combined_state(self):
+        cells = ctypes.c_int64(0)
+        origin = np.zeros(3)
+        rc = nat.check(self._lib.gvom_get_combined(self._h, None, 0, None, None, None, None, None, nat.ptr(origin),
+                                                   ctypes.byref(cells)), self._h)
+        if rc == nat.GVOM_NO_DATA:
+            return None
+        C = cells.value
+        idx = np.empty(self.voxel_count, np.int32)
+        hit = np.empty(C, np.int32)
+        tot = np.empty(C, np.int32)
+        mh = np.empty(C, np.float32)
+        met = np.empty((C, 10), np.float32)
+        ev = np.empty((C, 3), np.float32)
+        nat.check(self._lib.gvom_get_combined(self._h, nat.ptr(idx), C, nat.ptr(hit), nat.ptr(tot), nat.ptr(mh),
+                                              nat.ptr(met), nat.ptr(ev), nat.ptr(origin), ctypes.byref(cells)),
+                  self._h)
+        return dict(index_map=idx, hit=hit, total=tot, min_h=mh, metrics=met, eigen=ev, origin=origin)
+
+    def maps_state(self):
+        X = self.xy_size
+        f = {k: np.empty((X, X), np.float64) for k in ("height", "inferred", "slope_x", "slope_y", "rough", "delta")}
+        i = {k: np.empty((X, X), np.int32) for k in ("pos", "neg", "vis")}
+        rc = nat.check(self._lib.gvom_get_maps(self._h, *[nat.ptr(f[k]) for k in
+                                                           ("height", "inferred", "slope_x", "slope_y", "rough",
+                                                            "delta")],
+                                               *[nat.ptr(i[k]) for k in ("pos", "neg", "vis")]), self._h)
+        if rc == nat.GVOM_NO_DATA:
+            return None
+        return {**f, **i}
+
+    def set_timing(self, enabled=True):
+        nat.check(self._lib.gvom_set_timing(self._h, 1 if enabled else 0), self._h)
+
+    def stage_times(self):
+        ms = (ctypes.c_float * 7)()
+        nat.check(self._lib.gvom_stage_times(self._h, ms, 7), self._h)
+        return list(ms)
+
+    def sync(self):
+        nat.check(self._lib.gvom_sync(self._h), self._h)

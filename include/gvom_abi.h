@@ -1,0 +1,178 @@
+/*
+ * gvom_abi.h -- C-ABI of libgvom_hip.so, the MI355X (gfx950) implementation of
+ * G-VOM's scan->map path.
+ *
+ * The reference exposes this path as one Python class, Gvom (scripts/gvom.py:8),
+ * with no native boundary.  Each entry point below names the reference method or
+ * behaviour it replaces.  Calling conventions:
+ *   - plain pointers and sizes only; the library owns all device memory, the caller
+ *     owns every host buffer it passes;
+ *   - entry points return 0 on success, GVOM_NO_DATA (1) where the reference returns
+ *     None, and a negative GVOM_E* code on error; gvom_last_error() gives the text;
+ *   - every call on one handle is serialised by a per-handle mutex, and all device
+ *     work of a handle runs in order on one HIP stream;
+ *   - "_device" variants take device pointers and do not synchronise the stream.
+ *
+ * Map layout of all 2D outputs: C-order [x, y] arrays of xy_size * xy_size, as the
+ * reference's combine_maps returns them (gvom.py:393).
+ */
+#ifndef GVOM_ABI_H
+#define GVOM_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GVOM_OK 0
+#define GVOM_NO_DATA 1
+#define GVOM_EINVAL (-1)
+#define GVOM_EHIP (-2)
+#define GVOM_ECAP (-3)
+#define GVOM_ESTATE (-4)
+
+#define GVOM_F32 0
+#define GVOM_F64 1
+#define GVOM_NO_TRANSFORM (-1)
+
+/* The 14 constructor parameters of Gvom.__init__ (gvom.py:21-22), in order.
+ * slope_obstacle_threshold is the reference's `slope_obsacle_threshold`. */
+typedef struct gvom_params {
+    double xy_resolution;
+    double z_resolution;
+    int32_t xy_size;
+    int32_t z_size;
+    int32_t buffer_size;
+    int32_t reserved0;
+    double min_distance;
+    double positive_obstacle_threshold;
+    double negative_obstacle_threshold;
+    double slope_obstacle_threshold;
+    double robot_height;
+    double robot_radius;
+    double ground_to_lidar_height;
+    int32_t xy_eigen_dist;
+    int32_t z_eigen_dist;
+} gvom_params;
+
+typedef struct gvom_handle gvom_handle;
+
+/* Replaces Gvom.__init__ (gvom.py:21-101).  Allocates the ring of buffer_size slots,
+ * two combined maps and the 2D maps on the current HIP device. */
+int gvom_create(const gvom_params *params, gvom_handle **out);
+
+/* Releases everything gvom_create allocated (no reference counterpart: the
+ * reference relies on Numba's garbage collection of device arrays). */
+void gvom_destroy(gvom_handle *h);
+
+/* Text of the last error on this handle (or of the last failed gvom_create when
+ * h is NULL).  Replaces the Numba/CUDA exceptions the reference raises. */
+const char *gvom_last_error(const gvom_handle *h);
+
+/* Pre-sizes the per-scan work buffers and slot records for scans of up to
+ * max_points points and batches of up to max_batch_points points.  Optional: the
+ * process calls grow the buffers on demand (reference: per-call cuda.device_array,
+ * gvom.py:124-174). */
+int gvom_reserve(gvom_handle *h, int64_t max_points, int64_t max_batch_points);
+
+/* Replaces Gvom.Process_pointcloud(pointcloud, ego_position, transform)
+ * (gvom.py:105-220).  xyz: host array of n x 3 values of type dtype (GVOM_F32 or
+ * GVOM_F64), C-contiguous.  ego: 3 doubles (copied, not aliased).  T: host 4x4
+ * row-major matrix of type t_dtype, or NULL / t_dtype == GVOM_NO_TRANSFORM for no
+ * transform.  The caller's arrays are not modified (the reference transforms its own
+ * device copy, gvom.py:115, 1136-1138). */
+int gvom_process_pointcloud(gvom_handle *h, const void *xyz, int64_t n, int32_t dtype,
+                            const double ego[3], const void *T, int32_t t_dtype);
+
+/* As gvom_process_pointcloud, with xyz a device pointer (read only).  ego and T are
+ * host pointers.  Does not synchronise. */
+int gvom_process_pointcloud_device(gvom_handle *h, const void *xyz_device, int64_t n,
+                                   int32_t dtype, const double ego[3], const void *T,
+                                   int32_t t_dtype);
+
+/* Several scans in one launch sequence, with the same result as n_scans
+ * consecutive Process_pointcloud calls (gvom.py:105-220).  xyz: flat array of
+ * offsets[n_scans] points (host, or device when on_device != 0); offsets: n_scans+1
+ * host int64 prefix offsets; ego: n_scans x 3 host doubles; T: n_scans x 16 host
+ * values of t_dtype or NULL.  n_scans must not exceed buffer_size. */
+int gvom_process_batch(gvom_handle *h, const void *xyz, const int64_t *offsets, int32_t n_scans,
+                       int32_t dtype, const double *ego, const void *T, int32_t t_dtype,
+                       int32_t on_device);
+
+/* Replaces Gvom.combine_maps() (gvom.py:222-393).  Writes the map origin in metres
+ * and the four xy_size x xy_size maps to host memory and synchronises.  Returns
+ * GVOM_NO_DATA when no scan has been processed (gvom.py:225-227 prints "ERROR: No data
+ * in buffer" and returns None; the library leaves printing to its caller).  Any output pointer may be NULL to skip that copy. */
+int gvom_combine_maps(gvom_handle *h, double origin_m[3], int32_t *positive_obstacle,
+                      int32_t *negative_obstacle, double *roughness, int32_t *visibility);
+
+/* combine_maps without host copies or synchronisation: the maps stay on the device
+ * (see gvom_device_maps).  origin_m is written from the host-side mirror and may be
+ * NULL. */
+int gvom_combine_maps_device(gvom_handle *h, double origin_m[3]);
+
+/* Device pointers of the latest combined 2D maps (valid until the next combine or
+ * gvom_destroy): positive, negative, visibility (int32) and roughness (f64). */
+int gvom_device_maps(gvom_handle *h, int32_t **positive_obstacle, int32_t **negative_obstacle,
+                     double **roughness, int32_t **visibility);
+
+/* Waits for all work queued on the handle's stream. */
+int gvom_sync(gvom_handle *h);
+
+/* Replaces Gvom.make_debug_voxel_map (gvom.py:395-409): rows of 8 floats per
+ * combined cell, in cell-id order.  cap = rows available in out; *rows receives the
+ * cell count.  Returns GVOM_NO_DATA when no combine has run. */
+int gvom_debug_voxel_map(gvom_handle *h, float *out, int64_t cap, int64_t *rows);
+
+/* Replaces Gvom.make_debug_height_map (gvom.py:411-425): xy_size^2 rows of 7 floats. */
+int gvom_debug_height_map(gvom_handle *h, float *out);
+
+/* Replaces Gvom.make_debug_inferred_height_map (gvom.py:427-442): xy_size^2 rows of 3. */
+int gvom_debug_inferred_height_map(gvom_handle *h, float *out);
+
+/* State export for parity tests and checkpointing (the reference keeps this state
+ * in its ring lists, gvom.py:50-56, and combined_* members, 58-92).  index_map is
+ * written in the reference's layout (v = x + y*xy + z*xy*xy) and encoding (>= 0 cell
+ * id, -1 unknown, -t-1 free with t passes); compact arrays have cap entries.  Any
+ * output may be NULL.  Slot arrays: hit, total (int32), min_h (f32), metrics (f64
+ * [C][10]).  Returns GVOM_NO_DATA for an empty slot. */
+int gvom_get_slot(gvom_handle *h, int32_t slot, int32_t *index_map, int64_t cap, int32_t *hit,
+                  int32_t *total, float *min_h, double *metrics, double origin[3], int64_t *cells);
+
+/* The latest combined map: metrics f32[C][10], eigenvalues f32[C][3]
+ * (gvom.py:276, 315). */
+int gvom_get_combined(gvom_handle *h, int32_t *index_map, int64_t cap, int32_t *hit,
+                      int32_t *total, float *min_h, float *metrics, float *eigen, double origin[3],
+                      int64_t *cells);
+
+/* The latest 2D maps, including the intermediate ones the reference keeps as members
+ * (height_map, inferred_height_map, x/y_slope_map, roughness_map, guessed_height_delta,
+ * gvom.py:327-359).  Each output may be NULL. */
+int gvom_get_maps(gvom_handle *h, double *height, double *inferred, double *slope_x,
+                  double *slope_y, double *roughness, double *height_delta, int32_t *positive,
+                  int32_t *negative, int32_t *visibility);
+
+/* Host-side ring state (gvom.py:211-216): write index, newest slot, filled-slot mask,
+ * and whether a combined map exists. */
+int gvom_ring_state(const gvom_handle *h, int32_t *buffer_index, int32_t *last_buffer_index,
+                    int32_t *filled_mask, int32_t *have_combined);
+
+/* Per-stage device time (HIP events on the handle's stream), in milliseconds, averaged
+ * over every process / combine call made since timing was last enabled
+ * (gvom_set_timing resets the record).  Stage order: 0 clear+prepare, 1 ray cast,
+ * 2 sort+segments, 3 moments+metrics, 4 combine tables, 5 column scan+records+columns,
+ * 6 2D maps.  n = entries available in ms.  (The reference has commented-out
+ * wall-clock prints at these points, gvom.py:107-217, 224, 321-322, 390.) */
+int gvom_set_timing(gvom_handle *h, int32_t enabled);
+int gvom_stage_times(gvom_handle *h, float *ms, int32_t n);
+
+/* The handle's HIP stream (as void*), for callers that order their own work
+ * against it. */
+void *gvom_stream(gvom_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GVOM_ABI_H */

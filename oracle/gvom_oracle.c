@@ -1,0 +1,668 @@
+/*
+ * gvom_oracle.c -- CPU restatement of G-VOM's scan->map path (TEST INFRASTRUCTURE).
+ *
+ * This file is the oracle: a scalar C restatement of scripts/gvom.py that the
+ * parity tests, __graft_entry__.smoke() and the cpu_baseline leg of bench.py use
+ * as the checker.  Nothing in the product (g-vom_amd/) links, loads or calls it.
+ *
+ * Numeric semantics ("exact typing") follow the types Numba's CUDA target gives
+ * each expression in gvom.py:
+ *   - f32 op f32 -> f32, f32 op f64 -> f64, f32 op int64 -> f64, Python float
+ *     literals and parameters are f64;
+ *   - math.floor(f64) -> f64 (integral), so index arithmetic is exact;
+ *   - ray state pt/end/slope are f32 locals (gvom.py:1174-1176), `length` is f64;
+ *   - no FMA contraction anywhere (build with -ffp-contract=off);
+ *   - atomics become sequential updates in a fixed order (point order, or voxel
+ *     order in the reference linear layout x + y*xy + z*xy*xy), which is one of
+ *     the orders the reference's atomics may take.  Cell ids are assigned in
+ *     reference linear order.
+ *
+ * Defined behaviour where the reference is undefined (kept identical in the
+ * product):
+ *   - points with a non-finite coordinate, or with |p_k / res_k| >= 2^60, are
+ *     dropped before any use (the reference would index with NaN);
+ *   - an empty scan or a map with zero occupied cells yields the empty-state
+ *     outputs instead of a zero-block launch error.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+    double xy_resolution, z_resolution;
+    int32_t xy_size, z_size;
+    int32_t buffer_size, pad0;
+    double min_distance;
+    double positive_obstacle_threshold, negative_obstacle_threshold;
+    double slope_obstacle_threshold;
+    double robot_height, robot_radius, ground_to_lidar_height;
+    int32_t xy_eigen_dist, z_eigen_dist;
+} orc_params;
+
+#define ORC_F32 0
+#define ORC_F64 1
+#define ORC_NONE (-1)
+
+static inline double ld(const void *base, int dt, int64_t i) {
+    return dt == ORC_F32 ? (double)((const float *)base)[i] : ((const double *)base)[i];
+}
+
+/* Python's max(a, b) as Numba lowers it: b > a ? b : a  (numba cpython/builtins.py do_minmax). */
+static inline float py_maxf(float a, float b) { return (b > a) ? b : a; }
+static inline double py_max(double a, double b) { return (b > a) ? b : a; }
+static inline double py_min(double a, double b) { return (b < a) ? b : a; }
+static inline float py_minf(float a, float b) { return (b < a) ? b : a; }
+
+/* Origin of a scan: gvom.py:138-141 (host f64 arithmetic). */
+void orc_origin(const orc_params *p, const double ego[3], double origin[3]) {
+    origin[0] = floor((ego[0] / p->xy_resolution) - (double)p->xy_size / 2.0);
+    origin[1] = floor((ego[1] / p->xy_resolution) - (double)p->xy_size / 2.0);
+    origin[2] = floor((ego[2] / p->z_resolution) - (double)p->z_size / 2.0);
+}
+
+/* __transform_pointcloud, gvom.py:1122-1138.  Arithmetic type is the promotion of
+ * the point and transform dtypes; the result is stored back in the point dtype. */
+static void transform_points(void *pts, int dt, int64_t n, const void *T, int tdt) {
+    for (int64_t i = 0; i < n; ++i) {
+        if (dt == ORC_F32 && tdt == ORC_F32) {
+            float *q = (float *)pts + 3 * i;
+            const float *t = (const float *)T;
+            float x = q[0], y = q[1], z = q[2];
+            float r0 = x * t[0] + y * t[1] + z * t[2] + t[3];
+            float r1 = x * t[4] + y * t[5] + z * t[6] + t[7];
+            float r2 = x * t[8] + y * t[9] + z * t[10] + t[11];
+            q[0] = r0; q[1] = r1; q[2] = r2;
+        } else {
+            double x = ld(pts, dt, 3 * i), y = ld(pts, dt, 3 * i + 1), z = ld(pts, dt, 3 * i + 2);
+            double t[12];
+            for (int k = 0; k < 12; ++k) t[k] = ld(T, tdt, k);
+            double r0 = x * t[0] + y * t[1] + z * t[2] + t[3];
+            double r1 = x * t[4] + y * t[5] + z * t[6] + t[7];
+            double r2 = x * t[8] + y * t[9] + z * t[10] + t[11];
+            if (dt == ORC_F32) {
+                float *q = (float *)pts + 3 * i;
+                q[0] = (float)r0; q[1] = (float)r1; q[2] = (float)r2;
+            } else {
+                double *q = (double *)pts + 3 * i;
+                q[0] = r0; q[1] = r1; q[2] = r2;
+            }
+        }
+    }
+}
+
+/* Points the path uses at all (see header): finite and |p/res| < 2^60. */
+static int in_domain(const orc_params *p, double x, double y, double z) {
+    if (!isfinite(x) || !isfinite(y) || !isfinite(z)) return 0;
+    const double lim = 1152921504606846976.0; /* 2^60 */
+    if (fabs(x / p->xy_resolution) >= lim || fabs(y / p->xy_resolution) >= lim ||
+        fabs(z / p->z_resolution) >= lim)
+        return 0;
+    return 1;
+}
+
+/* Minimum-distance filter on world coordinates, in the point dtype (gvom.py:1145-1149). */
+static int passes_min_distance(const orc_params *p, const void *pts, int dt, int64_t i) {
+    double md2 = p->min_distance * p->min_distance;
+    if (dt == ORC_F32) {
+        const float *q = (const float *)pts + 3 * i;
+        float d2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+        return !((double)d2 < md2);
+    } else {
+        const double *q = (const double *)pts + 3 * i;
+        double d2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+        return !(d2 < md2);
+    }
+}
+
+static inline int64_t ref_lin(int64_t x, int64_t y, int64_t z, int64_t XY) {
+    return x + y * XY + z * XY * XY;
+}
+
+/* __point_2_map ray march, gvom.py:1174-1231. */
+static void trace_ray(const orc_params *p, const double ego[3], const double origin[3],
+                      double px, double py, double pz, int32_t *total) {
+    const int64_t XY = p->xy_size, Z = p->z_size;
+    float pt[3], end[3], s[3];
+    pt[0] = (float)(ego[0] / p->xy_resolution);
+    pt[1] = (float)(ego[1] / p->xy_resolution);
+    pt[2] = (float)(ego[2] / p->z_resolution);
+    end[0] = (float)(px / p->xy_resolution);
+    end[1] = (float)(py / p->xy_resolution);
+    end[2] = (float)(pz / p->z_resolution);
+    s[0] = end[0] - pt[0];
+    s[1] = end[1] - pt[1];
+    s[2] = end[2] - pt[2];
+    float ray_length = sqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    s[0] = s[0] / ray_length;
+    s[1] = s[1] / ray_length;
+    s[2] = s[2] / ray_length;
+    float smax = py_maxf(fabsf(s[0]), py_maxf(fabsf(s[1]), fabsf(s[2])));
+    int si = 0;
+    if (smax == fabsf(s[1])) si = 1;
+    if (smax == fabsf(s[2])) si = 2;
+    double length = 0.0;
+    float direction = s[si] / fabsf(s[si]);
+    int a1 = (si + 1) % 3, a2 = (si + 2) % 3;
+    while (length < (double)ray_length - 1.0) {
+        pt[si] += direction;
+        pt[a1] += s[a1] / fabsf(s[si]);
+        pt[a2] += s[a2] / fabsf(s[si]);
+        double xi = floor((double)pt[0] - origin[0]);
+        if (xi < 0 || xi >= (double)XY) return;
+        double yi = floor((double)pt[1] - origin[1]);
+        if (yi < 0 || yi >= (double)XY) return;
+        double zi = floor((double)pt[2] - origin[2]);
+        if (zi < 0 || zi >= (double)Z) return;
+        total[ref_lin((int64_t)xi, (int64_t)yi, (int64_t)zi, XY)] += 1;
+        length += fabs(1.0 / (double)s[si]);
+    }
+}
+
+/*
+ * One Process_pointcloud call (gvom.py:105-220) on one scan.
+ *
+ *   pts (n x 3, dtype dt) is copied to pts_world and transformed there when T != NULL.
+ *   index_map: int32[V] in reference layout and encoding (>=0 id, -1 unknown, -t-1 free).
+ *   hit/total/min_h/metrics: compact arrays of capacity cap (metrics f64[cap][10]).
+ *   scratch_hit/scratch_total: int32[V] work arrays.
+ * Returns the cell count C_s, or -1 when it exceeds cap.
+ */
+int64_t orc_process(const orc_params *p, const void *pts, int64_t n, int32_t dt,
+                    const double ego[3], const void *T, int32_t tdt,
+                    void *pts_world, double origin[3], int32_t *index_map,
+                    int32_t *hit_c, int32_t *total_c, double *metrics, float *min_h, int64_t cap,
+                    int32_t *scratch_hit, int32_t *scratch_total, uint8_t *valid) {
+    const int64_t XY = p->xy_size, Z = p->z_size, V = XY * XY * Z;
+    const size_t esz = dt == ORC_F32 ? 4 : 8;
+    memcpy(pts_world, pts, (size_t)n * 3 * esz);
+    orc_origin(p, ego, origin);
+    if (T != NULL && tdt != ORC_NONE) transform_points(pts_world, dt, n, T, tdt);
+
+    for (int64_t i = 0; i < n; ++i) {
+        double x = ld(pts_world, dt, 3 * i), y = ld(pts_world, dt, 3 * i + 1),
+               z = ld(pts_world, dt, 3 * i + 2);
+        valid[i] = (uint8_t)(in_domain(p, x, y, z) && passes_min_distance(p, pts_world, dt, i));
+    }
+
+    memset(scratch_hit, 0, (size_t)V * 4);
+    memset(scratch_total, 0, (size_t)V * 4);
+
+    /* __point_2_map, gvom.py:1141-1231 */
+    for (int64_t i = 0; i < n; ++i) {
+        if (!valid[i]) continue;
+        double x = ld(pts_world, dt, 3 * i), y = ld(pts_world, dt, 3 * i + 1),
+               z = ld(pts_world, dt, 3 * i + 2);
+        int oob = 0;
+        double xi = floor((x / p->xy_resolution) - origin[0]);
+        if (xi < 0 || xi >= (double)XY) oob = 1;
+        double yi = floor((y / p->xy_resolution) - origin[1]);
+        if (yi < 0 || yi >= (double)XY) oob = 1;
+        double zi = floor((z / p->z_resolution) - origin[2]);
+        if (zi < 0 || zi >= (double)Z) oob = 1;
+        if (!oob) {
+            int64_t v = ref_lin((int64_t)xi, (int64_t)yi, (int64_t)zi, XY);
+            scratch_hit[v] += 1;
+            scratch_total[v] += 1;
+        }
+        trace_ray(p, ego, origin, x, y, z, scratch_total);
+    }
+
+    /* __assign_indices + __move_data, gvom.py:1234-1247 */
+    int64_t cells = 0;
+    for (int64_t v = 0; v < V; ++v) {
+        if (scratch_hit[v] > 0) {
+            if (cells >= cap) return -1;
+            index_map[v] = (int32_t)cells;
+            hit_c[cells] = scratch_hit[v];
+            total_c[cells] = scratch_total[v];
+            ++cells;
+        } else {
+            index_map[v] = -scratch_total[v] - 1;
+        }
+    }
+
+    /* __calculate_metrics_master, gvom.py:1069-1119 */
+    memset(metrics, 0, (size_t)cells * 10 * sizeof(double));
+    for (int64_t c = 0; c < cells; ++c) min_h[c] = 1.0f;
+    const int ex = p->xy_eigen_dist, ez = p->z_eigen_dist;
+
+    /* __calculate_mean, gvom.py:1250-1298 */
+    for (int64_t i = 0; i < n; ++i) {
+        if (!valid[i]) continue;
+        double x = ld(pts_world, dt, 3 * i), y = ld(pts_world, dt, 3 * i + 1),
+               z = ld(pts_world, dt, 3 * i + 2);
+        double bx = floor((x / p->xy_resolution) - origin[0]);
+        double by = floor((y / p->xy_resolution) - origin[1]);
+        double bz = floor((z / p->z_resolution) - origin[2]);
+        if (bx < -ex - 1 || bx > XY + ex || by < -ex - 1 || by > XY + ex || bz < -ez - 1 || bz > Z + ez)
+            continue; /* no neighbour can be in bounds */
+        for (int64_t xi = (int64_t)bx - ex; xi < (int64_t)bx + 1 + ex; ++xi) {
+            if (xi < 0 || xi >= XY) continue;
+            for (int64_t yi = (int64_t)by - ex; yi < (int64_t)by + 1 + ex; ++yi) {
+                if (yi < 0 || yi >= XY) continue;
+                for (int64_t zi = (int64_t)bz - ez; zi < (int64_t)bz + 1 + ez; ++zi) {
+                    if (zi < 0 || zi >= Z) continue;
+                    double l0 = (x / p->xy_resolution) - origin[0] - (double)xi;
+                    double l1 = (y / p->xy_resolution) - origin[1] - (double)yi;
+                    double l2 = (z / p->z_resolution) - origin[2] - (double)zi;
+                    int32_t id = index_map[ref_lin(xi, yi, zi, XY)];
+                    if (id < 0) continue;
+                    double *m = metrics + 10 * (int64_t)id;
+                    m[0] += l0;
+                    m[1] += l1;
+                    m[2] += l2;
+                    m[9] += 1.0;
+                }
+            }
+        }
+    }
+    /* __normalize_mean, gvom.py:1301-1308 */
+    for (int64_t c = 0; c < cells; ++c)
+        for (int j = 0; j < 3; ++j) metrics[10 * c + j] = metrics[10 * c + j] / metrics[10 * c + 9];
+
+    /* __calculate_covariance, gvom.py:1311-1371 */
+    for (int64_t i = 0; i < n; ++i) {
+        if (!valid[i]) continue;
+        double x = ld(pts_world, dt, 3 * i), y = ld(pts_world, dt, 3 * i + 1),
+               z = ld(pts_world, dt, 3 * i + 2);
+        double bx = floor((x / p->xy_resolution) - origin[0]);
+        double by = floor((y / p->xy_resolution) - origin[1]);
+        double bz = floor((z / p->z_resolution) - origin[2]);
+        if (bx < -ex - 1 || bx > XY + ex || by < -ex - 1 || by > XY + ex || bz < -ez - 1 || bz > Z + ez)
+            continue;
+        for (int64_t xi = (int64_t)bx - ex; xi < (int64_t)bx + 1 + ex; ++xi) {
+            if (xi < 0 || xi >= XY) continue;
+            for (int64_t yi = (int64_t)by - ex; yi < (int64_t)by + 1 + ex; ++yi) {
+                if (yi < 0 || yi >= XY) continue;
+                for (int64_t zi = (int64_t)bz - ez; zi < (int64_t)bz + 1 + ez; ++zi) {
+                    if (zi < 0 || zi >= Z) continue;
+                    double l0 = (x / p->xy_resolution) - origin[0] - (double)xi;
+                    double l1 = (y / p->xy_resolution) - origin[1] - (double)yi;
+                    double l2 = (z / p->z_resolution) - origin[2] - (double)zi;
+                    int32_t id = index_map[ref_lin(xi, yi, zi, XY)];
+                    if (id < 0) continue;
+                    double *m = metrics + 10 * (int64_t)id;
+                    m[3] += (l0 - m[0]) * (l0 - m[0]);
+                    m[4] += (l0 - m[0]) * (l1 - m[1]);
+                    m[5] += (l0 - m[0]) * (l2 - m[2]);
+                    m[6] += (l1 - m[1]) * (l1 - m[1]);
+                    m[7] += (l1 - m[1]) * (l2 - m[2]);
+                    m[8] += (l2 - m[2]) * (l2 - m[2]);
+                }
+            }
+        }
+    }
+    /* __normalize_covariance, gvom.py:1376-1387 */
+    for (int64_t c = 0; c < cells; ++c) {
+        double *m = metrics + 10 * c;
+        for (int j = 0; j < 6; ++j) m[j + 3] = (m[9] <= 0) ? 0.0 : m[j + 3] / m[9];
+    }
+
+    /* __calculate_min_height, gvom.py:1390-1421 */
+    for (int64_t i = 0; i < n; ++i) {
+        if (!valid[i]) continue;
+        double x = ld(pts_world, dt, 3 * i), y = ld(pts_world, dt, 3 * i + 1),
+               z = ld(pts_world, dt, 3 * i + 2);
+        double xi = floor((x / p->xy_resolution) - origin[0]);
+        if (xi < 0 || xi >= (double)XY) continue;
+        double yi = floor((y / p->xy_resolution) - origin[1]);
+        if (yi < 0 || yi >= (double)XY) continue;
+        double zi = floor((z / p->z_resolution) - origin[2]);
+        if (zi < 0 || zi >= (double)Z) continue;
+        double l2 = (z / p->z_resolution) - origin[2] - zi;
+        int32_t id = index_map[ref_lin((int64_t)xi, (int64_t)yi, (int64_t)zi, XY)];
+        min_h[id] = py_minf(min_h[id], (float)l2);
+    }
+    return cells;
+}
+
+/* ------------------------------------------------------------------------- */
+/* combine_maps (gvom.py:222-393)                                            */
+/* ------------------------------------------------------------------------- */
+
+/* __combine_indices (gvom.py:1010-1035) and __combine_old_indices (1038-1063).
+ * New ids are taken from *counter in reference linear voxel order. */
+static void combine_indices(const orc_params *p, const double co[3], int32_t *comb,
+                            const int32_t *old, const double oo[3], int64_t *counter, int is_old) {
+    const int64_t XY = p->xy_size, Z = p->z_size;
+    double dx = co[0] - oo[0], dy = co[1] - oo[1], dz = co[2] - oo[2];
+    for (int64_t z = 0; z < Z; ++z)
+        for (int64_t y = 0; y < XY; ++y)
+            for (int64_t x = 0; x < XY; ++x) {
+                double ux = (double)x + dx, uy = (double)y + dy, uz = (double)z + dz;
+                if (ux >= (double)XY || uy >= (double)XY || uz >= (double)Z || ux < 0 || uy < 0 || uz < 0)
+                    continue;
+                int64_t v = ref_lin(x, y, z, XY);
+                int64_t u = ref_lin((int64_t)ux, (int64_t)uy, (int64_t)uz, XY);
+                int32_t o = old[u], c = comb[v];
+                int occ_ok = is_old ? (c <= -1 && c >= -11) : (c <= -1);
+                if (o >= 0 && occ_ok) {
+                    comb[v] = (int32_t)(*counter);
+                    *counter += 1;
+                } else if (o < -1 && c <= -1) {
+                    comb[v] = (int32_t)(uint32_t)((int64_t)c + (int64_t)o + 1);
+                }
+            }
+}
+
+/* __combine_metrics, gvom.py:889-980, with Numba's typing: products of two
+ * combined (f32) entries are f32; anything touching an f64 operand is f64.
+ * OT is the old record's metric type (double for a scan slot, float for the
+ * previous combined map). */
+#define MERGE_BODY(OT)                                                                        \
+    {                                                                                         \
+        float *cm = cmet + 10 * (int64_t)ci;                                                  \
+        const OT *om = omet + 10 * (int64_t)oi;                                               \
+        /* pooled means */                                                                    \
+        __typeof__((float)0 * (OT)0) mx = (cm[0] * cm[9] + om[0] * om[9]) / (cm[9] + om[9]);  \
+        __typeof__((float)0 * (OT)0) my = (cm[1] * cm[9] + om[1] * om[9]) / (cm[9] + om[9]);  \
+        __typeof__((float)0 * (OT)0) mz = (cm[2] * cm[9] + om[2] * om[9]) / (cm[9] + om[9]);  \
+        float c3 = (float)((cm[9] * cm[3] + om[9] * om[3] + cm[9] * (cm[0] - mx) * (cm[0] - mx) + \
+                            om[9] * (om[0] - mx) * (om[0] - mx)) / (cm[9] + om[9]));          \
+        float c4 = (float)((cm[9] * cm[4] + om[9] * om[4] + cm[9] * (cm[0] - mx) * (cm[1] - my) + \
+                            om[9] * (om[0] - mx) * (om[1] - my)) / (cm[9] + om[9]));          \
+        float c5 = (float)((cm[9] * cm[5] + om[9] * om[5] + cm[9] * (cm[0] - mx) * (cm[2] - mz) + \
+                            om[9] * (om[0] - mx) * (om[2] - mz)) / (cm[9] + om[9]));          \
+        float c6 = (float)((cm[9] * cm[6] + om[9] * om[6] + cm[9] * (cm[1] - my) * (cm[1] - my) + \
+                            om[9] * (om[1] - my) * (om[1] - my)) / (cm[9] + om[9]));          \
+        float c7 = (float)((cm[9] * cm[7] + om[9] * om[7] + cm[9] * (cm[1] - my) * (cm[2] - mz) + \
+                            om[9] * (om[1] - my) * (om[2] - mz)) / (cm[9] + om[9]));          \
+        float c8 = (float)((cm[9] * cm[8] + om[9] * om[8] + cm[9] * (cm[2] - mz) * (cm[2] - mz) + \
+                            om[9] * (om[2] - mz) * (om[2] - mz)) / (cm[9] + om[9]));          \
+        cm[3] = c3; cm[4] = c4; cm[5] = c5; cm[6] = c6; cm[7] = c7; cm[8] = c8;               \
+        cm[0] = (float)mx; cm[1] = (float)my; cm[2] = (float)mz;                              \
+        cm[9] = (float)(cm[9] + om[9]);                                                       \
+        chit[ci] = (int32_t)((uint32_t)chit[ci] + (uint32_t)ohit[oi]);                        \
+        ctot[ci] = (int32_t)((uint32_t)ctot[ci] + (uint32_t)otot[oi]);                        \
+        cmh[ci] = py_minf(cmh[ci], omh[oi]);                                                  \
+    }
+
+static void combine_metrics_d(const orc_params *p, const double co[3], const int32_t *comb,
+                              float *cmet, int32_t *chit, int32_t *ctot, float *cmh,
+                              const int32_t *old, const double oo[3], const double *omet,
+                              const int32_t *ohit, const int32_t *otot, const float *omh) {
+    const int64_t XY = p->xy_size, Z = p->z_size;
+    double dx = co[0] - oo[0], dy = co[1] - oo[1], dz = co[2] - oo[2];
+    for (int64_t z = 0; z < Z; ++z)
+        for (int64_t y = 0; y < XY; ++y)
+            for (int64_t x = 0; x < XY; ++x) {
+                double ux = (double)x + dx, uy = (double)y + dy, uz = (double)z + dz;
+                if (ux >= (double)XY || uy >= (double)XY || uz >= (double)Z || ux < 0 || uy < 0 || uz < 0)
+                    continue;
+                int32_t ci = comb[ref_lin(x, y, z, XY)];
+                int32_t oi = old[ref_lin((int64_t)ux, (int64_t)uy, (int64_t)uz, XY)];
+                if (ci < 0 || oi < 0) continue;
+                MERGE_BODY(double)
+            }
+}
+
+static void combine_metrics_f(const orc_params *p, const double co[3], const int32_t *comb,
+                              float *cmet, int32_t *chit, int32_t *ctot, float *cmh,
+                              const int32_t *old, const double oo[3], const float *omet,
+                              const int32_t *ohit, const int32_t *otot, const float *omh) {
+    const int64_t XY = p->xy_size, Z = p->z_size;
+    double dx = co[0] - oo[0], dy = co[1] - oo[1], dz = co[2] - oo[2];
+    for (int64_t z = 0; z < Z; ++z)
+        for (int64_t y = 0; y < XY; ++y)
+            for (int64_t x = 0; x < XY; ++x) {
+                double ux = (double)x + dx, uy = (double)y + dy, uz = (double)z + dz;
+                if (ux >= (double)XY || uy >= (double)XY || uz >= (double)Z || ux < 0 || uy < 0 || uz < 0)
+                    continue;
+                int32_t ci = comb[ref_lin(x, y, z, XY)];
+                int32_t oi = old[ref_lin((int64_t)ux, (int64_t)uy, (int64_t)uz, XY)];
+                if (ci < 0 || oi < 0) continue;
+                MERGE_BODY(float)
+            }
+}
+
+/* __calculate_eigenvalues, gvom.py:1424-1487: f32 loads, f64 math, f32 stores. */
+static void eigenvalues(const float *met, float *ev, int64_t cells) {
+    const double PI = 3.141592653589793;
+    for (int64_t i = 0; i < cells; ++i) {
+        const float *m = met + 10 * i;
+        float xx = m[3], xy = m[4], xz = m[5], yy = m[6], yz = m[7], zz = m[8];
+        float p1 = xy * xy + xz * xz + yz * yz;
+        double q = (double)(xx + yy + zz) / 3.0;
+        float *e = ev + 3 * i;
+        if (p1 == 0) {
+            e[0] = py_maxf(xx, py_maxf(yy, zz));
+            e[2] = py_minf(xx, py_minf(yy, zz));
+            e[1] = (float)(3.0 * q - (double)e[0] - (double)e[2]);
+        } else {
+            double p2 = ((double)xx - q) * ((double)xx - q) + ((double)yy - q) * ((double)yy - q) +
+                        ((double)zz - q) * ((double)zz - q) + 2.0 * (double)p1;
+            double pp = sqrt(p2 / 6.0);
+            double B0 = ((double)xx - q) / pp, B1 = (double)xy / pp, B2 = (double)xz / pp;
+            double B3 = ((double)yy - q) / pp, B4 = (double)yz / pp, B5 = ((double)zz - q) / pp;
+            double r = B0 * (B3 * B5 - B4 * B4) - B1 * (B1 * B5 - B4 * B2) + B2 * (B1 * B4 - B3 * B2);
+            r = r / 2.0;
+            double phi;
+            if (r <= -1) phi = PI / 3.0;
+            else if (r >= 1) phi = 0.0;
+            else phi = acos(r) / 3.0;
+            e[0] = (float)(q + 2.0 * pp * cos(phi));
+            e[2] = (float)(q + 2.0 * pp * cos(phi + (2.0 * PI / 3.0)));
+            e[1] = (float)(3.0 * q - (double)e[0] - (double)e[2]);
+        }
+    }
+}
+
+typedef struct {
+    const int32_t *index_map;
+    const int32_t *hit, *total;
+    const float *min_h;
+    const void *metrics; /* double for slots, float for the previous combined map */
+    const double *origin;
+} orc_source;
+
+/*
+ * combine_maps body.  slots[0..nslots) are the filled ring slots in slot order;
+ * last (may be NULL) is the previous combined map.  co = combined origin (the
+ * newest slot's origin).  ego = latest ego position (gvom.py:333-336).
+ * Outputs (capacity cap for the compact arrays): comb index map (reference
+ * layout/encoding), hit, total, min_h, metrics f32[cap][10], eigen f32[cap][3],
+ * and the f64/int32 2D maps, all [x, y] C-order.
+ * Returns the combined cell count, or -1 when it exceeds cap.
+ */
+int64_t orc_combine(const orc_params *p, int32_t nslots, const orc_source *slots,
+                    const orc_source *last, const double co[3], const double ego[3],
+                    int32_t *comb, int32_t *chit, int32_t *ctot, float *cmh, float *cmet,
+                    float *ev, int64_t cap,
+                    double *height, double *inferred, double *slope_x, double *slope_y,
+                    double *rough, double *delta, int32_t *pos, int32_t *neg, int32_t *vis) {
+    const int64_t XY = p->xy_size, Z = p->z_size, V = XY * XY * Z;
+    int64_t counter = 0;
+    for (int64_t v = 0; v < V; ++v) comb[v] = -1;
+    /* combine_indices per slot, then against the last combined map (gvom.py:242-257) */
+    for (int s = 0; s < nslots; ++s) combine_indices(p, co, comb, slots[s].index_map, slots[s].origin, &counter, 0);
+    if (last) combine_indices(p, co, comb, last->index_map, last->origin, &counter, 1);
+    int64_t cells = counter;
+    if (cells > cap) return -1;
+    for (int64_t c = 0; c < cells; ++c) {
+        chit[c] = 0; ctot[c] = 0; cmh[c] = 1.0f;
+        for (int j = 0; j < 10; ++j) cmet[10 * c + j] = 0.0f;
+    }
+    for (int s = 0; s < nslots; ++s)
+        combine_metrics_d(p, co, comb, cmet, chit, ctot, cmh, slots[s].index_map, slots[s].origin,
+                          (const double *)slots[s].metrics, slots[s].hit, slots[s].total, slots[s].min_h);
+    if (last)
+        combine_metrics_f(p, co, comb, cmet, chit, ctot, cmh, last->index_map, last->origin,
+                          (const float *)last->metrics, last->hit, last->total, last->min_h);
+    eigenvalues(cmet, ev, cells);
+
+    const double xyr = p->xy_resolution, zr = p->z_resolution;
+    /* __make_height_map (561-576) and __make_inferred_height_map (580-590) */
+    for (int64_t x = 0; x < XY; ++x)
+        for (int64_t y = 0; y < XY; ++y) {
+            int64_t c = x * XY + y;
+            height[c] = -1000.0;
+            inferred[c] = -1000.0;
+            double xp = ((co[0] + (double)x) * xyr) - ego[0];
+            double yp = ((co[1] + (double)y) * xyr) - ego[1];
+            if (xp * xp + yp * yp <= p->robot_radius * p->robot_radius)
+                height[c] = ego[2] - p->ground_to_lidar_height;
+            for (int64_t z = 0; z < Z; ++z) {
+                int32_t id = comb[ref_lin(x, y, z, XY)];
+                if (id >= 0) {
+                    height[c] = (((double)cmh[id] + (double)z) + co[2]) * zr;
+                    break;
+                }
+            }
+            for (int64_t z = 0; z < Z; ++z) {
+                int32_t id = comb[ref_lin(x, y, z, XY)];
+                if (id < -1) {
+                    inferred[c] = ((double)z + co[2]) * zr;
+                    break;
+                }
+            }
+        }
+
+    /* __calculate_slope (718-805); defaults rough -1, slopes 0 (gvom.py:342-349) */
+    for (int64_t x0 = 0; x0 < XY; ++x0)
+        for (int64_t y0 = 0; y0 < XY; ++y0) {
+            int64_t c0 = x0 * XY + y0;
+            rough[c0] = -1.0;
+            slope_x[c0] = 0.0;
+            slope_y[c0] = 0.0;
+            int64_t xlo = x0 - 1 < 0 ? 0 : x0 - 1, xhi = x0 + 2 > XY ? XY : x0 + 2;
+            int64_t ylo = y0 - 1 < 0 ? 0 : y0 - 1, yhi = y0 + 2 > XY ? XY : y0 + 2;
+            int ngood = 0;
+            for (int64_t x = xlo; x < xhi; ++x)
+                for (int64_t y = ylo; y < yhi; ++y)
+                    if (height[x * XY + y] > -1000) ++ngood;
+            if (ngood < 3) continue;
+            double px[9], py[9], pz[9];
+            int i = 0;
+            double mx = 0.0, my = 0.0, mz = 0.0;
+            for (int64_t x = xlo; x < xhi; ++x)
+                for (int64_t y = ylo; y < yhi; ++y) {
+                    double h = height[x * XY + y];
+                    if (h > -1000) {
+                        px[i] = (double)x * xyr;
+                        py[i] = (double)y * xyr;
+                        pz[i] = h;
+                        mx += px[i];
+                        my += py[i];
+                        mz += pz[i];
+                        ++i;
+                    }
+                }
+            mx /= (double)i;
+            my /= (double)i;
+            mz /= (double)i;
+            double xx = 0.0, xy = 0.0, xz = 0.0, yy = 0.0, yz = 0.0;
+            for (int k = 0; k < ngood; ++k) {
+                xx += (px[k] - mx) * (px[k] - mx);
+                xy += (px[k] - mx) * (py[k] - my);
+                xz += (px[k] - mx) * (pz[k] - mz);
+                yy += (py[k] - my) * (py[k] - my);
+                yz += (py[k] - my) * (pz[k] - mz);
+            }
+            double det = xx * yy - xy * xy;
+            if (det == 0.0) continue;
+            double a0 = (yy * xz - xy * yz) / det;
+            double a1 = (xx * yz - xy * xz) / det;
+            double m = sqrt(a0 * a0 + a1 * a1 + 1.0);
+            a0 /= m;
+            a1 /= m;
+            double err = 0.0;
+            for (int k = 0; k < ngood; ++k) {
+                double e = (pz[k] - mz) - (a0 * (px[k] - mx) + a1 * (py[k] - my));
+                err += e * e;
+            }
+            err /= (double)ngood;
+            if (err > 0) err = log(err);
+            rough[c0] = err;
+            slope_x[c0] = atan2(a0, 1.0 / m);
+            slope_y[c0] = atan2(a1, 1.0 / m);
+        }
+
+    /* __guess_height (593-714), with its loop condition and y_nh gating as written */
+    for (int64_t x0 = 0; x0 < XY; ++x0)
+        for (int64_t y0 = 0; y0 < XY; ++y0) {
+            int64_t c0 = x0 * XY + y0;
+            delta[c0] = 0.0;
+            if (height[c0] > -1000) continue;
+            if (inferred[c0] == -1000.0) continue;
+            int xpd = 0, xnd = 0, ypd = 0, ynd = 0;
+            int64_t xp = x0, xn = x0, yp = y0, yn = y0;
+            double xph = -1000, xnh = -1000, yph = -1000, ynh = -1000;
+            int i = 0;
+            while (i < 15 && !(xnd && xnd && ypd && ynd)) {
+                xp += 1; xn -= 1; yp += 1; yn -= 1;
+                i += 1;
+                if (!xpd) {
+                    if (xp < XY) {
+                        for (int64_t dy = -i; dy < i; ++dy) {
+                            if (y0 + dy >= XY || y0 + dy < 0) continue;
+                            if (height[xp * XY + y0 + dy] > -1000) { xph = height[xp * XY + y0 + dy]; xpd = 1; break; }
+                        }
+                    } else xpd = 1;
+                }
+                if (!xnd) {
+                    if (xn >= 0) {
+                        for (int64_t dy = -i + 1; dy < i + 1; ++dy) {
+                            if (y0 + dy >= XY || y0 + dy < 0) continue;
+                            if (height[xn * XY + y0 + dy] > -1000) { xnh = height[xn * XY + y0 + dy]; xnd = 1; break; }
+                        }
+                    } else xnd = 1;
+                }
+                if (!ypd) {
+                    if (yp < XY) {
+                        for (int64_t dx = -i + 1; dx < i + 1; ++dx) {
+                            if (x0 + dx >= XY || x0 + dx < 0) continue;
+                            if (height[(x0 + dx) * XY + yp] > -1000) { yph = height[(x0 + dx) * XY + yp]; ypd = 1; break; }
+                        }
+                    } else ypd = 1;
+                }
+                if (!ynd) {
+                    if (yn >= 0) {
+                        for (int64_t dx = -i; dx < i; ++dx) {
+                            if (x0 + dx >= XY || x0 + dx < 0) continue;
+                            if (height[(x0 + dx) * XY + yn] > -1000) { ynh = height[(x0 + dx) * XY + yn]; ynd = 1; break; }
+                        }
+                    } else ynd = 1;
+                }
+            }
+            double min_h = 1000.0, max_h = inferred[c0];
+            if (xph > -1000) { min_h = py_min(xph, min_h); max_h = py_max(xph, max_h); }
+            if (xnh > -1000) { min_h = py_min(xnh, min_h); max_h = py_max(xnh, max_h); }
+            if (yph > -1000) { min_h = py_min(yph, min_h); max_h = py_max(yph, max_h); }
+            if (xnh > -1000) { min_h = py_min(ynh, min_h); max_h = py_max(ynh, max_h); }
+            double dh = max_h - min_h;
+            if (dh > 0) delta[c0] = dh;
+        }
+
+    /* __make_positive_obstacle_map (516-555), negative (506-512), visibility (445-452) */
+    for (int64_t x = 0; x < XY; ++x)
+        for (int64_t y = 0; y < XY; ++y) {
+            int64_t c = x * XY + y;
+            pos[c] = 0;
+            neg[c] = (delta[c] > p->negative_obstacle_threshold) ? 100 : 0;
+            vis[c] = (height[c] > -1000) ? 1 : 0;
+            if (sqrt(slope_x[c] * slope_x[c] + slope_y[c] * slope_y[c]) >= p->slope_obstacle_threshold) {
+                pos[c] = 100;
+                continue;
+            }
+            double lo_h = height[c] + p->positive_obstacle_threshold;
+            double hi_h = height[c] + p->robot_height;
+            double flo = floor((lo_h / zr) - co[2]);
+            double fhi = floor((hi_h / zr) - co[2]);
+            /* int(floor(.)) then range checks; values far outside int range fail the checks */
+            if (!(flo + 1 >= 0 && flo + 1 < (double)Z)) continue;
+            if (!(fhi >= 0 && fhi < (double)Z)) continue;
+            int64_t lo = (int64_t)flo + 1, hi = (int64_t)fhi;
+            double density = 0.0, nn = 0.0;
+            for (int64_t z = lo; z < hi + 1; ++z) {
+                int32_t id = comb[ref_lin(x, y, z, XY)];
+                if (id >= 0 && chit[id] > 10) {
+                    nn += (double)ctot[id];
+                    density += (double)chit[id];
+                }
+            }
+            if (nn > 0.0) density /= nn;
+            pos[c] = (int32_t)(density * 100.0);
+        }
+    return cells;
+}
