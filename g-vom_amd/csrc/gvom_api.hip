@@ -13,7 +13,6 @@
 
 #include "gvom_kernels.h"
 
-hipError_t launch_scan_excl_u32(uint32_t *data, int64_t n, int32_t *total, hipStream_t s);
 hipError_t launch_clear_slots(int32_t *slot_tables, const DevState *st, int S, const Geo &g, hipStream_t s);
 
 namespace {
@@ -45,6 +44,7 @@ struct gvom_handle {
            *delta = nullptr;
     int32_t *pos = nullptr, *neg = nullptr, *vis = nullptr, *density = nullptr, *col_count = nullptr;
     int32_t *export_buf = nullptr;
+    int32_t *chunk_tot = nullptr;
 
     int64_t ptcap = 0;
     void *pts_in = nullptr;
@@ -53,7 +53,7 @@ struct gvom_handle {
     uint32_t *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr, *hist = nullptr, *blk = nullptr,
              *seg_key = nullptr;
     int32_t *seg_start = nullptr;
-    Moment *mom = nullptr;
+    Moment *mom = nullptr, *part = nullptr;
     ScanDesc *sd = nullptr;
 
     // host mirror of the ring state (gvom.py buffer_index / last_buffer_index / slots)
@@ -82,8 +82,8 @@ struct gvom_handle {
 static void free_all(gvom_handle *h) {
     void *ptrs[] = {h->st, h->slot_tables, h->slot_recs, h->comb_tables, h->comb_recs, h->height, h->inferred,
                     h->slope_x, h->slope_y, h->rough, h->delta, h->pos, h->neg, h->vis, h->density,
-                    h->col_count, h->export_buf, h->pts_in, h->world, h->valid, h->k0, h->v0, h->k1, h->v1,
-                    h->hist, h->blk, h->seg_key, h->seg_start, h->mom, h->sd};
+                    h->col_count, h->chunk_tot, h->export_buf, h->pts_in, h->world, h->valid, h->k0, h->v0, h->k1, h->v1,
+                    h->hist, h->blk, h->seg_key, h->seg_start, h->mom, h->part, h->sd};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : h->ev_pool)
@@ -174,7 +174,8 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     CKC(dalloc(&h->neg, X));
     CKC(dalloc(&h->vis, X));
     CKC(dalloc(&h->density, X));
-    CKC(dalloc(&h->col_count, X + 1));
+    CKC(dalloc(&h->col_count, X + 16));
+    CKC(dalloc(&h->chunk_tot, col_chunks(X) + 1));
     CKC(dalloc(&h->sd, GVOM_MAX_BATCH));
 #undef CKC
     *out = h;
@@ -229,7 +230,7 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
         }
         CKH(h, hipStreamSynchronize(h->stream));
         void *ptrs[] = {h->pts_in, h->world, h->valid, h->k0, h->v0, h->k1, h->v1, h->hist, h->blk,
-                        h->seg_key, h->seg_start, h->mom};
+                        h->seg_key, h->seg_start, h->mom, h->part};
         for (void *q : ptrs)
             if (q) CKH(h, hipFree(q));
         CKH(h, hipMalloc(&h->pts_in, (size_t)pc * 3 * sizeof(double)));
@@ -244,6 +245,7 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
         CKH(h, dalloc(&h->seg_key, pc));
         CKH(h, dalloc(&h->seg_start, pc + 1));
         CKH(h, dalloc(&h->mom, pc));
+        CKH(h, dalloc(&h->part, pc));
         h->ptcap = pc;
     }
     return GVOM_OK;
@@ -338,7 +340,7 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     int rc = ensure_capacity(h, max_scan, NT);
     if (rc) return rc;
     if (!h->slot_recs || !h->pts_in || !h->world || !h->valid || !h->k0 || !h->v0 || !h->k1 || !h->v1 ||
-        !h->hist || !h->blk || !h->seg_key || !h->seg_start || !h->mom || h->ptcap < NT || h->ptcap < 1) {
+        !h->hist || !h->blk || !h->seg_key || !h->seg_start || !h->mom || !h->part || h->ptcap < NT || h->ptcap < 1) {
         h->err = "internal: work buffers not allocated";
         return GVOM_ESTATE;
     }
@@ -380,9 +382,9 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     CKH(h, radix_sort(h->k0, h->v0, h->k1, h->v1, NT, g.VB + sbits, h->hist, &sk, &sv, st));
     CKH(h, launch_segments(sk, NT, h->seg_start, h->seg_key, h->blk, S, g, h->st, st));
     stage_event(h, 3);
-    CKH(h, launch_moments(dtype, h->world, sv, h->seg_start, h->seg_key, NT, h->sd, S, g, h->st, h->slot_tables,
-                          h->slot_recs, h->ncap, h->mom, st));
-    CKH(h, launch_metrics(h->seg_key, h->mom, NT, S, g, h->st, h->slot_tables, h->slot_recs, h->ncap, st));
+    CKH(h, launch_moments(dtype, h->world, sk, sv, h->seg_start, h->seg_key, NT, h->sd, g, h->st, h->slot_tables,
+                          h->slot_recs, h->ncap, h->part, h->mom, st));
+    CKH(h, launch_metrics(h->seg_key, h->mom, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, st));
     CKH(h, launch_commit_process(h->st, S, g.B, st));
     stage_event(h, 4);
 
@@ -432,13 +434,13 @@ static int combine_impl(gvom_handle *h) {
     stage_event(h, 5);
     CKH(h, launch_comb_tables(g, h->st, h->slot_tables, h->comb_tables, h->col_count, st));
     stage_event(h, 6);
-    CKH(h, launch_scan_excl_u32((uint32_t *)h->col_count, g.X, &h->st->comb_total, st));
+    CKH(h, launch_scan_columns(h->col_count, g.X, h->chunk_tot, st));
     CKH(h, launch_comb_records(g, h->st, h->slot_tables, h->slot_recs, h->ncap, h->comb_tables, h->comb_recs,
-                               h->ccap, h->col_count, h->height, h->inferred, h->density, st));
+                               h->ccap, h->col_count, h->chunk_tot, h->height, h->inferred, h->density, st));
     stage_event(h, 7);
     CKH(h, launch_maps2d(g, h->height, h->inferred, h->density, h->slope_x, h->slope_y, h->rough, h->delta,
                          h->pos, h->neg, h->vis, st));
-    CKH(h, launch_commit_combine(h->st, st));
+    CKH(h, launch_commit_combine(h->st, h->chunk_tot, g.X, st));
     stage_event(h, 8);
     h->parity = cur;
     h->have_last = 1;

@@ -22,7 +22,6 @@
 
 #include "gvom_kernels.h"
 
-hipError_t launch_scan_excl_u32(uint32_t *data, int64_t n, int32_t *total, hipStream_t s);
 
 namespace {
 
@@ -51,6 +50,10 @@ __device__ __forceinline__ SrcCol source_column(const int32_t *table, const doub
 }
 
 // ------------------------------------------------------------------------------------
+// The slot loop is unrolled in groups of 8 so that a wavefront has all of a group's
+// source loads in flight before it uses the first value.
+constexpr int SRC_GROUP = 8;
+
 __global__ __launch_bounds__(256) void k_comb_tables(Geo g, DevState *__restrict__ st,
                                                      const int32_t *__restrict__ slot_tables,
                                                      int32_t *__restrict__ comb_tables,
@@ -75,36 +78,89 @@ __global__ __launch_bounds__(256) void k_comb_tables(Geo g, DevState *__restrict
     uint32_t count = 0;
     for (int z0 = 0; z0 < g.Z; z0 += 64) {
         const int z = z0 + lane;
-        uint32_t mask = 0, P = 0;
-        if (z < g.Z) {
-            for (int s = 0; s < g.B; ++s) {
-                if (!(filled & (1 << s))) continue;
-                SrcCol sc = source_column(slot_tables + (int64_t)s * g.V, co, st->slot_origin[s], x, y, g);
-                if (!sc.col) continue;
-                int zs = z + sc.dz;
-                if (zs < 0 || zs >= g.Z) continue;
-                int32_t v = sc.col[zs];
-                if (v < 0) mask |= 1u << s;
-                else P += (uint32_t)v;
+        const bool zin = z < g.Z;
+        // previous combined map, loaded early
+        int32_t vl = 0;
+        bool okl = false;
+        if (lsrc.col && zin) {
+            int zl = z + lsrc.dz;
+            if (zl >= 0 && zl < g.Z) {
+                vl = lsrc.col[zl];
+                okl = true;
             }
-            if (lsrc.col) {
-                int zl = z + lsrc.dz;
-                if (zl >= 0 && zl < g.Z) {
-                    int32_t v = lsrc.col[zl];
-                    if (mask) {
-                        if (v < 0) mask |= 1u << LAST_BIT;
-                    } else if (v < 0) {
-                        if (P <= 10u) mask = 1u << LAST_BIT;   // carry-over (gvom.py:1058)
-                    } else {
-                        P += (uint32_t)v;
+        }
+        uint32_t mask = 0, P = 0;
+        for (int s0 = 0; s0 < g.B; s0 += SRC_GROUP) {
+            int32_t v[SRC_GROUP];
+            bool ok[SRC_GROUP];
+#pragma unroll
+            for (int k = 0; k < SRC_GROUP; ++k) {
+                const int s = s0 + k;
+                v[k] = 0;
+                ok[k] = false;
+                if (s < g.B && ((filled >> s) & 1)) {
+                    SrcCol sc = source_column(slot_tables + (int64_t)s * g.V, co, st->slot_origin[s], x, y, g);
+                    const int zs = z + sc.dz;
+                    if (sc.col && zin && zs >= 0 && zs < g.Z) {
+                        v[k] = sc.col[zs];
+                        ok[k] = true;
                     }
                 }
             }
-            out[z] = mask ? (int32_t)(GVOM_OCC | mask) : (int32_t)P;
+#pragma unroll
+            for (int k = 0; k < SRC_GROUP; ++k) {
+                if (!ok[k]) continue;
+                if (v[k] < 0) mask |= 1u << (s0 + k);
+                else P += (uint32_t)v[k];
+            }
         }
+        if (okl) {
+            if (mask) {
+                if (vl < 0) mask |= 1u << LAST_BIT;
+            } else if (vl < 0) {
+                if (P <= 10u) mask = 1u << LAST_BIT;   // carry-over (gvom.py:1058)
+            } else {
+                P += (uint32_t)vl;
+            }
+        }
+        if (zin) out[z] = mask ? (int32_t)(GVOM_OCC | mask) : (int32_t)P;
         count += __popcll(__ballot(mask != 0));
     }
     if (lane == 0) col_count[c] = (int32_t)count;
+}
+
+// ------------------------------------------------------------------------------------
+// Column id scan: blocks of 1024 threads scan COLSCAN consecutive column counts in place
+// (exclusive, int4 loads) and write their total; k_comb_records adds the totals of the
+// earlier chunks.
+constexpr int COLSCAN = 4096;
+
+__global__ __launch_bounds__(1024) void k_col_scan(int32_t *__restrict__ cnt, int64_t X,
+                                                   int32_t *__restrict__ chunk_tot) {
+    __shared__ int32_t wsum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t i = (int64_t)blockIdx.x * COLSCAN + 4 * t;
+    int4 v = make_int4(0, 0, 0, 0);
+    if (i < X) v = *reinterpret_cast<const int4 *>(cnt + i);   // X is a multiple of 16
+    const int32_t sum = v.x + v.y + v.z + v.w;
+    int32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        int32_t u = __shfl_up(incl, off);
+        if (lane >= off) incl += u;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int32_t pre = incl - sum;
+    for (int ww = 0; ww < w; ++ww) pre += wsum[ww];
+    if (i < X) {
+        int4 o;
+        o.x = pre;
+        o.y = pre + v.x;
+        o.z = o.y + v.y;
+        o.w = o.z + v.z;
+        *reinterpret_cast<int4 *>(cnt + i) = o;
+    }
+    if (t == 1023) chunk_tot[blockIdx.x] = pre + sum;
 }
 
 // ------------------------------------------------------------------------------------
@@ -174,11 +230,19 @@ __global__ __launch_bounds__(256) void k_comb_records(Geo g, const DevState *__r
                                                       int32_t *__restrict__ comb_tables,
                                                       CombRec *__restrict__ comb_recs, int64_t ccap,
                                                       const int32_t *__restrict__ col_base,
+                                                      const int32_t *__restrict__ chunk_tot,
                                                       double *__restrict__ height, double *__restrict__ inferred,
                                                       int32_t *__restrict__ density) {
     const int lane = threadIdx.x & 63;
     const int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (c >= g.X) return;
+    // first cell id of this column: in-chunk exclusive prefix + totals of earlier chunks
+    uint32_t chunk_pre = 0;
+    {
+        const int64_t q = c / COLSCAN;
+        for (int64_t k = lane; k < q; k += 64) chunk_pre += (uint32_t)chunk_tot[k];
+        for (int off = 32; off > 0; off >>= 1) chunk_pre += __shfl_xor(chunk_pre, off);
+    }
     const int parity = st->parity, cur = 1 - parity;
     const double *co = st->comb_origin[cur];
     const int64_t x = c / g.XY, y = c % g.XY;
@@ -186,7 +250,7 @@ __global__ __launch_bounds__(256) void k_comb_records(Geo g, const DevState *__r
     CombRec *crec = comb_recs + (int64_t)cur * ccap;
     const CombRec *lrec = comb_recs + (int64_t)parity * ccap;
     const uint64_t lt = (1ull << lane) - 1ull;
-    uint32_t run = (uint32_t)col_base[c];
+    uint32_t run = chunk_pre + (uint32_t)col_base[c];
     int first_occ = -1, first_free = -1;
     float mh_first = 1.0f;
     int32_t chit[MAX_ZCH], ctot[MAX_ZCH];
@@ -288,7 +352,15 @@ __global__ __launch_bounds__(256) void k_comb_records(Geo g, const DevState *__r
 // ------------------------------------------------------------------------------------
 constexpr int TILE = 16;
 constexpr int HALO = 15;
-constexpr int LT = TILE + 2 * HALO;   // 46
+constexpr int LT = TILE + 2 * HALO;   // 46 <= 64: one 64-bit mask per tile row / column
+
+// First set bit of m within [lo, hi) (bit positions), or -1.
+__device__ __forceinline__ int first_in(uint64_t m, int lo, int hi) {
+    if (hi <= lo) return -1;
+    uint64_t r = (m >> lo);
+    if (hi - lo < 64) r &= (1ull << (hi - lo)) - 1ull;
+    return r ? lo + __builtin_ctzll(r) : -1;
+}
 
 __global__ __launch_bounds__(256) void k_maps2d(Geo g, const double *__restrict__ height,
                                                 const double *__restrict__ inferred,
@@ -298,12 +370,28 @@ __global__ __launch_bounds__(256) void k_maps2d(Geo g, const double *__restrict_
                                                 int32_t *__restrict__ pos, int32_t *__restrict__ neg,
                                                 int32_t *__restrict__ vis) {
     __shared__ double H[LT][LT];
+    __shared__ uint64_t rowm[LT];   // bit ly set when H[lx][ly] is observed (rows: fixed x)
+    __shared__ uint64_t colm[LT];   // bit lx set when H[lx][ly] is observed (columns: fixed y)
     const int XY = g.XY;
     const int bx = blockIdx.y * TILE, by = blockIdx.x * TILE;   // tile origin (x rows, y fastest)
     for (int k = threadIdx.x; k < LT * LT; k += 256) {
         int lx = k / LT, ly = k % LT;
         int gx = bx - HALO + lx, gy = by - HALO + ly;
         H[lx][ly] = (gx >= 0 && gx < XY && gy >= 0 && gy < XY) ? height[(int64_t)gx * XY + gy] : -1000.0;
+    }
+    __syncthreads();
+    if (threadIdx.x < LT) {
+        const int lx = threadIdx.x;
+        uint64_t m = 0;
+        for (int ly = 0; ly < LT; ++ly)
+            if (H[lx][ly] > -1000) m |= 1ull << ly;
+        rowm[lx] = m;
+    } else if (threadIdx.x >= 64 && threadIdx.x < 64 + LT) {
+        const int ly = threadIdx.x - 64;
+        uint64_t m = 0;
+        for (int lx = 0; lx < LT; ++lx)
+            if (H[lx][ly] > -1000) m |= 1ull << lx;
+        colm[ly] = m;
     }
     __syncthreads();
     const int tx = threadIdx.x / TILE, ty = threadIdx.x % TILE;
@@ -377,52 +465,42 @@ __global__ __launch_bounds__(256) void k_maps2d(Geo g, const double *__restrict_
         }
     }
 
-    // __guess_height (gvom.py:593-714), its loop condition and the y_nh gating as written
+    // __guess_height (gvom.py:593-714), its loop condition and the y_nh gating as written.
+    // Each one-sided edge scan returns the first observed cell in scan order, i.e. the
+    // lowest set bit of the row / column mask restricted to the scanned range (cells off
+    // the map are never observed, matching the reference's bounds `continue`).
     double dh_out = 0.0;
     const double h0 = HAT(x0, y0);
     const double inf0 = inferred[c0];
     if (!(h0 > -1000) && !(inf0 == -1000.0)) {
+        const int lx0 = x0 - bx + HALO, ly0 = y0 - by + HALO;
         bool xpd = false, xnd = false, ypd = false, ynd = false;
-        int xp = x0, xn = x0, yp = y0, yn = y0;
         double xph = -1000, xnh = -1000, yph = -1000, ynh = -1000;
         int i = 0;
         while (i < 15 && !(xnd && xnd && ypd && ynd)) {
-            xp += 1; xn -= 1; yp += 1; yn -= 1;
             i += 1;
             if (!xpd) {
-                if (xp < XY) {
-                    for (int dy = -i; dy < i; ++dy) {
-                        if (y0 + dy >= XY || y0 + dy < 0) continue;
-                        double v = HAT(xp, y0 + dy);
-                        if (v > -1000) { xph = v; xpd = true; break; }
-                    }
+                if (x0 + i < XY) {   // row x0+i, dy in [-i, i)
+                    int f = first_in(rowm[lx0 + i], ly0 - i, ly0 + i);
+                    if (f >= 0) { xph = H[lx0 + i][f]; xpd = true; }
                 } else xpd = true;
             }
             if (!xnd) {
-                if (xn >= 0) {
-                    for (int dy = -i + 1; dy < i + 1; ++dy) {
-                        if (y0 + dy >= XY || y0 + dy < 0) continue;
-                        double v = HAT(xn, y0 + dy);
-                        if (v > -1000) { xnh = v; xnd = true; break; }
-                    }
+                if (x0 - i >= 0) {   // row x0-i, dy in (-i, i]
+                    int f = first_in(rowm[lx0 - i], ly0 - i + 1, ly0 + i + 1);
+                    if (f >= 0) { xnh = H[lx0 - i][f]; xnd = true; }
                 } else xnd = true;
             }
             if (!ypd) {
-                if (yp < XY) {
-                    for (int dx = -i + 1; dx < i + 1; ++dx) {
-                        if (x0 + dx >= XY || x0 + dx < 0) continue;
-                        double v = HAT(x0 + dx, yp);
-                        if (v > -1000) { yph = v; ypd = true; break; }
-                    }
+                if (y0 + i < XY) {   // column y0+i, dx in (-i, i]
+                    int f = first_in(colm[ly0 + i], lx0 - i + 1, lx0 + i + 1);
+                    if (f >= 0) { yph = H[f][ly0 + i]; ypd = true; }
                 } else ypd = true;
             }
             if (!ynd) {
-                if (yn >= 0) {
-                    for (int dx = -i; dx < i; ++dx) {
-                        if (x0 + dx >= XY || x0 + dx < 0) continue;
-                        double v = HAT(x0 + dx, yn);
-                        if (v > -1000) { ynh = v; ynd = true; break; }
-                    }
+                if (y0 - i >= 0) {   // column y0-i, dx in [-i, i)
+                    int f = first_in(colm[ly0 - i], lx0 - i, lx0 + i);
+                    if (f >= 0) { ynh = H[f][ly0 - i]; ynd = true; }
                 } else ynd = true;
             }
         }
@@ -446,10 +524,17 @@ __global__ __launch_bounds__(256) void k_maps2d(Geo g, const double *__restrict_
     vis[c0] = (h0 > -1000) ? 1 : 0;
 }
 
-__global__ void k_commit_combine(DevState *st) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One wavefront: combined cell count = sum of the column-scan chunk totals; flip parity.
+__global__ void k_commit_combine(DevState *st, const int32_t *__restrict__ chunk_tot, int64_t nchunk) {
+    if (blockIdx.x != 0) return;
+    const int lane = threadIdx.x & 63;
+    uint32_t tot = 0;
+    for (int64_t k = lane; k < nchunk; k += 64) tot += (uint32_t)chunk_tot[k];
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    if (threadIdx.x != 0) return;
     const int cur = 1 - st->parity;
-    st->comb_cells[cur] = st->comb_total;
+    st->comb_cells[cur] = (int32_t)tot;
+    st->comb_total = (int32_t)tot;
     st->parity = cur;
     st->have_last = 1;
 }
@@ -533,19 +618,20 @@ hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_ta
     return hipGetLastError();
 }
 
-hipError_t launch_scan_columns(int32_t *col_count, int64_t X, DevState *st, hipStream_t s) {
-    // reads the parity on device: the total goes to comb_cells[1 - parity]; the host mirror
-    // passes the right slot through the pointer it chooses.
-    (void)st;
-    return launch_scan_excl_u32((uint32_t *)col_count, X, nullptr, s);
+int64_t col_chunks(int64_t X) { return (X + COLSCAN - 1) / COLSCAN; }
+
+hipError_t launch_scan_columns(int32_t *col_count, int64_t X, int32_t *chunk_tot, hipStream_t s) {
+    k_col_scan<<<(unsigned)col_chunks(X), 1024, 0, s>>>(col_count, X, chunk_tot);
+    return hipGetLastError();
 }
 
 hipError_t launch_comb_records(const Geo &g, DevState *st, const int32_t *slot_tables,
                                const SlotRec *slot_recs, int64_t ncap, int32_t *comb_tables,
-                               CombRec *comb_recs, int64_t ccap, const int32_t *col_base, double *height,
-                               double *inferred, int32_t *density, hipStream_t s) {
+                               CombRec *comb_recs, int64_t ccap, const int32_t *col_base,
+                               const int32_t *chunk_tot, double *height, double *inferred, int32_t *density,
+                               hipStream_t s) {
     k_comb_records<<<nblocks(g.X, 4), 256, 0, s>>>(g, st, slot_tables, slot_recs, ncap, comb_tables, comb_recs,
-                                                   ccap, col_base, height, inferred, density);
+                                                   ccap, col_base, chunk_tot, height, inferred, density);
     return hipGetLastError();
 }
 
@@ -557,8 +643,8 @@ hipError_t launch_maps2d(const Geo &g, const double *height, const double *infer
     return hipGetLastError();
 }
 
-hipError_t launch_commit_combine(DevState *st, hipStream_t s) {
-    k_commit_combine<<<1, 64, 0, s>>>(st);
+hipError_t launch_commit_combine(DevState *st, const int32_t *chunk_tot, int64_t X, hipStream_t s) {
+    k_commit_combine<<<1, 64, 0, s>>>(st, chunk_tot, col_chunks(X));
     return hipGetLastError();
 }
 

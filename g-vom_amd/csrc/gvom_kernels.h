@@ -23,29 +23,32 @@ int64_t radix_hist_entries(int64_t n);
 // Segments of equal keys in the sorted array, per-scan bounds and slot cell counts.
 hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start, uint32_t *seg_key,
                            uint32_t *blk, int S, const Geo &g, DevState *st, hipStream_t s);
-// Per-segment moments + slot table occupancy marks + hit/total/min-height records.
-hipError_t launch_moments(int dtype, const void *world, const uint32_t *svals, const int32_t *seg_start,
-                          const uint32_t *seg_key, int64_t seg_cap, const ScanDesc *sd, int S,
-                          const Geo &g, DevState *st, int32_t *slot_tables, SlotRec *slot_recs,
-                          int64_t ncap, Moment *mom, hipStream_t s);
+// Per-segment moments (chunk partials, then per-segment sums) + slot table occupancy
+// marks + hit/total/min-height records.
+hipError_t launch_moments(int dtype, const void *world, const uint32_t *skeys, const uint32_t *svals,
+                          const int32_t *seg_start, const uint32_t *seg_key, int64_t seg_cap,
+                          const ScanDesc *sd, const Geo &g, DevState *st, int32_t *slot_tables,
+                          SlotRec *slot_recs, int64_t ncap, Moment *part, Moment *mom, hipStream_t s);
 // Neighbourhood mean/covariance per occupied voxel (gvom.py:1250-1387).
-hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t seg_cap, int S,
-                          const Geo &g, const DevState *st, const int32_t *slot_tables,
-                          SlotRec *slot_recs, int64_t ncap, hipStream_t s);
+hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t seg_cap, const Geo &g,
+                          const DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap,
+                          hipStream_t s);
 hipError_t launch_commit_process(DevState *st, int S, int B, hipStream_t s);
 
 // ---- combine stage (gvom_combine.hip) ----------------------------------------------------
 hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_tables,
                               int32_t *comb_tables, int32_t *col_count, hipStream_t s);
-hipError_t launch_scan_columns(int32_t *col_count, int64_t X, DevState *st, hipStream_t s);
+int64_t col_chunks(int64_t X);
+hipError_t launch_scan_columns(int32_t *col_count, int64_t X, int32_t *chunk_tot, hipStream_t s);
 hipError_t launch_comb_records(const Geo &g, DevState *st, const int32_t *slot_tables,
                                const SlotRec *slot_recs, int64_t ncap, int32_t *comb_tables,
                                CombRec *comb_recs, int64_t ccap, const int32_t *col_base,
-                               double *height, double *inferred, int32_t *density, hipStream_t s);
+                               const int32_t *chunk_tot, double *height, double *inferred, int32_t *density,
+                               hipStream_t s);
 hipError_t launch_maps2d(const Geo &g, const double *height, const double *inferred,
                          const int32_t *density, double *slope_x, double *slope_y, double *rough,
                          double *delta, int32_t *pos, int32_t *neg, int32_t *vis, hipStream_t s);
-hipError_t launch_commit_combine(DevState *st, hipStream_t s);
+hipError_t launch_commit_combine(DevState *st, const int32_t *chunk_tot, int64_t X, hipStream_t s);
 
 // ---- export (gvom_combine.hip) -----------------------------------------------------------
 // Internal z-contiguous table -> reference layout and encoding.

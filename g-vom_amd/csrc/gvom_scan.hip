@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
 constexpr int RS_TILE = 2048;
 
 __global__ __launch_bounds__(256) void k_rs_hist(const uint32_t *__restrict__ keys, int64_t n, int shift,
-                                                 uint32_t *__restrict__ hist, int nblk) {
+                                                 uint32_t *__restrict__ hist) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -252,57 +252,40 @@ __global__ __launch_bounds__(256) void k_rs_hist(const uint32_t *__restrict__ ke
         if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
-}
-
-// Exclusive scan in place of data[0..n), one block of 1024 threads; total -> *total.
-__global__ __launch_bounds__(1024) void k_scan_excl(uint32_t *__restrict__ data, int64_t n,
-                                                    int32_t *__restrict__ total) {
-    __shared__ uint32_t wsum[16];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    int64_t per = (n + 1023) / 1024;
-    int64_t b = t * per, e = b + per < n ? b + per : n;
-    uint32_t sum = 0;
-    for (int64_t i = b; i < e; ++i) sum += data[i];
-    // inclusive wave scan
-    uint32_t v = sum;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t u = __shfl_up(v, off);
-        if (lane >= off) v += u;
-    }
-    if (lane == 63) wsum[w] = v;
-    __syncthreads();
-    if (t < 16) {
-        uint32_t a = wsum[t];
-        uint32_t x = a;
-#pragma unroll
-        for (int off = 1; off < 16; off <<= 1) {
-            uint32_t u = __shfl_up(x, off, 16);
-            if (t >= off) x += u;
-        }
-        wsum[t] = x - a;  // exclusive over waves
-    }
-    __syncthreads();
-    uint32_t run = wsum[w] + v - sum;
-    for (int64_t i = b; i < e; ++i) {
-        uint32_t d = data[i];
-        data[i] = run;
-        run += d;
-    }
-    if (t == 1023 && total) *total = (int32_t)run;
+    hist[(int64_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];   // block-major
 }
 
 __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t *__restrict__ kin,
                                                     const uint32_t *__restrict__ vin,
                                                     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                    int64_t n, int shift, const uint32_t *__restrict__ offs,
+                                                    int64_t n, int shift, const uint32_t *__restrict__ hist,
                                                     int nblk) {
     __shared__ uint32_t wcnt[4][256];
     __shared__ uint32_t gbase[256];
+    __shared__ uint32_t wtot[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int k = threadIdx.x; k < 1024; k += 256) (&wcnt[0][0])[k] = 0;
-    gbase[threadIdx.x] = offs[(int64_t)threadIdx.x * nblk + blockIdx.x];
+    {
+        // global offset of digit d for this block: keys of smaller digits in all blocks plus
+        // keys of digit d in earlier blocks (hist is block-major, so the reads coalesce)
+        const int d = threadIdx.x;
+        uint32_t before = 0, total = 0;
+        for (int b = 0; b < nblk; ++b) {
+            uint32_t c = hist[(int64_t)b * 256 + d];
+            total += c;
+            if (b < (int)blockIdx.x) before += c;
+        }
+        uint32_t v = total;
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t u = __shfl_up(v, off);
+            if (lane >= off) v += u;
+        }
+        if (lane == 63) wtot[w] = v;
+        __syncthreads();
+        uint32_t wo = 0;
+        for (int ww = 0; ww < w; ++ww) wo += wtot[ww];
+        gbase[d] = wo + v - total + before;
+    }
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE + w * 512;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -379,11 +362,17 @@ __global__ __launch_bounds__(256) void k_seg_count(const uint32_t *__restrict__ 
 }
 
 __global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *__restrict__ k, int64_t n,
-                                                     const uint32_t *__restrict__ blk_off,
+                                                     const uint32_t *__restrict__ blk,
                                                      int32_t *__restrict__ seg_start,
                                                      uint32_t *__restrict__ seg_key) {
     __shared__ uint32_t wsum[4];
+    __shared__ uint32_t wpre[4];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    // heads in earlier blocks
+    uint32_t pre = 0;
+    for (int b = t; b < (int)blockIdx.x; b += 256) pre += blk[b];
+    for (int off = 32; off > 0; off >>= 1) pre += __shfl_down(pre, off);
+    if (lane == 0) wpre[w] = pre;
     int64_t base = (int64_t)blockIdx.x * RS_TILE + t * 8;
     uint32_t flags = 0, cnt = 0;
     for (int j = 0; j < 8; ++j) {
@@ -400,9 +389,9 @@ __global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *__restrict_
     }
     if (lane == 63) wsum[w] = v;
     __syncthreads();
-    uint32_t wo = 0;
-    for (int ww = 0; ww < w; ++ww) wo += wsum[ww];
-    uint32_t r = blk_off[blockIdx.x] + wo + v - cnt;
+    uint32_t r = wpre[0] + wpre[1] + wpre[2] + wpre[3];
+    for (int ww = 0; ww < w; ++ww) r += wsum[ww];
+    r += v - cnt;
     for (int j = 0; j < 8; ++j) {
         if (flags & (1u << j)) {
             int64_t i = base + j;
@@ -425,11 +414,16 @@ __device__ __forceinline__ int32_t lower_bound_u32(const uint32_t *__restrict__ 
 
 // One block: nvalid, end sentinel of the last segment, per-scan [first, in-bounds end).
 __global__ __launch_bounds__(64) void k_seg_bounds(const uint32_t *__restrict__ k, int64_t n,
+                                                   const uint32_t *__restrict__ blk, int nblk,
                                                    int32_t *__restrict__ seg_start,
                                                    const uint32_t *__restrict__ seg_key, int S, Geo g,
                                                    DevState *__restrict__ st) {
     const int t = threadIdx.x;
-    const int32_t nseg = st->nseg;
+    uint32_t tot = 0;
+    for (int b = t; b < nblk; b += 64) tot += blk[b];
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    const int32_t nseg = (int32_t)tot;
+    if (t == 0) st->nseg = nseg;
     if (t == 0) {
         int64_t lo = 0, hi = n;
         while (lo < hi) {
@@ -451,25 +445,24 @@ __global__ __launch_bounds__(64) void k_seg_bounds(const uint32_t *__restrict__ 
 }
 
 // ------------------------------------------------------------------------------------
-// Per-segment moments about the segment's own voxel corner.  For an in-map segment
-// this is also where the slot record is born: hit = points in the voxel
-// (gvom.py:1170), total = ray passes + hits (1171, 1229), min_h = min local z over
-// the voxel's points (1390-1421), and the table entry becomes OCC | id.
-template <typename P>
-__global__ __launch_bounds__(256) void k_moments(const P *__restrict__ world, const uint32_t *__restrict__ svals,
-                                                 const int32_t *__restrict__ seg_start,
-                                                 const uint32_t *__restrict__ seg_key,
-                                                 const ScanDesc *__restrict__ sd, Geo g,
-                                                 const DevState *__restrict__ st,
-                                                 int32_t *__restrict__ tables, SlotRec *__restrict__ recs,
-                                                 int64_t ncap, Moment *__restrict__ mom) {
-    int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gi >= st->nseg) return;
-    const uint32_t key = seg_key[gi];
-    const int s = (int)(key >> g.VB);
+// Per-segment moments about the segment's own voxel corner, in two deterministic steps:
+//   k_mom_part: one thread per chunk of MCH sorted points accumulates sequentially and
+//               writes a partial Moment at the last position of every run of equal keys
+//               inside its chunk;
+//   k_mom_seg : one thread per segment adds the partials of the chunks it spans, in
+//               chunk order.
+// A segment of L points costs ceil(L / MCH) + 1 partial loads instead of L serial point
+// loads, so the long segments of voxels next to the sensor no longer set the tail.
+// k_mom_seg is also where the slot record is born: hit = points in the voxel
+// (gvom.py:1170), total = ray passes + hits (1171, 1229), min_h = min local z over the
+// voxel's points (1390-1421), and the table entry becomes OCC | id.
+constexpr int MCH = 16;
+
+__device__ __forceinline__ void key_voxel(uint32_t key, const Geo &g, int &s, bool &halo, int64_t &wx, int64_t &wy,
+                                          int64_t &wz) {
+    s = (int)(key >> g.VB);
     const uint32_t vk = key & ((1u << g.VB) - 1u);
-    const bool halo = (vk >> (g.VB - 1)) & 1u;
-    int64_t wx, wy, wz;
+    halo = (vk >> (g.VB - 1)) & 1u;
     if (!halo) {
         int64_t l = vk;
         wz = l % g.Z;
@@ -483,36 +476,103 @@ __global__ __launch_bounds__(256) void k_moments(const P *__restrict__ world, co
         wy = c % g.XYp - g.ex;
         wx = c / g.XYp - g.ex;
     }
-    double o[3];
-    scan_origin(sd[s], g, o);
-    double S0 = 0, S1 = 0, S2 = 0, Qxx = 0, Qxy = 0, Qxz = 0, Qyy = 0, Qyz = 0, Qzz = 0;
-    float mh = 1.0f;
-    const int32_t a = seg_start[gi], b = seg_start[gi + 1];
-    for (int32_t j = a; j < b; ++j) {
-        const int64_t i = svals[j];
-        double r0 = (((double)world[3 * i] / g.xy_res) - o[0]) - (double)wx;
-        double r1 = (((double)world[3 * i + 1] / g.xy_res) - o[1]) - (double)wy;
-        double r2 = (((double)world[3 * i + 2] / g.z_res) - o[2]) - (double)wz;
-        S0 += r0; S1 += r1; S2 += r2;
-        Qxx += r0 * r0; Qxy += r0 * r1; Qxz += r0 * r2;
-        Qyy += r1 * r1; Qyz += r1 * r2; Qzz += r2 * r2;
-        mh = py_minf(mh, (float)r2);
+}
+
+struct MomAcc {
+    double S0, S1, S2, Qxx, Qxy, Qxz, Qyy, Qyz, Qzz;
+    int32_t n;
+    float mh;
+    __device__ void zero() {
+        S0 = S1 = S2 = Qxx = Qxy = Qxz = Qyy = Qyz = Qzz = 0.0;
+        n = 0;
+        mh = 1.0f;
     }
+};
+
+__device__ __forceinline__ void store_moment(Moment *m, const MomAcc &a) {
+    m->S[0] = a.S0; m->S[1] = a.S1; m->S[2] = a.S2;
+    m->Q[0] = a.Qxx; m->Q[1] = a.Qxy; m->Q[2] = a.Qxz; m->Q[3] = a.Qyy; m->Q[4] = a.Qyz; m->Q[5] = a.Qzz;
+    m->n = a.n;
+    m->pad = __float_as_int(a.mh);
+}
+
+template <typename P>
+__global__ __launch_bounds__(256) void k_mom_part(const P *__restrict__ world, const uint32_t *__restrict__ skeys,
+                                                  const uint32_t *__restrict__ svals,
+                                                  const ScanDesc *__restrict__ sd, Geo g,
+                                                  const DevState *__restrict__ st, Moment *__restrict__ part) {
+    const int64_t j0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * MCH;
+    const int64_t nv = st->nvalid;
+    if (j0 >= nv) return;
+    const int64_t j1 = j0 + MCH < nv ? j0 + MCH : nv;
+    uint32_t cur = GVOM_NOKEY;
+    MomAcc a;
+    a.zero();
+    double o0 = 0, o1 = 0, o2 = 0;
+    int64_t wx = 0, wy = 0, wz = 0;
+    for (int64_t j = j0; j < j1; ++j) {
+        const uint32_t key = skeys[j];
+        if (key != cur) {
+            if (cur != GVOM_NOKEY) store_moment(part + (j - 1), a);
+            cur = key;
+            a.zero();
+            int s;
+            bool halo;
+            key_voxel(key, g, s, halo, wx, wy, wz);
+            double o[3];
+            scan_origin(sd[s], g, o);
+            o0 = o[0]; o1 = o[1]; o2 = o[2];
+        }
+        const int64_t i = svals[j];
+        double r0 = (((double)world[3 * i] / g.xy_res) - o0) - (double)wx;
+        double r1 = (((double)world[3 * i + 1] / g.xy_res) - o1) - (double)wy;
+        double r2 = (((double)world[3 * i + 2] / g.z_res) - o2) - (double)wz;
+        a.S0 += r0; a.S1 += r1; a.S2 += r2;
+        a.Qxx += r0 * r0; a.Qxy += r0 * r1; a.Qxz += r0 * r2;
+        a.Qyy += r1 * r1; a.Qyz += r1 * r2; a.Qzz += r2 * r2;
+        a.n += 1;
+        a.mh = py_minf(a.mh, (float)r2);
+    }
+    store_moment(part + (j1 - 1), a);
+}
+
+__global__ __launch_bounds__(256) void k_mom_seg(const int32_t *__restrict__ seg_start,
+                                                 const uint32_t *__restrict__ seg_key,
+                                                 const Moment *__restrict__ part, Geo g,
+                                                 const DevState *__restrict__ st,
+                                                 int32_t *__restrict__ tables, SlotRec *__restrict__ recs,
+                                                 int64_t ncap, Moment *__restrict__ mom) {
+    const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= st->nseg) return;
+    const int32_t a = seg_start[gi], b = seg_start[gi + 1];
     Moment m;
-    m.S[0] = S0; m.S[1] = S1; m.S[2] = S2;
-    m.Q[0] = Qxx; m.Q[1] = Qxy; m.Q[2] = Qxz; m.Q[3] = Qyy; m.Q[4] = Qyz; m.Q[5] = Qzz;
-    m.n = b - a;
+    for (int k = 0; k < 3; ++k) m.S[k] = 0.0;
+    for (int k = 0; k < 6; ++k) m.Q[k] = 0.0;
+    int32_t n = 0;
+    float mh = 1.0f;
+    for (int64_t c = a / MCH; c <= (int64_t)(b - 1) / MCH; ++c) {
+        const int64_t pos = ((c + 1) * MCH < b ? (c + 1) * MCH : b) - 1;
+        const Moment &p = part[pos];
+        for (int k = 0; k < 3; ++k) m.S[k] += p.S[k];
+        for (int k = 0; k < 6; ++k) m.Q[k] += p.Q[k];
+        n += p.n;
+        mh = py_minf(mh, __int_as_float(p.pad));
+    }
+    m.n = n;
     m.pad = 0;
     mom[gi] = m;
-    if (!halo) {
+    const uint32_t key = seg_key[gi];
+    const uint32_t vk = key & ((1u << g.VB) - 1u);
+    if (!((vk >> (g.VB - 1)) & 1u)) {
+        const int s = (int)(key >> g.VB);
         const int slot = (st->next_slot + s) % g.B;
         const int32_t id = (int32_t)(gi - st->first_seg[s]);
         int32_t *tv = tables + (int64_t)slot * g.V + vk;
         const int32_t passes = *tv;
         *tv = (int32_t)(GVOM_OCC | (uint32_t)id);
         SlotRec *r = recs + (int64_t)slot * ncap + id;
-        r->hit = b - a;
-        r->total = (int32_t)((uint32_t)passes + (uint32_t)(b - a));
+        r->hit = n;
+        r->total = (int32_t)((uint32_t)passes + (uint32_t)n);
         r->min_h = mh;
         r->pad = 0;
     }
@@ -544,14 +604,23 @@ __device__ __forceinline__ int64_t neighbour_segment(int64_t x, int64_t y, int64
     return (lo < nseg && seg_key[lo] == key) ? lo : -1;
 }
 
+// 32 lanes (half a wavefront) per occupied voxel, one lane per neighbour; sums over
+// neighbours are shuffle-tree reductions in a fixed lane order.
+__device__ __forceinline__ double red32(double v) {
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 32);
+    return v;
+}
+
 __global__ __launch_bounds__(256) void k_metrics(const uint32_t *__restrict__ seg_key,
                                                  const Moment *__restrict__ mom, Geo g,
                                                  const DevState *__restrict__ st,
                                                  const int32_t *__restrict__ tables,
                                                  SlotRec *__restrict__ recs, int64_t ncap) {
-    int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gi = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+    const int l32 = threadIdx.x & 31;
     const int32_t nseg = st->nseg;
-    if (gi >= nseg) return;
+    if (gi >= nseg) return;   // uniform across the 32-lane group
     const uint32_t key = seg_key[gi];
     const uint32_t vk = key & ((1u << g.VB) - 1u);
     if ((vk >> (g.VB - 1)) & 1u) return;   // halo segment: no record
@@ -560,39 +629,42 @@ __global__ __launch_bounds__(256) void k_metrics(const uint32_t *__restrict__ se
     const int32_t fs = st->first_seg[s], ie = st->inb_end[s];
     const int32_t *table = tables + (int64_t)slot * g.V;
     const int64_t uz = vk % g.Z, uc = vk / g.Z, uy = uc % g.XY, ux = uc / g.XY;
+    const int nx = 2 * g.ex + 1, nz = 2 * g.ez + 1, nn = nx * nx * nz;
 
     double n = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
-    for (int dx = -g.ex; dx <= g.ex; ++dx)
-        for (int dy = -g.ex; dy <= g.ex; ++dy)
-            for (int dz = -g.ez; dz <= g.ez; ++dz) {
-                int64_t w = neighbour_segment(ux + dx, uy + dy, uz + dz, s, g, table, fs, ie, nseg, seg_key);
-                if (w < 0) continue;
-                const Moment &M = mom[w];
-                double nw = (double)M.n;
-                m0 += M.S[0] + nw * (double)dx;
-                m1 += M.S[1] + nw * (double)dy;
-                m2 += M.S[2] + nw * (double)dz;
-                n += nw;
-            }
-    m0 = m0 / n;
-    m1 = m1 / n;
-    m2 = m2 / n;
+    for (int k = l32; k < nn; k += 32) {
+        const int dz = k % nz - g.ez, dy = (k / nz) % nx - g.ex, dx = k / (nz * nx) - g.ex;
+        int64_t w = neighbour_segment(ux + dx, uy + dy, uz + dz, s, g, table, fs, ie, nseg, seg_key);
+        if (w < 0) continue;
+        const Moment &M = mom[w];
+        double nw = (double)M.n;
+        m0 += M.S[0] + nw * (double)dx;
+        m1 += M.S[1] + nw * (double)dy;
+        m2 += M.S[2] + nw * (double)dz;
+        n += nw;
+    }
+    n = red32(n);
+    m0 = red32(m0) / n;
+    m1 = red32(m1) / n;
+    m2 = red32(m2) / n;
     double cxx = 0, cxy = 0, cxz = 0, cyy = 0, cyz = 0, czz = 0;
-    for (int dx = -g.ex; dx <= g.ex; ++dx)
-        for (int dy = -g.ex; dy <= g.ex; ++dy)
-            for (int dz = -g.ez; dz <= g.ez; ++dz) {
-                int64_t w = neighbour_segment(ux + dx, uy + dy, uz + dz, s, g, table, fs, ie, nseg, seg_key);
-                if (w < 0) continue;
-                const Moment &M = mom[w];
-                double nw = (double)M.n;
-                double a0 = (double)dx - m0, a1 = (double)dy - m1, a2 = (double)dz - m2;
-                cxx += M.Q[0] + 2.0 * M.S[0] * a0 + nw * a0 * a0;
-                cxy += M.Q[1] + M.S[0] * a1 + a0 * M.S[1] + nw * a0 * a1;
-                cxz += M.Q[2] + M.S[0] * a2 + a0 * M.S[2] + nw * a0 * a2;
-                cyy += M.Q[3] + 2.0 * M.S[1] * a1 + nw * a1 * a1;
-                cyz += M.Q[4] + M.S[1] * a2 + a1 * M.S[2] + nw * a1 * a2;
-                czz += M.Q[5] + 2.0 * M.S[2] * a2 + nw * a2 * a2;
-            }
+    for (int k = l32; k < nn; k += 32) {
+        const int dz = k % nz - g.ez, dy = (k / nz) % nx - g.ex, dx = k / (nz * nx) - g.ex;
+        int64_t w = neighbour_segment(ux + dx, uy + dy, uz + dz, s, g, table, fs, ie, nseg, seg_key);
+        if (w < 0) continue;
+        const Moment &M = mom[w];
+        double nw = (double)M.n;
+        double a0 = (double)dx - m0, a1 = (double)dy - m1, a2 = (double)dz - m2;
+        cxx += M.Q[0] + 2.0 * M.S[0] * a0 + nw * a0 * a0;
+        cxy += M.Q[1] + M.S[0] * a1 + a0 * M.S[1] + nw * a0 * a1;
+        cxz += M.Q[2] + M.S[0] * a2 + a0 * M.S[2] + nw * a0 * a2;
+        cyy += M.Q[3] + 2.0 * M.S[1] * a1 + nw * a1 * a1;
+        cyz += M.Q[4] + M.S[1] * a2 + a1 * M.S[2] + nw * a1 * a2;
+        czz += M.Q[5] + 2.0 * M.S[2] * a2 + nw * a2 * a2;
+    }
+    cxx = red32(cxx); cxy = red32(cxy); cxz = red32(cxz);
+    cyy = red32(cyy); cyz = red32(cyz); czz = red32(czz);
+    if (l32 != 0) return;
     SlotRec *r = recs + (int64_t)slot * ncap + (gi - fs);
     r->m[0] = m0;
     r->m[1] = m1;
@@ -684,7 +756,7 @@ hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, co
     return hipGetLastError();
 }
 
-int64_t radix_hist_entries(int64_t n) { return 256 * (int64_t)nblocks(n, RS_TILE) + 1024; }
+int64_t radix_hist_entries(int64_t n) { return 256 * (int64_t)nblocks(n, RS_TILE) + 256; }
 
 hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits,
                       uint32_t *hist, uint32_t **kout, uint32_t **vout, hipStream_t s) {
@@ -694,8 +766,7 @@ hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, in
     int nblk = (int)nblocks(n, RS_TILE);
     uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
     for (int shift = 0; shift < bits; shift += 8) {
-        k_rs_hist<<<nblk, 256, 0, s>>>(ka, n, shift, hist, nblk);
-        k_scan_excl<<<1, 1024, 0, s>>>(hist, (int64_t)256 * nblk, nullptr);
+        k_rs_hist<<<nblk, 256, 0, s>>>(ka, n, shift, hist);
         k_rs_scatter<<<nblk, 256, 0, s>>>(ka, va, kb, vb, n, shift, hist, nblk);
         uint32_t *tk = ka; ka = kb; kb = tk;
         uint32_t *tv = va; va = vb; vb = tv;
@@ -707,38 +778,33 @@ hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, in
 
 hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start, uint32_t *seg_key,
                            uint32_t *blk, int S, const Geo &g, DevState *st, hipStream_t s) {
-    int nblk = (int)nblocks(n, RS_TILE);
+    int nblk = n > 0 ? (int)nblocks(n, RS_TILE) : 0;
     if (n > 0) {
         k_seg_count<<<nblk, 256, 0, s>>>(skeys, n, blk);
-        k_scan_excl<<<1, 1024, 0, s>>>(blk, nblk, &st->nseg);
         k_seg_compact<<<nblk, 256, 0, s>>>(skeys, n, blk, seg_start, seg_key);
-    } else {
-        k_scan_excl<<<1, 1024, 0, s>>>(blk, 0, &st->nseg);
     }
-    k_seg_bounds<<<1, 64, 0, s>>>(skeys, n, seg_start, seg_key, S, g, st);
+    k_seg_bounds<<<1, 64, 0, s>>>(skeys, n, blk, nblk, seg_start, seg_key, S, g, st);
     return hipGetLastError();
 }
 
-hipError_t launch_moments(int dtype, const void *world, const uint32_t *svals, const int32_t *seg_start,
-                          const uint32_t *seg_key, int64_t seg_cap, const ScanDesc *sd, int S,
-                          const Geo &g, DevState *st, int32_t *slot_tables, SlotRec *slot_recs,
-                          int64_t ncap, Moment *mom, hipStream_t s) {
-    (void)S;
-    unsigned nb = nblocks(seg_cap, 256);
+hipError_t launch_moments(int dtype, const void *world, const uint32_t *skeys, const uint32_t *svals,
+                          const int32_t *seg_start, const uint32_t *seg_key, int64_t seg_cap,
+                          const ScanDesc *sd, const Geo &g, DevState *st, int32_t *slot_tables,
+                          SlotRec *slot_recs, int64_t ncap, Moment *part, Moment *mom, hipStream_t s) {
+    unsigned nbp = nblocks((seg_cap + MCH - 1) / MCH, 256);
     if (dtype == GVOM_F32)
-        k_moments<float><<<nb, 256, 0, s>>>((const float *)world, svals, seg_start, seg_key, sd, g, st,
-                                             slot_tables, slot_recs, ncap, mom);
+        k_mom_part<float><<<nbp, 256, 0, s>>>((const float *)world, skeys, svals, sd, g, st, part);
     else
-        k_moments<double><<<nb, 256, 0, s>>>((const double *)world, svals, seg_start, seg_key, sd, g, st,
-                                              slot_tables, slot_recs, ncap, mom);
+        k_mom_part<double><<<nbp, 256, 0, s>>>((const double *)world, skeys, svals, sd, g, st, part);
+    k_mom_seg<<<nblocks(seg_cap, 256), 256, 0, s>>>(seg_start, seg_key, part, g, st, slot_tables, slot_recs, ncap,
+                                                    mom);
     return hipGetLastError();
 }
 
-hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t seg_cap, int S,
-                          const Geo &g, const DevState *st, const int32_t *slot_tables,
-                          SlotRec *slot_recs, int64_t ncap, hipStream_t s) {
-    (void)S;
-    k_metrics<<<nblocks(seg_cap, 256), 256, 0, s>>>(seg_key, mom, g, st, slot_tables, slot_recs, ncap);
+hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t seg_cap, const Geo &g,
+                          const DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap,
+                          hipStream_t s) {
+    k_metrics<<<nblocks(seg_cap * 32, 256), 256, 0, s>>>(seg_key, mom, g, st, slot_tables, slot_recs, ncap);
     return hipGetLastError();
 }
 
@@ -749,11 +815,5 @@ hipError_t launch_clear_slots(int32_t *slot_tables, const DevState *st, int S, c
 
 hipError_t launch_commit_process(DevState *st, int S, int B, hipStream_t s) {
     k_commit_process<<<1, 64, 0, s>>>(st, S, B);
-    return hipGetLastError();
-}
-
-// Shared with gvom_combine.hip for the column scan.
-hipError_t launch_scan_excl_u32(uint32_t *data, int64_t n, int32_t *total, hipStream_t s) {
-    k_scan_excl<<<1, 1024, 0, s>>>(data, n, total);
     return hipGetLastError();
 }

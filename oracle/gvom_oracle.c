@@ -417,7 +417,7 @@ static void combine_metrics_f(const orc_params *p, const double co[3], const int
 }
 
 /* __calculate_eigenvalues, gvom.py:1424-1487: f32 loads, f64 math, f32 stores. */
-static void eigenvalues(const float *met, float *ev, int64_t cells) {
+void orc_eigenvalues(const float *met, float *ev, int64_t cells) {
     const double PI = 3.141592653589793;
     for (int64_t i = 0; i < cells; ++i) {
         const float *m = met + 10 * i;
@@ -448,76 +448,12 @@ static void eigenvalues(const float *met, float *ev, int64_t cells) {
     }
 }
 
-typedef struct {
-    const int32_t *index_map;
-    const int32_t *hit, *total;
-    const float *min_h;
-    const void *metrics; /* double for slots, float for the previous combined map */
-    const double *origin;
-} orc_source;
-
-/*
- * combine_maps body.  slots[0..nslots) are the filled ring slots in slot order;
- * last (may be NULL) is the previous combined map.  co = combined origin (the
- * newest slot's origin).  ego = latest ego position (gvom.py:333-336).
- * Outputs (capacity cap for the compact arrays): comb index map (reference
- * layout/encoding), hit, total, min_h, metrics f32[cap][10], eigen f32[cap][3],
- * and the f64/int32 2D maps, all [x, y] C-order.
- * Returns the combined cell count, or -1 when it exceeds cap.
- */
-int64_t orc_combine(const orc_params *p, int32_t nslots, const orc_source *slots,
-                    const orc_source *last, const double co[3], const double ego[3],
-                    int32_t *comb, int32_t *chit, int32_t *ctot, float *cmh, float *cmet,
-                    float *ev, int64_t cap,
-                    double *height, double *inferred, double *slope_x, double *slope_y,
-                    double *rough, double *delta, int32_t *pos, int32_t *neg, int32_t *vis) {
-    const int64_t XY = p->xy_size, Z = p->z_size, V = XY * XY * Z;
-    int64_t counter = 0;
-    for (int64_t v = 0; v < V; ++v) comb[v] = -1;
-    /* combine_indices per slot, then against the last combined map (gvom.py:242-257) */
-    for (int s = 0; s < nslots; ++s) combine_indices(p, co, comb, slots[s].index_map, slots[s].origin, &counter, 0);
-    if (last) combine_indices(p, co, comb, last->index_map, last->origin, &counter, 1);
-    int64_t cells = counter;
-    if (cells > cap) return -1;
-    for (int64_t c = 0; c < cells; ++c) {
-        chit[c] = 0; ctot[c] = 0; cmh[c] = 1.0f;
-        for (int j = 0; j < 10; ++j) cmet[10 * c + j] = 0.0f;
-    }
-    for (int s = 0; s < nslots; ++s)
-        combine_metrics_d(p, co, comb, cmet, chit, ctot, cmh, slots[s].index_map, slots[s].origin,
-                          (const double *)slots[s].metrics, slots[s].hit, slots[s].total, slots[s].min_h);
-    if (last)
-        combine_metrics_f(p, co, comb, cmet, chit, ctot, cmh, last->index_map, last->origin,
-                          (const float *)last->metrics, last->hit, last->total, last->min_h);
-    eigenvalues(cmet, ev, cells);
-
-    const double xyr = p->xy_resolution, zr = p->z_resolution;
-    /* __make_height_map (561-576) and __make_inferred_height_map (580-590) */
-    for (int64_t x = 0; x < XY; ++x)
-        for (int64_t y = 0; y < XY; ++y) {
-            int64_t c = x * XY + y;
-            height[c] = -1000.0;
-            inferred[c] = -1000.0;
-            double xp = ((co[0] + (double)x) * xyr) - ego[0];
-            double yp = ((co[1] + (double)y) * xyr) - ego[1];
-            if (xp * xp + yp * yp <= p->robot_radius * p->robot_radius)
-                height[c] = ego[2] - p->ground_to_lidar_height;
-            for (int64_t z = 0; z < Z; ++z) {
-                int32_t id = comb[ref_lin(x, y, z, XY)];
-                if (id >= 0) {
-                    height[c] = (((double)cmh[id] + (double)z) + co[2]) * zr;
-                    break;
-                }
-            }
-            for (int64_t z = 0; z < Z; ++z) {
-                int32_t id = comb[ref_lin(x, y, z, XY)];
-                if (id < -1) {
-                    inferred[c] = ((double)z + co[2]) * zr;
-                    break;
-                }
-            }
-        }
-
+/* __calculate_slope (gvom.py:718-805, defaults rough -1 and slopes 0 from 342-349) and
+ * __guess_height (593-714) on [x, y] C-order maps. */
+void orc_slope_guess(const orc_params *p, const double *height, const double *inferred,
+                     double *slope_x, double *slope_y, double *rough, double *delta) {
+    const int64_t XY = p->xy_size;
+    const double xyr = p->xy_resolution;
     /* __calculate_slope (718-805); defaults rough -1, slopes 0 (gvom.py:342-349) */
     for (int64_t x0 = 0; x0 < XY; ++x0)
         for (int64_t y0 = 0; y0 < XY; ++y0) {
@@ -633,6 +569,80 @@ int64_t orc_combine(const orc_params *p, int32_t nslots, const orc_source *slots
             double dh = max_h - min_h;
             if (dh > 0) delta[c0] = dh;
         }
+
+}
+
+typedef struct {
+    const int32_t *index_map;
+    const int32_t *hit, *total;
+    const float *min_h;
+    const void *metrics; /* double for slots, float for the previous combined map */
+    const double *origin;
+} orc_source;
+
+/*
+ * combine_maps body.  slots[0..nslots) are the filled ring slots in slot order;
+ * last (may be NULL) is the previous combined map.  co = combined origin (the
+ * newest slot's origin).  ego = latest ego position (gvom.py:333-336).
+ * Outputs (capacity cap for the compact arrays): comb index map (reference
+ * layout/encoding), hit, total, min_h, metrics f32[cap][10], eigen f32[cap][3],
+ * and the f64/int32 2D maps, all [x, y] C-order.
+ * Returns the combined cell count, or -1 when it exceeds cap.
+ */
+int64_t orc_combine(const orc_params *p, int32_t nslots, const orc_source *slots,
+                    const orc_source *last, const double co[3], const double ego[3],
+                    int32_t *comb, int32_t *chit, int32_t *ctot, float *cmh, float *cmet,
+                    float *ev, int64_t cap,
+                    double *height, double *inferred, double *slope_x, double *slope_y,
+                    double *rough, double *delta, int32_t *pos, int32_t *neg, int32_t *vis) {
+    const int64_t XY = p->xy_size, Z = p->z_size, V = XY * XY * Z;
+    int64_t counter = 0;
+    for (int64_t v = 0; v < V; ++v) comb[v] = -1;
+    /* combine_indices per slot, then against the last combined map (gvom.py:242-257) */
+    for (int s = 0; s < nslots; ++s) combine_indices(p, co, comb, slots[s].index_map, slots[s].origin, &counter, 0);
+    if (last) combine_indices(p, co, comb, last->index_map, last->origin, &counter, 1);
+    int64_t cells = counter;
+    if (cells > cap) return -1;
+    for (int64_t c = 0; c < cells; ++c) {
+        chit[c] = 0; ctot[c] = 0; cmh[c] = 1.0f;
+        for (int j = 0; j < 10; ++j) cmet[10 * c + j] = 0.0f;
+    }
+    for (int s = 0; s < nslots; ++s)
+        combine_metrics_d(p, co, comb, cmet, chit, ctot, cmh, slots[s].index_map, slots[s].origin,
+                          (const double *)slots[s].metrics, slots[s].hit, slots[s].total, slots[s].min_h);
+    if (last)
+        combine_metrics_f(p, co, comb, cmet, chit, ctot, cmh, last->index_map, last->origin,
+                          (const float *)last->metrics, last->hit, last->total, last->min_h);
+    orc_eigenvalues(cmet, ev, cells);
+
+    const double xyr = p->xy_resolution, zr = p->z_resolution;
+    /* __make_height_map (561-576) and __make_inferred_height_map (580-590) */
+    for (int64_t x = 0; x < XY; ++x)
+        for (int64_t y = 0; y < XY; ++y) {
+            int64_t c = x * XY + y;
+            height[c] = -1000.0;
+            inferred[c] = -1000.0;
+            double xp = ((co[0] + (double)x) * xyr) - ego[0];
+            double yp = ((co[1] + (double)y) * xyr) - ego[1];
+            if (xp * xp + yp * yp <= p->robot_radius * p->robot_radius)
+                height[c] = ego[2] - p->ground_to_lidar_height;
+            for (int64_t z = 0; z < Z; ++z) {
+                int32_t id = comb[ref_lin(x, y, z, XY)];
+                if (id >= 0) {
+                    height[c] = (((double)cmh[id] + (double)z) + co[2]) * zr;
+                    break;
+                }
+            }
+            for (int64_t z = 0; z < Z; ++z) {
+                int32_t id = comb[ref_lin(x, y, z, XY)];
+                if (id < -1) {
+                    inferred[c] = ((double)z + co[2]) * zr;
+                    break;
+                }
+            }
+        }
+
+    orc_slope_guess(p, height, inferred, slope_x, slope_y, rough, delta);
 
     /* __make_positive_obstacle_map (516-555), negative (506-512), visibility (445-452) */
     for (int64_t x = 0; x < XY; ++x)

@@ -66,6 +66,10 @@ def load_library():
                                 P, P, P, P, P, P, P, P, P]
     lib.orc_origin.restype = None
     lib.orc_origin.argtypes = [P, P, P]
+    lib.orc_eigenvalues.restype = None
+    lib.orc_eigenvalues.argtypes = [P, P, ctypes.c_int64]
+    lib.orc_slope_guess.restype = None
+    lib.orc_slope_guess.argtypes = [P, P, P, P, P, P, P]
     _LIB = lib
     return lib
 
@@ -299,6 +303,27 @@ def debug_inferred_height_map(cm: CombinedMap, xy, xyr, zr):
     out[index, 1] = ((y + cm.origin[1]) * xyr).ravel()
     out[index, 2] = (cm.delta - zr).ravel()
     return out
+
+
+def eigenvalues(metrics_f32):
+    """The oracle's __calculate_eigenvalues on rows of a f32 [C, 10] metrics array."""
+    lib = load_library()
+    m = np.ascontiguousarray(metrics_f32, dtype=np.float32).reshape(-1, 10)
+    ev = np.empty((m.shape[0], 3), np.float32)
+    lib.orc_eigenvalues(_ptr(m), _ptr(ev), m.shape[0])
+    return ev
+
+
+def slope_guess(height, inferred, xy_resolution):
+    """The oracle's __calculate_slope + __guess_height on [x, y] maps -> (sx, sy, rough, delta)."""
+    lib = load_library()
+    h = np.ascontiguousarray(height, dtype=np.float64)
+    f = np.ascontiguousarray(inferred, dtype=np.float64)
+    XY = h.shape[0]
+    p = _Params(float(xy_resolution), 1.0, XY, 4, 1, 0, 1.0, 0.5, 0.5, 0.3, 2.0, 4.0, 1.0, 1, 1)
+    outs = [np.empty((XY, XY), np.float64) for _ in range(4)]
+    lib.orc_slope_guess(ctypes.byref(p), _ptr(h), _ptr(f), *[_ptr(o) for o in outs])
+    return tuple(outs)
 
 
 # -- canonicalisation ------------------------------------------------------------------
