@@ -7,9 +7,20 @@ with a ring buffer of 8 scans.  One step = one frame = Process_pointcloud of one
 + combine_maps, the reference's per-scan cadence.  Inputs (8 distinct synthetic scans,
 cycled) are resident in HBM before the timed region; the maps stay on the device.
 
-Prints one JSON line.  Per-stage device times come from HIP events the library
-records on its own stream over the timed region; the dominant kernel's achieved HBM
-bandwidth is its algorithmic bytes per launch over its average duration.
+`value` is the frame rate of that sequence replayed from a captured hipGraph (8 frames
+per launch); `plain_eager` is the same frames issued call by call.  The eager pass
+records HIP events around every stage on the library's stream; the dominant kernel's
+achieved HBM bandwidth is its algorithmic bytes per launch over its average duration.
+
+Extra legs (on by default, --quick skips them):
+  batched   3 sensors (131,072 / 65,536 / 16,384 points, 10-40 % dropout) x 8 steps,
+            one combine per step, captured as one hipGraph: scans/s, against the same
+            24 scans sent as 24 plain frames;
+  diff      the differentiable map loss on one 65,536-point scan: forward and
+            forward+backward ms, and 20 steps of extrinsic refinement from a transform
+            perturbed by N(0, 0.5 deg) / N(0, 5 cm).
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -58,6 +69,130 @@ def cpu_baseline(wl, seconds):
                        f"through the single-threaded C oracle, {el:.1f} s")
 
 
+def graph_rate(g, dev, scans, reps, warm=2):
+    """Frames/s of the 8-frame sequence replayed from one captured graph."""
+    g.sync()
+    g.graph_begin(len(scans))
+    for k, s in enumerate(scans):
+        g.process_pointcloud_device(dev[k], s.points.shape[0], s.ego, s.transform)
+        g.combine_maps_device()
+    gr = g.graph_end()
+    egos = np.stack([s.ego for s in scans])
+    Ts = np.stack([s.transform for s in scans])
+    for _ in range(warm):
+        gr.launch(egos, Ts)
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gr.launch(egos, Ts)
+    g.sync()
+    return reps * len(scans) / (time.perf_counter() - t0)
+
+
+def batched_leg(reps):
+    import gvom
+    import gvom_native as nat
+    import gvom_synth
+    wl = gvom_synth.workload(4, frames=8)
+    flat = [np.concatenate([s.points for s in st]) for st in wl.steps]
+    offs = [np.cumsum([0] + [s.points.shape[0] for s in st]) for st in wl.steps]
+    egos = np.concatenate([np.stack([s.ego for s in st]) for st in wl.steps])
+    Ts = np.concatenate([np.stack([s.transform for s in st]) for st in wl.steps])
+    dev = [nat.DeviceArray(f) for f in flat]
+    scans = [s for st in wl.steps for s in st]
+    sdev = [nat.DeviceArray(s.points) for s in scans]
+    n_pts = sum(f.shape[0] for f in flat)
+    g = gvom.Gvom(*wl.map_args)
+    g.reserve(max(s.points.shape[0] for s in scans), max(f.shape[0] for f in flat))
+    g.graph_begin(len(scans))
+    for k in range(len(wl.steps)):
+        g.process_batch_device(dev[k], offs[k], np.stack([s.ego for s in wl.steps[k]]),
+                               np.stack([s.transform for s in wl.steps[k]]))
+        g.combine_maps_device()
+    gr = g.graph_end()
+    for _ in range(2):
+        gr.launch(egos, Ts)
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gr.launch(egos, Ts)
+    g.sync()
+    t_b = (time.perf_counter() - t0) / reps
+    p = gvom.Gvom(*wl.map_args)
+    p.reserve(max(s.points.shape[0] for s in scans))
+    for k, s in enumerate(scans[:8]):
+        p.process_pointcloud_device(sdev[k], s.points.shape[0], s.ego, s.transform)
+        p.combine_maps_device()
+    p.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for k, s in enumerate(scans):
+            p.process_pointcloud_device(sdev[k], s.points.shape[0], s.ego, s.transform)
+            p.combine_maps_device()
+    p.sync()
+    t_p = (time.perf_counter() - t0) / reps
+    return {"workload": "3 sensors (128/64/16 x 1024, 10-40% dropout) x 8 steps, 256x256x64, buffer 8; "
+                        "one hipGraph launch = 8 x (3-scan batch + combine_maps)",
+            "scans_per_launch": len(scans), "points_per_launch": int(n_pts),
+            "scans_per_s": round(len(scans) / t_b, 1), "ms_per_launch": round(1e3 * t_b, 3),
+            "plain_frames_per_s": round(len(scans) / t_p, 1),
+            "speedup_vs_plain": round(t_p / t_b, 2)}
+
+
+def diff_leg(reps):
+    import gvom_native as nat
+    import gvom_synth
+    from gvom_torch import MapLoss, se3_exp
+    wl = gvom_synth.workload(5)
+    s = wl.steps[0][0]
+    rng = np.random.default_rng(105)
+    T_gt = s.transform.astype(np.float64)
+    T = gvom_synth.perturb_transform(s.transform, rng).astype(np.float64)
+    pts = s.points.astype(np.float64)
+    dev = nat.DeviceArray(pts)
+    gdev = nat.DeviceArray(np.zeros_like(pts))
+    loss = MapLoss(0.4, 0.2, 256, 64)
+    loss.forward(dev.ptr.value, pts.shape[0], T, s.ego)
+    loss.backward(1.0, gdev.ptr.value)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss.forward(dev.ptr.value, pts.shape[0], T, s.ego)
+    t_f = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss.forward(dev.ptr.value, pts.shape[0], T, s.ego)
+        loss.backward(1.0, gdev.ptr.value)
+    t_fb = (time.perf_counter() - t0) / reps
+
+    def err(Ta):
+        dR = Ta[:3, :3] @ T_gt[:3, :3].T
+        ang = np.degrees(np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1)))
+        return float(ang), float(np.linalg.norm(Ta[:3, 3] - T_gt[:3, 3]))
+
+    e0 = err(T)
+    L0 = loss.forward(dev.ptr.value, pts.shape[0], T, s.ego)[0]
+    Tk = T.copy()
+    m = np.zeros(6)
+    v = np.zeros(6)
+    lr = np.array([0.01, 0.01, 0.01, 0.002, 0.002, 0.002])
+    for k in range(1, 21):                   # Adam on the left se(3) perturbation, re-anchored each step
+        loss.forward(dev.ptr.value, pts.shape[0], Tk, s.ego)
+        gx = loss.backward(1.0)
+        m = 0.9 * m + 0.1 * gx
+        v = 0.999 * v + 0.001 * gx * gx
+        step = lr * (m / (1 - 0.9 ** k)) / (np.sqrt(v / (1 - 0.999 ** k)) + 1e-12)
+        Tk = se3_exp(-step) @ Tk
+    L1 = loss.forward(dev.ptr.value, pts.shape[0], Tk, s.ego)[0]
+    e1 = err(Tk)
+    return {"workload": "one 64x1024 scan (65,536 rays, 5% dropout), 256x256x64; L = sum lambda_min + "
+                        "0.1 * column-height variance",
+            "points": int(pts.shape[0]), "fwd_ms": round(1e3 * t_f, 3), "fwd_bwd_ms": round(1e3 * t_fb, 3),
+            "bwd_over_fwd": round(t_fb / t_f, 3),
+            "refine_20_steps": {"loss": [round(float(L0), 4), round(float(L1), 4)],
+                                "rot_err_deg": [round(e0[0], 4), round(e1[0], 4)],
+                                "trans_err_m": [round(e0[1], 4), round(e1[1], 4)]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +200,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline only (no graph, batched or diff legs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU only; multi-GPU runs are not supported "
@@ -101,8 +237,11 @@ def main():
     stages = g.stage_times()
     g.set_timing(False)
 
-    fps = args.steps / el
-    ms = 1e3 * el / args.steps
+    eager_fps = args.steps / el
+    fps = eager_fps
+    if not args.quick:
+        fps = graph_rate(g, dev, scans, max(1, args.steps // len(scans)))
+    ms = 1e3 / fps
     V = 256 * 256 * 64
     X = 256 * 256
     slot = g.ring_state()["last_buffer_index"]
@@ -131,9 +270,11 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": "plain frame: Process_pointcloud (131072-pt 128x1024 f32 scan, f32 4x4 transform) "
-                               "+ combine_maps, 256x256x64 voxels @0.4/0.2 m, buffer 8",
+                               "+ combine_maps, 256x256x64 voxels @0.4/0.2 m, buffer 8"
+                               + ("" if args.quick else "; 8 frames per hipGraph launch"),
                    "points_per_scan": N, "grid": [256, 256, 64], "buffer": 8, "frames_cycled": len(scans)},
         "points_per_s": round(fps * N, 1),
+        "plain_eager": {"frames_per_s": round(eager_fps, 2), "ms_per_frame": round(1e3 / eager_fps, 4)},
         "stages_ms": {k: round(v, 5) for k, v in zip(STAGES, stages)},
         "scan": {"cells": ss["cells"], "ray_steps": ss["steps"], "combined_cells": C},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -141,6 +282,9 @@ def main():
                      "algorithmic_bytes": int(b_dom)},
         "gen_seconds": round(gen_s, 1),
     }
+    if not args.quick:
+        out["batched"] = batched_leg(max(1, args.steps // 24))
+        out["fwd_bwd"] = diff_leg(max(5, args.steps // 8))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     else:

@@ -14,6 +14,17 @@
 #include "gvom_kernels.h"
 
 hipError_t launch_clear_slots(int32_t *slot_tables, const DevState *st, int S, const Geo &g, hipStream_t s);
+int64_t diff_partials(int64_t n_points, int64_t X, int64_t ncap);
+int64_t diff_rec_bytes();
+hipError_t launch_diff_forward(const Geo &g, const DevState *st, int slot, const int32_t *table,
+                               const SlotRec *recs, int64_t ncap, const int32_t *seg_start, const uint32_t *svals,
+                               const double *world, int64_t n, void *dr, double *colh, int32_t *colpt,
+                               int32_t *pmark, double *part, double *sums, double wh, hipStream_t s);
+hipError_t launch_diff_backward(const Geo &g, const DevState *st, int slot, const ScanDesc *sd,
+                                const double *world, const uint8_t *valid, const int32_t *table, const void *dr,
+                                const int32_t *pmark, const double *colh, const double *sums, double wh, double dL,
+                                int64_t n, double *gsensor, double *gworld, double *part, double *grad6,
+                                hipStream_t s);
 
 namespace {
 thread_local std::string g_create_error;
@@ -42,7 +53,8 @@ struct gvom_handle {
     int64_t ccap = 0;
     double *height = nullptr, *inferred = nullptr, *slope_x = nullptr, *slope_y = nullptr, *rough = nullptr,
            *delta = nullptr;
-    int32_t *pos = nullptr, *neg = nullptr, *vis = nullptr, *density = nullptr, *col_count = nullptr;
+    int32_t *pos = nullptr, *neg = nullptr, *vis = nullptr, *col_count = nullptr;
+    uint2 *occ_list = nullptr;
     int32_t *export_buf = nullptr;
     int32_t *chunk_tot = nullptr;
 
@@ -63,11 +75,29 @@ struct gvom_handle {
 
     // Stage timing: one set of 9 events per process call (0..4) and combine call (5..8);
     // gvom_stage_times averages every stage over all sets recorded since timing was enabled.
+    // differentiable form: buffers (allocated on first use) and the forward's context
+    void *diff_rec = nullptr;
+    double *diff_colh = nullptr, *diff_part = nullptr, *diff_sums = nullptr;
+    int32_t *diff_colpt = nullptr, *diff_pmark = nullptr;
+    int64_t diff_rec_cap = 0, diff_pt_cap = 0, diff_part_cap = 0;
+    const uint32_t *sorted_vals = nullptr;
+    int64_t diff_n = -1;
+    int diff_slot = 0;
+    double diff_wh = 0.0;
+
+    // hipGraph capture: ops recorded while capturing (process = S scans, combine = -1)
+    bool capturing = false;
+    std::vector<int> cap_ops;
+    std::vector<ScanDesc> cap_sd;   // descriptors of the captured scans, in capture order
+    ScanDesc *graph_sd = nullptr;   // device pool the captured kernels read descriptors from
+    int64_t graph_sd_cap = 0;
+    int mirror_next = 0, mirror_last = 0, mirror_filled = 0, mirror_parity = 0, mirror_have = 0;
+
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> set_used;   // bit k set when event k of that set was recorded
     int cur_set = -1;
-    int ray_block = 256, ray_rpt = 1;
+    int ray_block = 256, ray_rpt = 1, ray_box = 1;
 };
 
 #define CKH(h, expr)                                                                 \
@@ -81,9 +111,10 @@ struct gvom_handle {
 
 static void free_all(gvom_handle *h) {
     void *ptrs[] = {h->st, h->slot_tables, h->slot_recs, h->comb_tables, h->comb_recs, h->height, h->inferred,
-                    h->slope_x, h->slope_y, h->rough, h->delta, h->pos, h->neg, h->vis, h->density,
+                    h->slope_x, h->slope_y, h->rough, h->delta, h->pos, h->neg, h->vis, h->occ_list,
                     h->col_count, h->chunk_tot, h->export_buf, h->pts_in, h->world, h->valid, h->k0, h->v0, h->k1, h->v1,
-                    h->hist, h->blk, h->seg_key, h->seg_start, h->mom, h->part, h->sd};
+                    h->hist, h->blk, h->seg_key, h->seg_start, h->mom, h->part, h->sd, h->diff_rec,
+                    h->diff_colh, h->diff_part, h->diff_sums, h->diff_colpt, h->diff_pmark, h->graph_sd};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : h->ev_pool)
@@ -173,7 +204,7 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     CKC(dalloc(&h->pos, X));
     CKC(dalloc(&h->neg, X));
     CKC(dalloc(&h->vis, X));
-    CKC(dalloc(&h->density, X));
+    CKC(dalloc(&h->occ_list, V));
     CKC(dalloc(&h->col_count, X + 16));
     CKC(dalloc(&h->chunk_tot, col_chunks(X) + 1));
     CKC(dalloc(&h->sd, GVOM_MAX_BATCH));
@@ -201,6 +232,14 @@ extern "C" void *gvom_stream(gvom_handle *h) { return h ? (void *)h->stream : nu
 // Grow per-batch work buffers (points) and per-slot record capacity (cells per scan).
 static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) {
     const Geo &g = h->g;
+    if (h->capturing) {
+        int64_t need_ncap = max_scan < g.V ? max_scan : g.V;
+        if (need_ncap > h->ncap || batch_pts > h->ptcap || h->ptcap < 1) {
+            h->err = "buffers must be reserved (gvom_reserve) before graph capture";
+            return GVOM_ESTATE;
+        }
+        return GVOM_OK;
+    }
     int64_t need_ncap = max_scan < g.V ? max_scan : g.V;
     if (need_ncap < 1) need_ncap = 1;
     if (need_ncap > h->ncap) {
@@ -241,7 +280,7 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
         CKH(h, dalloc(&h->k1, pc));
         CKH(h, dalloc(&h->v1, pc));
         CKH(h, dalloc(&h->hist, radix_hist_entries(pc)));
-        CKH(h, dalloc(&h->blk, pc / 2048 + 2));
+        CKH(h, dalloc(&h->blk, (pc / 2048 + 2) * (2 + 3 * GVOM_MAX_BATCH)));
         CKH(h, dalloc(&h->seg_key, pc));
         CKH(h, dalloc(&h->seg_start, pc + 1));
         CKH(h, dalloc(&h->mom, pc));
@@ -294,6 +333,7 @@ static void stage_event(gvom_handle *h, int k) {
 static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets, int S, int dtype,
                         const double *ego, const void *T, int t_dtype, bool on_device) {
     const Geo &g = h->g;
+    h->diff_n = -1;   // the work buffers a pending gvom_diff_backward reads are reused below
     if (S < 1 || S > GVOM_MAX_BATCH || S > g.B) {
         h->err = "number of scans must be in [1, min(32, buffer_size)]";
         return GVOM_EINVAL;
@@ -366,23 +406,41 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     hipStream_t st = h->stream;
     const size_t esz = dtype == GVOM_F32 ? 4 : 8;
     const void *pts = xyz;
-    if (!on_device && NT > 0) {
-        CKH(h, hipMemcpyAsync(h->pts_in, xyz, (size_t)NT * 3 * esz, hipMemcpyHostToDevice, st));
-        pts = h->pts_in;
+    ScanDesc *sd = h->sd;
+    if (h->capturing) {
+        if (!on_device) {
+            h->err = "only device-pointer process calls can be captured";
+            return GVOM_ESTATE;
+        }
+        // descriptors come from the graph pool, written by gvom_graph_launch before each replay
+        const int64_t at = (int64_t)h->cap_sd.size();
+        if (at + S > h->graph_sd_cap) {
+            h->err = "too many scans in one captured sequence";
+            return GVOM_ECAP;
+        }
+        sd = h->graph_sd + at;
+        for (int s = 0; s < S; ++s) h->cap_sd.push_back(sdh[s]);
+        h->cap_ops.push_back(S);
+    } else {
+        if (!on_device && NT > 0) {
+            CKH(h, hipMemcpyAsync(h->pts_in, xyz, (size_t)NT * 3 * esz, hipMemcpyHostToDevice, st));
+            pts = h->pts_in;
+        }
+        CKH(h, hipMemcpyAsync(h->sd, sdh, sizeof(ScanDesc) * S, hipMemcpyHostToDevice, st));
     }
-    CKH(h, hipMemcpyAsync(h->sd, sdh, sizeof(ScanDesc) * S, hipMemcpyHostToDevice, st));
     stage_event(h, 0);
     CKH(h, launch_clear_slots(h->slot_tables, h->st, S, g, st));
-    CKH(h, launch_prepare(dtype, pts, h->world, h->k0, h->v0, h->valid, h->sd, S, NT, g, h->st, st));
+    CKH(h, launch_prepare(dtype, pts, h->world, h->k0, h->v0, h->valid, sd, S, NT, g, h->st, st));
     stage_event(h, 1);
-    CKH(h, launch_raycast(dtype, h->world, h->valid, h->sd, S, NT, g, h->st, h->slot_tables, h->ray_block,
-                          h->ray_rpt, st));
+    CKH(h, launch_raycast(dtype, h->world, h->valid, sd, S, NT, g, h->st, h->slot_tables, h->ray_block,
+                          h->ray_rpt, h->ray_box, -1, st));
     stage_event(h, 2);
     uint32_t *sk = h->k0, *sv = h->v0;
     CKH(h, radix_sort(h->k0, h->v0, h->k1, h->v1, NT, g.VB + sbits, h->hist, &sk, &sv, st));
+    h->sorted_vals = sv;
     CKH(h, launch_segments(sk, NT, h->seg_start, h->seg_key, h->blk, S, g, h->st, st));
     stage_event(h, 3);
-    CKH(h, launch_moments(dtype, h->world, sk, sv, h->seg_start, h->seg_key, NT, h->sd, g, h->st, h->slot_tables,
+    CKH(h, launch_moments(dtype, h->world, sk, sv, h->seg_start, h->seg_key, NT, sd, S, g, h->st, h->slot_tables,
                           h->slot_recs, h->ncap, h->part, h->mom, st));
     CKH(h, launch_metrics(h->seg_key, h->mom, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, st));
     CKH(h, launch_commit_process(h->st, S, g.B, st));
@@ -430,16 +488,17 @@ static int combine_impl(gvom_handle *h) {
     const Geo &g = h->g;
     if (!(h->filled_mask & (1 << h->last_slot))) return GVOM_NO_DATA;
     hipStream_t st = h->stream;
+    if (h->capturing) h->cap_ops.push_back(-1);
     const int cur = 1 - h->parity;
     stage_event(h, 5);
     CKH(h, launch_comb_tables(g, h->st, h->slot_tables, h->comb_tables, h->col_count, st));
     stage_event(h, 6);
-    CKH(h, launch_scan_columns(h->col_count, g.X, h->chunk_tot, st));
+    CKH(h, launch_scan_columns(h->col_count, g.X, h->chunk_tot, h->st, st));
     CKH(h, launch_comb_records(g, h->st, h->slot_tables, h->slot_recs, h->ncap, h->comb_tables, h->comb_recs,
-                               h->ccap, h->col_count, h->chunk_tot, h->height, h->inferred, h->density, st));
+                               h->ccap, h->col_count, h->chunk_tot, h->occ_list, h->height, h->inferred, st));
     stage_event(h, 7);
-    CKH(h, launch_maps2d(g, h->height, h->inferred, h->density, h->slope_x, h->slope_y, h->rough, h->delta,
-                         h->pos, h->neg, h->vis, st));
+    CKH(h, launch_maps2d(g, h->st, h->height, h->inferred, h->comb_tables, h->comb_recs, h->ccap, h->slope_x,
+                         h->slope_y, h->rough, h->delta, h->pos, h->neg, h->vis, st));
     CKH(h, launch_commit_combine(h->st, h->chunk_tot, g.X, st));
     stage_event(h, 8);
     h->parity = cur;
@@ -458,6 +517,10 @@ extern "C" int gvom_combine_maps(gvom_handle *h, double origin_m[3], int32_t *po
                                  int32_t *vis) {
     if (!h) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    if (h->capturing) {
+        h->err = "combine_maps copies to the host; capture gvom_combine_maps_device instead";
+        return GVOM_ESTATE;
+    }
     int rc = combine_impl(h);
     if (rc) return rc;
     const size_t X = (size_t)h->g.X;
@@ -703,5 +766,311 @@ extern "C" int gvom_stage_times(gvom_handle *h, float *ms, int32_t n) {
         }
         ms[k] = cnt ? (float)(sum / cnt) : 0.f;
     }
+    return GVOM_OK;
+}
+
+// ---- differentiable form -------------------------------------------------------------
+extern "C" int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, const double T[16],
+                                 const double ego[3], double w_height, double out[3]) {
+    if (!h || !T || !ego || n < 0) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    int64_t off[2] = {0, n};
+    int rc = process_impl(h, points, off, 1, GVOM_F64, ego, T, GVOM_F64, true);
+    if (rc) return rc;
+    const Geo &g = h->g;
+    const int64_t X = g.X;
+    if (!h->diff_colh) {
+        CKH(h, dalloc(&h->diff_colh, X));
+        CKH(h, dalloc(&h->diff_colpt, X));
+        CKH(h, dalloc(&h->diff_sums, 16));
+    }
+    if (h->diff_rec_cap < h->ncap) {
+        if (h->diff_rec) CKH(h, hipFree(h->diff_rec));
+        CKH(h, hipMalloc(&h->diff_rec, (size_t)h->ncap * (size_t)diff_rec_bytes()));
+        h->diff_rec_cap = h->ncap;
+    }
+    const int64_t need_pt = n > 0 ? n : 1;
+    if (h->diff_pt_cap < need_pt) {
+        if (h->diff_pmark) CKH(h, hipFree(h->diff_pmark));
+        CKH(h, dalloc(&h->diff_pmark, need_pt));
+        h->diff_pt_cap = need_pt;
+    }
+    const int64_t np = diff_partials(need_pt, X, h->ncap);
+    if (h->diff_part_cap < np) {
+        if (h->diff_part) CKH(h, hipFree(h->diff_part));
+        CKH(h, dalloc(&h->diff_part, np));
+        h->diff_part_cap = np;
+    }
+    const int slot = h->last_slot;
+    CKH(h, launch_diff_forward(g, h->st, slot, h->slot_tables, h->slot_recs + (int64_t)slot * h->ncap, h->ncap,
+                               h->seg_start, h->sorted_vals, (const double *)h->world, n, h->diff_rec, h->diff_colh,
+                               h->diff_colpt, h->diff_pmark, h->diff_part, h->diff_sums, w_height, h->stream));
+    double sums[8];
+    CKH(h, hipMemcpyAsync(sums, h->diff_sums, sizeof(sums), hipMemcpyDeviceToHost, h->stream));
+    CKH(h, hipStreamSynchronize(h->stream));
+    if (out) {
+        out[0] = sums[4];
+        out[1] = sums[5];
+        out[2] = sums[6];
+    }
+    h->diff_n = n;
+    h->diff_slot = slot;
+    h->diff_wh = w_height;
+    return GVOM_OK;
+}
+
+extern "C" int gvom_diff_backward(gvom_handle *h, double dL, double *grad_points, double *grad_world,
+                                  double grad_xi[6]) {
+    if (!h) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (h->diff_n < 0) {
+        h->err = "gvom_diff_backward needs a preceding gvom_diff_forward";
+        return GVOM_ESTATE;
+    }
+    double *g6 = h->diff_sums + 8;
+    CKH(h, launch_diff_backward(h->g, h->st, h->diff_slot, h->sd, (const double *)h->world, h->valid, h->slot_tables,
+                                h->diff_rec, h->diff_pmark, h->diff_colh, h->diff_sums, h->diff_wh, dL, h->diff_n,
+                                grad_points, grad_world, h->diff_part, g6, h->stream));
+    double gx[6];
+    CKH(h, hipMemcpyAsync(gx, g6, sizeof(gx), hipMemcpyDeviceToHost, h->stream));
+    CKH(h, hipStreamSynchronize(h->stream));
+    if (grad_xi)
+        for (int k = 0; k < 6; ++k) grad_xi[k] = gx[k];
+    return GVOM_OK;
+}
+
+// ---- hipGraph capture of a launch sequence --------------------------------------------
+struct gvom_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<int> ops;
+    std::vector<ScanDesc> sd;
+};
+
+extern "C" int gvom_graph_begin(gvom_handle *h, int32_t max_scans) {
+    if (!h || max_scans < 1) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (h->capturing) {
+        h->err = "a capture is already in progress";
+        return GVOM_ESTATE;
+    }
+    if (h->timing) {
+        h->err = "disable timing before graph capture";
+        return GVOM_ESTATE;
+    }
+    if (h->graph_sd_cap < max_scans) {
+        if (h->graph_sd) CKH(h, hipFree(h->graph_sd));
+        CKH(h, dalloc(&h->graph_sd, max_scans));
+        h->graph_sd_cap = max_scans;
+    }
+    CKH(h, hipStreamSynchronize(h->stream));
+    h->cap_ops.clear();
+    h->cap_sd.clear();
+    h->mirror_next = h->next_slot;
+    h->mirror_last = h->last_slot;
+    h->mirror_filled = h->filled_mask;
+    h->mirror_parity = h->parity;
+    h->mirror_have = h->have_last;
+    CKH(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    h->capturing = true;
+    return GVOM_OK;
+}
+
+extern "C" int gvom_graph_end(gvom_handle *h, gvom_graph **out) {
+    if (!h || !out) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    *out = nullptr;
+    if (!h->capturing) {
+        h->err = "no capture in progress";
+        return GVOM_ESTATE;
+    }
+    h->capturing = false;
+    // nothing ran: restore the host mirror; gvom_graph_launch replays the bookkeeping
+    h->next_slot = h->mirror_next;
+    h->last_slot = h->mirror_last;
+    h->filled_mask = h->mirror_filled;
+    h->parity = h->mirror_parity;
+    h->have_last = h->mirror_have;
+    gvom_graph *gr = new gvom_graph();
+    hipError_t e = hipStreamEndCapture(h->stream, &gr->graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        h->err = std::string("graph capture failed: ") + hipGetErrorString(e);
+        if (gr->graph) (void)hipGraphDestroy(gr->graph);
+        delete gr;
+        return GVOM_EHIP;
+    }
+    gr->ops = h->cap_ops;
+    gr->sd = h->cap_sd;
+    *out = gr;
+    return GVOM_OK;
+}
+
+extern "C" int gvom_graph_scans(const gvom_graph *gr) {
+    return gr ? (int)gr->sd.size() : 0;
+}
+
+extern "C" int gvom_graph_launch(gvom_handle *h, gvom_graph *gr, const double *ego, const void *T, int32_t t_dtype) {
+    if (!h || !gr || (!ego && !gr->sd.empty())) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (h->capturing) {
+        h->err = "cannot launch while capturing";
+        return GVOM_ESTATE;
+    }
+    const int n = (int)gr->sd.size();
+    if (n > h->graph_sd_cap) {
+        h->err = "graph descriptor pool smaller than the graph";
+        return GVOM_ESTATE;
+    }
+    std::vector<ScanDesc> sd = gr->sd;
+    for (int k = 0; k < n; ++k) {
+        for (int j = 0; j < 3; ++j) {
+            const double res = j < 2 ? h->g.xy_res : h->g.z_res;
+            if (!std::isfinite(ego[3 * k + j]) || std::fabs(ego[3 * k + j] / res) >= 4194304.0) {
+                h->err = "ego position must be finite with |ego / resolution| < 2^22";
+                return GVOM_EINVAL;
+            }
+            sd[k].ego[j] = ego[3 * k + j];
+        }
+        if (T && t_dtype == GVOM_F32) {
+            for (int j = 0; j < 12; ++j) sd[k].T[j] = (double)((const float *)T)[16 * k + j];
+            sd[k].t_mode = GVOM_F32;
+        } else if (T && t_dtype == GVOM_F64) {
+            for (int j = 0; j < 12; ++j) sd[k].T[j] = ((const double *)T)[16 * k + j];
+            sd[k].t_mode = GVOM_F64;
+        } else {
+            sd[k].t_mode = GVOM_NO_TRANSFORM;
+        }
+    }
+    if (n > 0) CKH(h, hipMemcpyAsync(h->graph_sd, sd.data(), sizeof(ScanDesc) * n, hipMemcpyHostToDevice, h->stream));
+    CKH(h, hipGraphLaunch(gr->exec, h->stream));
+    // host mirror of the ring, as the captured calls would have left it
+    int k = 0;
+    for (int op : gr->ops) {
+        if (op > 0) {
+            for (int s = 0; s < op; ++s, ++k) {
+                const int slot = (h->next_slot + s) % h->g.B;
+                h->filled_mask |= 1 << slot;
+                host_origin(h->g, sd[k].ego, h->slot_origin[slot]);
+            }
+            h->last_slot = (h->next_slot + op - 1) % h->g.B;
+            h->next_slot = (h->next_slot + op) % h->g.B;
+            for (int j = 0; j < 3; ++j) h->latest_ego[j] = sd[k - 1].ego[j];
+        } else {
+            h->parity = 1 - h->parity;
+            h->have_last = 1;
+        }
+    }
+    return GVOM_OK;
+}
+
+extern "C" void gvom_graph_destroy(gvom_graph *gr) {
+    if (!gr) return;
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+    if (gr->graph) (void)hipGraphDestroy(gr->graph);
+    delete gr;
+}
+
+// ---- autotuner -----------------------------------------------------------------------------
+// Times the ray-cast kernel on one real scan for every (block, rays per thread, LDS box)
+// configuration, into a scratch table so the ring is not touched (the device state is
+// saved and restored around it), and keeps the fastest.
+extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int32_t dtype, const double ego[3],
+                             const void *T, int32_t t_dtype, int32_t chosen[3], float *best_ms) {
+    if (!h || !ego || n <= 0 || (dtype != GVOM_F32 && dtype != GVOM_F64)) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (h->capturing) {
+        h->err = "cannot autotune while capturing";
+        return GVOM_ESTATE;
+    }
+    const Geo &g = h->g;
+    int rc = ensure_capacity(h, n, n);
+    if (rc) return rc;
+    h->diff_n = -1;
+    ScanDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.t_mode = (T && t_dtype != GVOM_NO_TRANSFORM) ? t_dtype : GVOM_NO_TRANSFORM;
+    for (int k = 0; k < 12 && d.t_mode != GVOM_NO_TRANSFORM; ++k)
+        d.T[k] = d.t_mode == GVOM_F32 ? (double)((const float *)T)[k] : ((const double *)T)[k];
+    for (int k = 0; k < 3; ++k) d.ego[k] = ego[k];
+    d.offset = 0;
+    d.count = n;
+    hipStream_t st = h->stream;
+    DevState *saved = nullptr;
+    int32_t *scratch = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int out = GVOM_OK;
+    int bb = h->ray_block, br = h->ray_rpt, bx = h->ray_box;
+    float best = 1e30f;
+    do {
+        if (dalloc(&saved, 1) != hipSuccess || dalloc(&scratch, g.V) != hipSuccess ||
+            hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            h->err = "autotune: allocation failed";
+            out = GVOM_EHIP;
+            break;
+        }
+        if (hipMemcpyAsync(saved, h->st, sizeof(DevState), hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(h->sd, &d, sizeof(ScanDesc), hipMemcpyHostToDevice, st) != hipSuccess ||
+            launch_prepare(dtype, points, h->world, h->k0, h->v0, h->valid, h->sd, 1, n, g, h->st, st) != hipSuccess) {
+            h->err = "autotune: setup failed";
+            out = GVOM_EHIP;
+            break;
+        }
+        const int blocks[3] = {64, 128, 256}, rpts[3] = {1, 2, 4};
+        for (int ib = 0; ib < 3 && out == GVOM_OK; ++ib)
+            for (int ir = 0; ir < 3 && out == GVOM_OK; ++ir)
+                for (int ibox = 0; ibox < 2 && out == GVOM_OK; ++ibox) {
+                    float t_min = 1e30f;
+                    for (int rep = 0; rep < 3; ++rep) {
+                        float t = 0.f;
+                        if (launch_fill_i32(scratch, 0, g.V, st) != hipSuccess || hipEventRecord(e0, st) != hipSuccess ||
+                            launch_raycast(dtype, h->world, h->valid, h->sd, 1, n, g, h->st, scratch, blocks[ib],
+                                           rpts[ir], ibox, 0, st) != hipSuccess ||
+                            hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+                            h->err = "autotune: timing run failed";
+                            out = GVOM_EHIP;
+                            break;
+                        }
+                        if (t < t_min) t_min = t;
+                    }
+                    if (out == GVOM_OK && t_min < best) {
+                        best = t_min;
+                        bb = blocks[ib];
+                        br = rpts[ir];
+                        bx = ibox;
+                    }
+                }
+        if (hipMemcpyAsync(h->st, saved, sizeof(DevState), hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            h->err = "autotune: restoring state failed";
+            out = GVOM_EHIP;
+        }
+    } while (false);
+    if (saved) (void)hipFree(saved);
+    if (scratch) (void)hipFree(scratch);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (out == GVOM_OK) {
+        h->ray_block = bb;
+        h->ray_rpt = br;
+        h->ray_box = bx;
+        if (chosen) {
+            chosen[0] = bb;
+            chosen[1] = br;
+            chosen[2] = bx;
+        }
+        if (best_ms) *best_ms = best;
+    }
+    return out;
+}
+
+extern "C" int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t rays_per_thread, int32_t use_box) {
+    if (!h || (block != 64 && block != 128 && block != 256) || rays_per_thread < 1 || rays_per_thread > 16)
+        return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->ray_block = block;
+    h->ray_rpt = rays_per_thread;
+    h->ray_box = use_box ? 1 : 0;
     return GVOM_OK;
 }

@@ -50,83 +50,107 @@ __device__ __forceinline__ SrcCol source_column(const int32_t *table, const doub
 }
 
 // ------------------------------------------------------------------------------------
-// The slot loop is unrolled in groups of 8 so that a wavefront has all of a group's
-// source loads in flight before it uses the first value.
+// One wavefront handles CPW consecutive columns (lane = z).  The shifted source columns
+// of all sources of a group are computed from uniform data first; then every lane
+// issues its CPW x group loads before it uses the first value, so a wavefront keeps
+// CPW * (group + 1) independent loads in flight.
+constexpr int CPW = 4;
 constexpr int SRC_GROUP = 8;
 
-__global__ __launch_bounds__(256) void k_comb_tables(Geo g, DevState *__restrict__ st,
+__device__ __forceinline__ void col_xy(int64_t c, const Geo &g, int64_t &x, int64_t &y) {
+    x = c / g.XY;
+    y = c - x * g.XY;
+}
+
+__global__ __launch_bounds__(256) void k_comb_tables(Geo g, const DevState *__restrict__ st,
                                                      const int32_t *__restrict__ slot_tables,
                                                      int32_t *__restrict__ comb_tables,
                                                      int32_t *__restrict__ col_count) {
     const int lane = threadIdx.x & 63;
-    const int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t c0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * CPW;
+    if (c0 >= g.X) return;
     const int parity = st->parity, cur = 1 - parity;
     const int filled = st->filled_mask, have_last = st->have_last;
     const double *co = st->slot_origin[st->last_slot];   // combined origin = newest slot's (gvom.py:229)
-    if (c == 0 && lane == 0) {
-        st->comb_origin[cur][0] = co[0];
-        st->comb_origin[cur][1] = co[1];
-        st->comb_origin[cur][2] = co[2];
+    int64_t cx[CPW], cy[CPW];
+    bool cin[CPW];
+#pragma unroll
+    for (int j = 0; j < CPW; ++j) {
+        cin[j] = c0 + j < g.X;
+        col_xy(c0 + j, g, cx[j], cy[j]);
     }
-    if (c >= g.X) return;
-    const int64_t x = c / g.XY, y = c % g.XY;
-    SrcCol lsrc;
-    lsrc.col = nullptr;
-    lsrc.dz = 0;
-    if (have_last) lsrc = source_column(comb_tables + (int64_t)parity * g.V, co, st->comb_origin[parity], x, y, g);
-    int32_t *out = comb_tables + (int64_t)cur * g.V + c * g.Z;
-    uint32_t count = 0;
+    const int32_t *ltab = comb_tables + (int64_t)parity * g.V;
+    uint32_t count[CPW] = {0, 0, 0, 0};
     for (int z0 = 0; z0 < g.Z; z0 += 64) {
         const int z = z0 + lane;
         const bool zin = z < g.Z;
-        // previous combined map, loaded early
-        int32_t vl = 0;
-        bool okl = false;
-        if (lsrc.col && zin) {
-            int zl = z + lsrc.dz;
-            if (zl >= 0 && zl < g.Z) {
-                vl = lsrc.col[zl];
-                okl = true;
+        // previous combined map
+        int32_t vl[CPW];
+        bool okl[CPW];
+        {
+            const int64_t dx = have_last ? (int64_t)(co[0] - st->comb_origin[parity][0]) : 0;
+            const int64_t dy = have_last ? (int64_t)(co[1] - st->comb_origin[parity][1]) : 0;
+            const int dz = have_last ? (int)(co[2] - st->comb_origin[parity][2]) : 0;
+            const int zl = z + dz;
+#pragma unroll
+            for (int j = 0; j < CPW; ++j) {
+                const int64_t ux = cx[j] + dx, uy = cy[j] + dy;
+                okl[j] = have_last && cin[j] && zin && ux >= 0 && ux < g.XY && uy >= 0 && uy < g.XY && zl >= 0 &&
+                         zl < g.Z;
+                vl[j] = okl[j] ? ltab[(ux * g.XY + uy) * g.Z + zl] : 0;
             }
         }
-        uint32_t mask = 0, P = 0;
+        uint32_t mask[CPW] = {0, 0, 0, 0}, P[CPW] = {0, 0, 0, 0};
         for (int s0 = 0; s0 < g.B; s0 += SRC_GROUP) {
-            int32_t v[SRC_GROUP];
-            bool ok[SRC_GROUP];
+            int32_t v[SRC_GROUP][CPW];
 #pragma unroll
             for (int k = 0; k < SRC_GROUP; ++k) {
                 const int s = s0 + k;
-                v[k] = 0;
-                ok[k] = false;
-                if (s < g.B && ((filled >> s) & 1)) {
-                    SrcCol sc = source_column(slot_tables + (int64_t)s * g.V, co, st->slot_origin[s], x, y, g);
-                    const int zs = z + sc.dz;
-                    if (sc.col && zin && zs >= 0 && zs < g.Z) {
-                        v[k] = sc.col[zs];
-                        ok[k] = true;
-                    }
+                const bool use = s < g.B && ((filled >> s) & 1);
+                const double *so = st->slot_origin[use ? s : 0];
+                const int64_t dx = (int64_t)(co[0] - so[0]), dy = (int64_t)(co[1] - so[1]);
+                const int zs = z + (int)(co[2] - so[2]);
+                const int32_t *tab = slot_tables + (int64_t)(use ? s : 0) * g.V;
+#pragma unroll
+                for (int j = 0; j < CPW; ++j) {
+                    const int64_t ux = cx[j] + dx, uy = cy[j] + dy;
+                    const bool ok = use && cin[j] && zin && ux >= 0 && ux < g.XY && uy >= 0 && uy < g.XY && zs >= 0 &&
+                                    zs < g.Z;
+                    // unused sources and off-map cells read as "no data" (0)
+                    v[k][j] = ok ? tab[(ux * g.XY + uy) * g.Z + zs] : 0;
                 }
             }
 #pragma unroll
-            for (int k = 0; k < SRC_GROUP; ++k) {
-                if (!ok[k]) continue;
-                if (v[k] < 0) mask |= 1u << (s0 + k);
-                else P += (uint32_t)v[k];
-            }
+            for (int k = 0; k < SRC_GROUP; ++k)
+#pragma unroll
+                for (int j = 0; j < CPW; ++j) {
+                    if (v[k][j] < 0) mask[j] |= 1u << (s0 + k);
+                    else P[j] += (uint32_t)v[k][j];
+                }
         }
-        if (okl) {
-            if (mask) {
-                if (vl < 0) mask |= 1u << LAST_BIT;
-            } else if (vl < 0) {
-                if (P <= 10u) mask = 1u << LAST_BIT;   // carry-over (gvom.py:1058)
-            } else {
-                P += (uint32_t)vl;
+#pragma unroll
+        for (int j = 0; j < CPW; ++j) {
+            uint32_t m = mask[j], p = P[j];
+            if (okl[j]) {
+                if (m) {
+                    if (vl[j] < 0) m |= 1u << LAST_BIT;
+                } else if (vl[j] < 0) {
+                    if (p <= 10u) m = 1u << LAST_BIT;   // carry-over (gvom.py:1058)
+                } else {
+                    p += (uint32_t)vl[j];
+                }
             }
+            if (zin && cin[j]) comb_tables[(int64_t)cur * g.V + (c0 + j) * g.Z + z] = m ? (int32_t)(GVOM_OCC | m) : (int32_t)p;
+            count[j] += __popcll(__ballot(m != 0));
         }
-        if (zin) out[z] = mask ? (int32_t)(GVOM_OCC | mask) : (int32_t)P;
-        count += __popcll(__ballot(mask != 0));
     }
-    if (lane == 0) col_count[c] = (int32_t)count;
+    if (lane < CPW && cin[lane]) {
+        uint32_t cnt = count[0];
+#pragma unroll
+        for (int j = 1; j < CPW; ++j)
+            if (lane == j) cnt = count[j];
+        col_count[c0 + lane] = (int32_t)cnt;
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -136,9 +160,17 @@ __global__ __launch_bounds__(256) void k_comb_tables(Geo g, DevState *__restrict
 constexpr int COLSCAN = 4096;
 
 __global__ __launch_bounds__(1024) void k_col_scan(int32_t *__restrict__ cnt, int64_t X,
-                                                   int32_t *__restrict__ chunk_tot) {
+                                                   int32_t *__restrict__ chunk_tot, DevState *__restrict__ st) {
     __shared__ int32_t wsum[16];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (blockIdx.x == 0 && t == 0) {
+        // origin of the combined map being built = newest slot's origin (gvom.py:229)
+        const int cur = 1 - st->parity;
+        const double *co = st->slot_origin[st->last_slot];
+        st->comb_origin[cur][0] = co[0];
+        st->comb_origin[cur][1] = co[1];
+        st->comb_origin[cur][2] = co[2];
+    }
     const int64_t i = (int64_t)blockIdx.x * COLSCAN + 4 * t;
     int4 v = make_int4(0, 0, 0, 0);
     if (i < X) v = *reinterpret_cast<const int4 *>(cnt + i);   // X is a multiple of 16
@@ -222,130 +254,158 @@ __device__ __forceinline__ void eigen3(const float m[10], float ev[3]) {
     }
 }
 
-constexpr int MAX_ZCH = 4;   // z_size <= 256
-
-__global__ __launch_bounds__(256) void k_comb_records(Geo g, const DevState *__restrict__ st,
-                                                      const int32_t *__restrict__ slot_tables,
-                                                      const SlotRec *__restrict__ slot_recs, int64_t ncap,
-                                                      int32_t *__restrict__ comb_tables,
-                                                      CombRec *__restrict__ comb_recs, int64_t ccap,
-                                                      const int32_t *__restrict__ col_base,
-                                                      const int32_t *__restrict__ chunk_tot,
-                                                      double *__restrict__ height, double *__restrict__ inferred,
-                                                      int32_t *__restrict__ density) {
+// Cell ids: one wavefront per column assigns ids in z order (first id of the column =
+// earlier chunks' totals + in-chunk exclusive prefix), rewrites the table entry to
+// OCC | id and appends (voxel, source mask) to the compact list.  It also writes the
+// inferred height (lowest free voxel, gvom.py:580-590) and, for columns without an
+// occupied voxel, the height-map default (robot-radius cell corner, gvom.py:561-570).
+// Bit 31 of the list's voxel word marks the lowest occupied voxel of a column.
+__global__ __launch_bounds__(256) void k_comb_ids(Geo g, const DevState *__restrict__ st,
+                                                  int32_t *__restrict__ comb_tables,
+                                                  const int32_t *__restrict__ col_base,
+                                                  const int32_t *__restrict__ chunk_tot,
+                                                  uint2 *__restrict__ occ_list, double *__restrict__ height,
+                                                  double *__restrict__ inferred) {
     const int lane = threadIdx.x & 63;
-    const int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (c >= g.X) return;
-    // first cell id of this column: in-chunk exclusive prefix + totals of earlier chunks
+    const int64_t c0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * CPW;
+    if (c0 >= g.X) return;
+    const int cur = 1 - st->parity;
+    const double *co = st->comb_origin[cur];
+    int32_t *ctab = comb_tables + (int64_t)cur * g.V;
+    // column bases: earlier chunks' totals + in-chunk exclusive prefix (CPW divides COLSCAN)
     uint32_t chunk_pre = 0;
     {
-        const int64_t q = c / COLSCAN;
+        const int64_t q = c0 / COLSCAN;
         for (int64_t k = lane; k < q; k += 64) chunk_pre += (uint32_t)chunk_tot[k];
         for (int off = 32; off > 0; off >>= 1) chunk_pre += __shfl_xor(chunk_pre, off);
     }
+    uint32_t run[CPW];
+    bool any_occ[CPW];
+    int first_free[CPW];
+#pragma unroll
+    for (int j = 0; j < CPW; ++j) {
+        run[j] = c0 + j < g.X ? chunk_pre + (uint32_t)col_base[c0 + j] : 0u;
+        any_occ[j] = false;
+        first_free[j] = -1;
+    }
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int z0 = 0; z0 < g.Z; z0 += 64) {
+        const int z = z0 + lane;
+        int32_t v[CPW];
+#pragma unroll
+        for (int j = 0; j < CPW; ++j) v[j] = (z < g.Z && c0 + j < g.X) ? ctab[(c0 + j) * g.Z + z] : 0;
+#pragma unroll
+        for (int j = 0; j < CPW; ++j) {
+            const bool occ = v[j] < 0;
+            const uint64_t ob = __ballot(occ);
+            const uint64_t fb = __ballot(!occ && v[j] >= 1);
+            if (occ) {
+                const uint32_t id = run[j] + __popcll(ob & lt);
+                const bool lowest = !any_occ[j] && lane == __builtin_ctzll(ob);
+                ctab[(c0 + j) * g.Z + z] = (int32_t)(GVOM_OCC | id);
+                occ_list[id] = make_uint2((uint32_t)((c0 + j) * g.Z + z) | (lowest ? 0x80000000u : 0u),
+                                          (uint32_t)v[j] & MASK31);
+            }
+            run[j] += __popcll(ob);
+            any_occ[j] = any_occ[j] || ob != 0;
+            if (first_free[j] < 0 && fb) first_free[j] = z0 + __builtin_ctzll(fb);
+        }
+    }
+    if (lane < CPW && c0 + lane < g.X) {
+        bool ao = any_occ[0];
+        int ff = first_free[0];
+#pragma unroll
+        for (int j = 1; j < CPW; ++j)
+            if (lane == j) {
+                ao = any_occ[j];
+                ff = first_free[j];
+            }
+        const int64_t c = c0 + lane;
+        inferred[c] = ff >= 0 ? ((double)ff + co[2]) * g.z_res : -1000.0;
+        if (!ao) {
+            int64_t x, y;
+            col_xy(c, g, x, y);
+            const double *ego = st->latest_ego;
+            double h = -1000.0;
+            double xp = ((co[0] + (double)x) * g.xy_res) - ego[0];
+            double yp = ((co[1] + (double)y) * g.xy_res) - ego[1];
+            if (xp * xp + yp * yp <= g.robot_r * g.robot_r) h = ego[2] - g.g2l;
+            height[c] = h;
+        }
+    }
+}
+
+// Record merge per occupied voxel of the compact list (grid-stride).  Sources are taken
+// in slot order, then the previous combined map (gvom.py:280-298).  Per group of
+// MERGE_GROUP slots, all source-table loads are issued, then all record loads, then the
+// merges run in order, so a thread has a group's gathers in flight together.  The
+// thread holding a column's lowest occupied voxel writes its height
+// (min_h + z + oz) * z_res (gvom.py:572-576).
+constexpr int MERGE_GROUP = 4;
+
+__global__ __launch_bounds__(256) void k_comb_merge(Geo g, const DevState *__restrict__ st,
+                                                    const int32_t *__restrict__ slot_tables,
+                                                    const SlotRec *__restrict__ slot_recs, int64_t ncap,
+                                                    const int32_t *__restrict__ comb_tables,
+                                                    CombRec *__restrict__ comb_recs, int64_t ccap,
+                                                    const uint2 *__restrict__ occ_list,
+                                                    const int32_t *__restrict__ chunk_tot, int64_t nchunk,
+                                                    double *__restrict__ height) {
     const int parity = st->parity, cur = 1 - parity;
     const double *co = st->comb_origin[cur];
-    const int64_t x = c / g.XY, y = c % g.XY;
-    int32_t *ctab = comb_tables + (int64_t)cur * g.V + c * g.Z;
+    const int lane = threadIdx.x & 63;
+    uint32_t C = 0;
+    for (int64_t k = lane; k < nchunk; k += 64) C += (uint32_t)chunk_tot[k];
+    for (int off = 32; off > 0; off >>= 1) C += __shfl_xor(C, off);
     CombRec *crec = comb_recs + (int64_t)cur * ccap;
     const CombRec *lrec = comb_recs + (int64_t)parity * ccap;
-    const uint64_t lt = (1ull << lane) - 1ull;
-    uint32_t run = chunk_pre + (uint32_t)col_base[c];
-    int first_occ = -1, first_free = -1;
-    float mh_first = 1.0f;
-    int32_t chit[MAX_ZCH], ctot[MAX_ZCH];
-    bool cocc[MAX_ZCH];
-#pragma unroll
-    for (int k = 0; k < MAX_ZCH; ++k) {
-        const int z = k * 64 + lane;
-        chit[k] = 0;
-        ctot[k] = 0;
-        cocc[k] = false;
-        if (k * 64 >= g.Z) continue;
-        int32_t v = (z < g.Z) ? ctab[z] : 0;
-        const bool occ = v < 0;
-        const uint64_t ob = __ballot(occ);
-        const uint64_t fb = __ballot(!occ && v >= 1);
+    const int32_t *ltab = comb_tables + (int64_t)parity * g.V;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < (int64_t)C; id += stride) {
+        const uint2 e = occ_list[id];
+        const uint32_t lin = e.x & MASK31, mask = e.y;
+        const bool lowest = (e.x >> 31) != 0;
+        const int64_t z = lin % g.Z, c = lin / g.Z, x = c / g.XY, y = c % g.XY;
+        float cm[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        int32_t hit = 0, tot = 0;
         float mh = 1.0f;
-        if (occ) {
-            const uint32_t id = run + __popcll(ob & lt);
-            const uint32_t mask = (uint32_t)v & MASK31;
-            float cm[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-            int32_t hit = 0, tot = 0;
-            // sources in slot order (gvom.py:280-292), then the previous map (295-298)
-            for (int s = 0; s < g.B; ++s) {
-                if (!(mask & (1u << s))) continue;
-                SrcCol sc = source_column(slot_tables + (int64_t)s * g.V, co, st->slot_origin[s], x, y, g);
-                const uint32_t sid = (uint32_t)sc.col[z + sc.dz] & MASK31;
-                const SlotRec &r = slot_recs[(int64_t)s * ncap + sid];
-                merge_record<double>(cm, hit, tot, mh, r.m, r.hit, r.total, r.min_h);
-            }
-            if (mask & (1u << LAST_BIT)) {
-                SrcCol sc = source_column(comb_tables + (int64_t)parity * g.V, co, st->comb_origin[parity], x, y, g);
-                const uint32_t lid = (uint32_t)sc.col[z + sc.dz] & MASK31;
-                const CombRec &r = lrec[lid];
-                merge_record<float>(cm, hit, tot, mh, r.m, r.hit, r.total, r.min_h);
-            }
-            CombRec out;
-            out.hit = hit;
-            out.total = tot;
-            out.min_h = mh;
+        for (int s0 = 0; s0 < g.B; s0 += MERGE_GROUP) {
+            uint32_t sid[MERGE_GROUP];
+            bool use[MERGE_GROUP];
 #pragma unroll
-            for (int j = 0; j < 10; ++j) out.m[j] = cm[j];
-            eigen3(cm, out.ev);
-            crec[id] = out;
-            ctab[z] = (int32_t)(GVOM_OCC | id);
-            chit[k] = hit;
-            ctot[k] = tot;
-            cocc[k] = true;
-        }
-        run += __popcll(ob);
-        if (first_occ < 0 && ob) {
-            const int l0 = __builtin_ctzll(ob);
-            first_occ = k * 64 + l0;
-            mh_first = __shfl(mh, l0);
-        }
-        if (first_free < 0 && fb) first_free = k * 64 + __builtin_ctzll(fb);
-    }
-    // __make_height_map (gvom.py:561-576): robot-radius default from the cell corner and
-    // the latest ego, overridden by the lowest occupied voxel.
-    double h = -1000.0;
-    {
-        const double *ego = st->latest_ego;
-        double xp = ((co[0] + (double)x) * g.xy_res) - ego[0];
-        double yp = ((co[1] + (double)y) * g.xy_res) - ego[1];
-        if (xp * xp + yp * yp <= g.robot_r * g.robot_r) h = ego[2] - g.g2l;
-        if (first_occ >= 0) h = (((double)mh_first + (double)first_occ) + co[2]) * g.z_res;
-    }
-    // positive-obstacle density window (gvom.py:530-555)
-    int32_t dens = 0;
-    {
-        double flo = floor(((h + g.pos_thr) / g.z_res) - co[2]);
-        double fhi = floor(((h + g.robot_h) / g.z_res) - co[2]);
-        if (flo + 1 >= 0 && flo + 1 < g.Z && fhi >= 0 && fhi < g.Z) {
-            const int lo = (int)flo + 1, hi = (int)fhi;
-            int64_t sh = 0, st_ = 0;
-#pragma unroll
-            for (int k = 0; k < MAX_ZCH; ++k) {
-                const int z = k * 64 + lane;
-                if (cocc[k] && z >= lo && z <= hi && chit[k] > 10) {
-                    sh += chit[k];
-                    st_ += ctot[k];
+            for (int k = 0; k < MERGE_GROUP; ++k) {
+                const int s = s0 + k;
+                use[k] = s < g.B && ((mask >> s) & 1u);
+                const int32_t *addr = slot_tables;   // any valid word when the source is unused
+                if (use[k]) {
+                    SrcCol sc = source_column(slot_tables + (int64_t)s * g.V, co, st->slot_origin[s], x, y, g);
+                    addr = sc.col + z + sc.dz;
                 }
+                sid[k] = (uint32_t)*addr & MASK31;
             }
-            for (int off = 32; off > 0; off >>= 1) {
-                sh += __shfl_xor(sh, off);
-                st_ += __shfl_xor(st_, off);
-            }
-            double dsum = (double)sh, nsum = (double)st_;
-            if (nsum > 0.0) dsum /= nsum;
-            dens = (int32_t)(dsum * 100.0);
+            SlotRec r[MERGE_GROUP];
+#pragma unroll
+            for (int k = 0; k < MERGE_GROUP; ++k)
+                r[k] = slot_recs[(int64_t)(use[k] ? s0 + k : 0) * ncap + (use[k] ? sid[k] : 0)];
+#pragma unroll
+            for (int k = 0; k < MERGE_GROUP; ++k)
+                if (use[k]) merge_record<double>(cm, hit, tot, mh, r[k].m, r[k].hit, r[k].total, r[k].min_h);
         }
-    }
-    if (lane == 0) {
-        height[c] = h;
-        inferred[c] = first_free >= 0 ? ((double)first_free + co[2]) * g.z_res : -1000.0;
-        density[c] = dens;
+        if (mask & (1u << LAST_BIT)) {
+            SrcCol sc = source_column(ltab, co, st->comb_origin[parity], x, y, g);
+            const uint32_t lid = (uint32_t)sc.col[z + sc.dz] & MASK31;
+            const CombRec &r = lrec[lid];
+            merge_record<float>(cm, hit, tot, mh, r.m, r.hit, r.total, r.min_h);
+        }
+        CombRec out;
+        out.hit = hit;
+        out.total = tot;
+        out.min_h = mh;
+#pragma unroll
+        for (int j = 0; j < 10; ++j) out.m[j] = cm[j];
+        eigen3(cm, out.ev);
+        crec[id] = out;
+        if (lowest) height[c] = (((double)mh + (double)z) + co[2]) * g.z_res;
     }
 }
 
@@ -362,9 +422,11 @@ __device__ __forceinline__ int first_in(uint64_t m, int lo, int hi) {
     return r ? lo + __builtin_ctzll(r) : -1;
 }
 
-__global__ __launch_bounds__(256) void k_maps2d(Geo g, const double *__restrict__ height,
+__global__ __launch_bounds__(256) void k_maps2d(Geo g, const DevState *__restrict__ st,
+                                                const double *__restrict__ height,
                                                 const double *__restrict__ inferred,
-                                                const int32_t *__restrict__ density,
+                                                const int32_t *__restrict__ comb_tables,
+                                                const CombRec *__restrict__ comb_recs, int64_t ccap,
                                                 double *__restrict__ slope_x, double *__restrict__ slope_y,
                                                 double *__restrict__ rough, double *__restrict__ delta,
                                                 int32_t *__restrict__ pos, int32_t *__restrict__ neg,
@@ -517,9 +579,35 @@ __global__ __launch_bounds__(256) void k_maps2d(Geo g, const double *__restrict_
     slope_y[c0] = sy;
     rough[c0] = rg;
     delta[c0] = dh_out;
-    // __make_positive_obstacle_map slope test (gvom.py:524-526) over the density from the
-    // column pass; __make_negative_obstacle_map (506-512); __make_visability_map (445-452)
-    pos[c0] = (sqrt(sx * sx + sy * sy) >= g.slope_thr) ? 100 : density[c0];
+    // __make_positive_obstacle_map (gvom.py:516-555): slope test, else the density of
+    // occupied voxels with hit > 10 in [floor((h + pos)/zr - oz) + 1, floor((h + robot_h)/zr - oz)]
+    int32_t pv = 100;
+    if (!(sqrt(sx * sx + sy * sy) >= g.slope_thr)) {
+        pv = 0;
+        const int cur = 1 - st->parity;
+        const double oz = st->comb_origin[cur][2];
+        const double flo = floor(((h0 + g.pos_thr) / g.z_res) - oz);
+        const double fhi = floor(((h0 + g.robot_h) / g.z_res) - oz);
+        if (flo + 1 >= 0 && flo + 1 < g.Z && fhi >= 0 && fhi < g.Z) {
+            const int lo = (int)flo + 1, hi = (int)fhi;
+            const int32_t *col = comb_tables + (int64_t)cur * g.V + c0 * g.Z;
+            const CombRec *recs = comb_recs + (int64_t)cur * ccap;
+            double dens = 0.0, nn = 0.0;
+            for (int z = lo; z <= hi; ++z) {
+                const int32_t t = col[z];
+                if (t >= 0) continue;
+                const CombRec &r = recs[(uint32_t)t & MASK31];
+                if (r.hit > 10) {
+                    nn += (double)r.total;
+                    dens += (double)r.hit;
+                }
+            }
+            if (nn > 0.0) dens /= nn;
+            pv = (int32_t)(dens * 100.0);
+        }
+    }
+    // __make_negative_obstacle_map (506-512); __make_visability_map (445-452)
+    pos[c0] = pv;
     neg[c0] = (dh_out > g.neg_thr) ? 100 : 0;
     vis[c0] = (h0 > -1000) ? 1 : 0;
 }
@@ -614,32 +702,36 @@ inline unsigned nblocks(int64_t n, int per) { return (unsigned)((n + per - 1) / 
 
 hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_tables, int32_t *comb_tables,
                               int32_t *col_count, hipStream_t s) {
-    k_comb_tables<<<nblocks(g.X, 4), 256, 0, s>>>(g, st, slot_tables, comb_tables, col_count);
+    k_comb_tables<<<nblocks((g.X + CPW - 1) / CPW, 4), 256, 0, s>>>(g, st, slot_tables, comb_tables, col_count);
     return hipGetLastError();
 }
 
 int64_t col_chunks(int64_t X) { return (X + COLSCAN - 1) / COLSCAN; }
 
-hipError_t launch_scan_columns(int32_t *col_count, int64_t X, int32_t *chunk_tot, hipStream_t s) {
-    k_col_scan<<<(unsigned)col_chunks(X), 1024, 0, s>>>(col_count, X, chunk_tot);
+hipError_t launch_scan_columns(int32_t *col_count, int64_t X, int32_t *chunk_tot, DevState *st, hipStream_t s) {
+    k_col_scan<<<(unsigned)col_chunks(X), 1024, 0, s>>>(col_count, X, chunk_tot, st);
     return hipGetLastError();
 }
 
 hipError_t launch_comb_records(const Geo &g, DevState *st, const int32_t *slot_tables,
                                const SlotRec *slot_recs, int64_t ncap, int32_t *comb_tables,
                                CombRec *comb_recs, int64_t ccap, const int32_t *col_base,
-                               const int32_t *chunk_tot, double *height, double *inferred, int32_t *density,
+                               const int32_t *chunk_tot, uint2 *occ_list, double *height, double *inferred,
                                hipStream_t s) {
-    k_comb_records<<<nblocks(g.X, 4), 256, 0, s>>>(g, st, slot_tables, slot_recs, ncap, comb_tables, comb_recs,
-                                                   ccap, col_base, chunk_tot, height, inferred, density);
+    k_comb_ids<<<nblocks((g.X + CPW - 1) / CPW, 4), 256, 0, s>>>(g, st, comb_tables, col_base, chunk_tot, occ_list,
+                                                                 height, inferred);
+    k_comb_merge<<<2048, 256, 0, s>>>(g, st, slot_tables, slot_recs, ncap, comb_tables, comb_recs, ccap, occ_list,
+                                      chunk_tot, col_chunks(g.X), height);
     return hipGetLastError();
 }
 
-hipError_t launch_maps2d(const Geo &g, const double *height, const double *inferred, const int32_t *density,
-                         double *slope_x, double *slope_y, double *rough, double *delta, int32_t *pos,
-                         int32_t *neg, int32_t *vis, hipStream_t s) {
+hipError_t launch_maps2d(const Geo &g, const DevState *st, const double *height, const double *inferred,
+                         const int32_t *comb_tables, const CombRec *comb_recs, int64_t ccap, double *slope_x,
+                         double *slope_y, double *rough, double *delta, int32_t *pos, int32_t *neg, int32_t *vis,
+                         hipStream_t s) {
     dim3 grid((unsigned)((g.XY + TILE - 1) / TILE), (unsigned)((g.XY + TILE - 1) / TILE));
-    k_maps2d<<<grid, 256, 0, s>>>(g, height, inferred, density, slope_x, slope_y, rough, delta, pos, neg, vis);
+    k_maps2d<<<grid, 256, 0, s>>>(g, st, height, inferred, comb_tables, comb_recs, ccap, slope_x, slope_y, rough,
+                                  delta, pos, neg, vis);
     return hipGetLastError();
 }
 

@@ -1,0 +1,321 @@
+// Differentiable form of the scan stage (no reference counterpart): a scalar map loss
+// over one scan and its gradient with respect to the points and to a 6-DoF se(3)
+// perturbation of the sensor-to-world transform.
+//
+//   L = sum over occupied voxels u of lambda_min(C_u)  +  w_h * Var_c(h_c)
+//
+// C_u is the voxel's neighbourhood population covariance in voxel units (the per-scan
+// metrics of gvom.py:1250-1387), h_c the world z of the lowest point (least local z,
+// first in sorted order on ties) in the lowest occupied voxel of column c, so that
+// h_c = (min_h + z + oz) * z_res of gvom.py:572-576 in exact arithmetic, and the variance
+// runs over columns holding an occupied voxel.  Voxel assignment, counts and the argmin
+// points are those of the forward pass and are held fixed.
+//
+// Gradients (q = p / res component-wise, l = q - u, m_u the voxel mean, v_u the unit
+// eigenvector of lambda_min):
+//   d lambda_min(C_u) / d l_i = (2 / n_u) v_u (v_u . (l_i - m_u))
+//   d Var / d h_c             = 2 (h_c - mean h) / K
+//   T' = exp(xi^) T, p' = exp(xi^) p:  dL/d rho = sum g_i,  dL/d phi = sum p_i x g_i
+// Every reduction is two-stage (per-block partials, then one block in block order), so
+// results are deterministic.
+#include <hip/hip_runtime.h>
+
+#include "gvom_kernels.h"
+
+namespace {
+
+constexpr int RB = 256;   // threads per reduction block
+
+struct DiffRec {
+    double v[3];
+    double m[3];
+    double coef;   // 2 / n
+};
+
+__device__ __forceinline__ double block_sum(double x, double *sh) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if (lane == 0) sh[w] = x;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += sh[k];
+    __syncthreads();
+    return t;   // valid in thread 0
+}
+
+// Smallest eigenvalue and its unit eigenvector of a symmetric 3x3 (f64).  Eigenvalue by
+// the trigonometric closed form; eigenvector = the largest cross product of two rows of
+// (C - lambda I).  A zero matrix (single point) gives lambda = 0 and v = e_z, whose
+// gradient term vanishes because l - m = 0 there.
+__device__ void min_eig(double xx, double xy, double xz, double yy, double yz, double zz, double &lam,
+                        double v[3]) {
+    const double PI = 3.141592653589793;
+    const double p1 = xy * xy + xz * xz + yz * yz;
+    const double q = (xx + yy + zz) / 3.0;
+    if (p1 == 0.0) {
+        lam = fmin(xx, fmin(yy, zz));
+        v[0] = lam == xx ? 1.0 : 0.0;
+        v[1] = (lam == yy && lam != xx) ? 1.0 : 0.0;
+        v[2] = (v[0] == 0.0 && v[1] == 0.0) ? 1.0 : 0.0;
+        return;
+    }
+    const double p2 = (xx - q) * (xx - q) + (yy - q) * (yy - q) + (zz - q) * (zz - q) + 2.0 * p1;
+    const double p = sqrt(p2 / 6.0);
+    const double b0 = (xx - q) / p, b1 = xy / p, b2 = xz / p, b3 = (yy - q) / p, b4 = yz / p, b5 = (zz - q) / p;
+    double r = (b0 * (b3 * b5 - b4 * b4) - b1 * (b1 * b5 - b4 * b2) + b2 * (b1 * b4 - b3 * b2)) / 2.0;
+    r = fmin(1.0, fmax(-1.0, r));
+    const double phi = acos(r) / 3.0;
+    lam = q + 2.0 * p * cos(phi + 2.0 * PI / 3.0);
+    const double r0[3] = {xx - lam, xy, xz}, r1[3] = {xy, yy - lam, yz}, r2[3] = {xz, yz, zz - lam};
+    const double c01[3] = {r0[1] * r1[2] - r0[2] * r1[1], r0[2] * r1[0] - r0[0] * r1[2], r0[0] * r1[1] - r0[1] * r1[0]};
+    const double c02[3] = {r0[1] * r2[2] - r0[2] * r2[1], r0[2] * r2[0] - r0[0] * r2[2], r0[0] * r2[1] - r0[1] * r2[0]};
+    const double c12[3] = {r1[1] * r2[2] - r1[2] * r2[1], r1[2] * r2[0] - r1[0] * r2[2], r1[0] * r2[1] - r1[1] * r2[0]};
+    const double n01 = c01[0] * c01[0] + c01[1] * c01[1] + c01[2] * c01[2];
+    const double n02 = c02[0] * c02[0] + c02[1] * c02[1] + c02[2] * c02[2];
+    const double n12 = c12[0] * c12[0] + c12[1] * c12[1] + c12[2] * c12[2];
+    const double *c = c01;
+    double nn = n01;
+    if (n02 > nn) { c = c02; nn = n02; }
+    if (n12 > nn) { c = c12; nn = n12; }
+    if (nn > 0.0) {
+        const double inv = 1.0 / sqrt(nn);
+        v[0] = c[0] * inv; v[1] = c[1] * inv; v[2] = c[2] * inv;
+    } else {
+        v[0] = 0.0; v[1] = 0.0; v[2] = 1.0;
+    }
+}
+
+// Per occupied voxel of the slot: lambda_min, eigenvector, mean, 2/n; block partial of
+// sum lambda_min.
+__global__ __launch_bounds__(RB) void k_diff_eigen(const DevState *__restrict__ st, int slot, const SlotRec *__restrict__ recs,
+                                                   DiffRec *__restrict__ dr, double *__restrict__ part) {
+    __shared__ double sh[RB / 64];
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t C = st->slot_cells[slot];
+    double lam = 0.0;
+    if (id < C) {
+        const SlotRec &r = recs[id];
+        double v[3];
+        min_eig(r.m[3], r.m[4], r.m[5], r.m[6], r.m[7], r.m[8], lam, v);
+        DiffRec d;
+        d.v[0] = v[0]; d.v[1] = v[1]; d.v[2] = v[2];
+        d.m[0] = r.m[0]; d.m[1] = r.m[1]; d.m[2] = r.m[2];
+        d.coef = 2.0 / r.m[9];
+        dr[id] = d;
+    }
+    const double t = block_sum(lam, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// Per column: lowest occupied voxel, its argmin point, h_c; block partials of (K, sum h).
+__global__ __launch_bounds__(RB) void k_diff_columns(Geo g, const DevState *__restrict__ st, int slot,
+                                                     const int32_t *__restrict__ table,
+                                                     const int32_t *__restrict__ seg_start,
+                                                     const uint32_t *__restrict__ svals,
+                                                     const double *__restrict__ world, double *__restrict__ colh,
+                                                     int32_t *__restrict__ colpt, int32_t *__restrict__ pmark,
+                                                     double *__restrict__ partK, double *__restrict__ partH) {
+    __shared__ double sh[RB / 64];
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double k = 0.0, hsum = 0.0;
+    if (c < g.X) {
+        const int32_t *col = table + (int64_t)slot * g.V + c * g.Z;
+        int32_t pt = -1;
+        double h = 0.0;
+        for (int z = 0; z < g.Z; ++z) {
+            const int32_t t = col[z];
+            if (t >= 0) continue;
+            const int64_t seg = st->first_seg[0] + (int64_t)(t & 0x7FFFFFFF);
+            const int32_t a = seg_start[seg], b = seg_start[seg + 1];
+            double best = 0.0;
+            for (int32_t j = a; j < b; ++j) {
+                const int32_t i = (int32_t)svals[j];
+                const double wz = world[3 * (int64_t)i + 2];
+                if (pt < 0 || wz < best) {
+                    best = wz;
+                    pt = i;
+                }
+            }
+            h = best;
+            break;
+        }
+        colpt[c] = pt;
+        colh[c] = h;
+        if (pt >= 0) {
+            pmark[pt] = (int32_t)c;
+            k = 1.0;
+            hsum = h;
+        }
+    }
+    const double tk = block_sum(k, sh);
+    const double th = block_sum(hsum, sh);
+    if (threadIdx.x == 0) {
+        partK[blockIdx.x] = tk;
+        partH[blockIdx.x] = th;
+    }
+}
+
+// One block: ordered sum of n partials into *out.
+__global__ __launch_bounds__(RB) void k_diff_sum(const double *__restrict__ part, int64_t n, double *__restrict__ out) {
+    __shared__ double sh[RB / 64];
+    // each thread sums a contiguous run of partials, then the block combines in thread order
+    const int64_t per = (n + RB - 1) / RB;
+    double x = 0.0;
+    for (int64_t i = threadIdx.x * per; i < (threadIdx.x + 1) * per && i < n; ++i) x += part[i];
+    const double t = block_sum(x, sh);
+    if (threadIdx.x == 0) *out = t;
+}
+
+// Per column: (h_c - mean)^2 partials.
+__global__ __launch_bounds__(RB) void k_diff_colvar(Geo g, const double *__restrict__ colh,
+                                                    const int32_t *__restrict__ colpt, const double *__restrict__ sums,
+                                                    double *__restrict__ part) {
+    __shared__ double sh[RB / 64];
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double K = sums[1], mean = K > 0 ? sums[2] / K : 0.0;
+    double x = 0.0;
+    if (c < g.X && colpt[c] >= 0) {
+        const double d = colh[c] - mean;
+        x = d * d;
+    }
+    const double t = block_sum(x, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// Final scalars: sums = [sum lambda, K, sum h, sum (h - mean)^2, L, L_rough, L_height]
+__global__ void k_diff_loss(double *__restrict__ sums, double wh) {
+    if (threadIdx.x != 0) return;
+    const double K = sums[1];
+    const double var = K > 0 ? sums[3] / K : 0.0;
+    sums[5] = sums[0];
+    sums[6] = wh * var;
+    sums[4] = sums[5] + sums[6];
+}
+
+// Per point: dL/dp (world) from every occupied voxel whose neighbourhood holds the point,
+// plus the height term for argmin points; the sensor-frame gradient R^T g; block partials
+// of sum g and sum p x g.
+__global__ __launch_bounds__(RB) void k_diff_backward(Geo g, const DevState *__restrict__ st, int slot,
+                                                      const ScanDesc *__restrict__ sd, const double *__restrict__ world,
+                                                      const uint8_t *__restrict__ valid,
+                                                      const int32_t *__restrict__ table,
+                                                      const DiffRec *__restrict__ dr, const int32_t *__restrict__ pmark,
+                                                      const double *__restrict__ colh, const double *__restrict__ sums,
+                                                      double wh, double dL, int64_t n, double *__restrict__ gsensor,
+                                                      double *__restrict__ gworld, double *__restrict__ part6) {
+    __shared__ double sh[RB / 64];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double gx = 0.0, gy = 0.0, gz = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+    if (i < n && valid[i]) {
+        px = world[3 * i];
+        py = world[3 * i + 1];
+        pz = world[3 * i + 2];
+        const ScanDesc &d = sd[0];
+        const double o0 = floor((d.ego[0] / g.xy_res) - (double)g.XY / 2.0);
+        const double o1 = floor((d.ego[1] / g.xy_res) - (double)g.XY / 2.0);
+        const double o2 = floor((d.ego[2] / g.z_res) - (double)g.Z / 2.0);
+        const double q0 = px / g.xy_res - o0, q1 = py / g.xy_res - o1, q2 = pz / g.z_res - o2;
+        const double f0 = floor(q0), f1 = floor(q1), f2 = floor(q2);
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;   // gradient in q (voxel units)
+        if (f0 >= -g.ex - 1 && f0 <= g.XY + g.ex && f1 >= -g.ex - 1 && f1 <= g.XY + g.ex && f2 >= -g.ez - 1 &&
+            f2 <= g.Z + g.ez) {
+            const int64_t b0 = (int64_t)f0, b1 = (int64_t)f1, b2 = (int64_t)f2;
+            const int32_t *tab = table + (int64_t)slot * g.V;
+            for (int64_t ux = b0 - g.ex; ux <= b0 + g.ex; ++ux) {
+                if (ux < 0 || ux >= g.XY) continue;
+                for (int64_t uy = b1 - g.ex; uy <= b1 + g.ex; ++uy) {
+                    if (uy < 0 || uy >= g.XY) continue;
+                    for (int64_t uz = b2 - g.ez; uz <= b2 + g.ez; ++uz) {
+                        if (uz < 0 || uz >= g.Z) continue;
+                        const int32_t t = tab[(ux * g.XY + uy) * g.Z + uz];
+                        if (t >= 0) continue;
+                        const DiffRec &r = dr[t & 0x7FFFFFFF];
+                        const double l0 = q0 - (double)ux - r.m[0], l1 = q1 - (double)uy - r.m[1],
+                                     l2 = q2 - (double)uz - r.m[2];
+                        const double s = r.coef * (r.v[0] * l0 + r.v[1] * l1 + r.v[2] * l2);
+                        a0 += s * r.v[0];
+                        a1 += s * r.v[1];
+                        a2 += s * r.v[2];
+                    }
+                }
+            }
+        }
+        gx = a0 / g.xy_res;
+        gy = a1 / g.xy_res;
+        gz = a2 / g.z_res;
+        const int32_t c = pmark[i];
+        if (c >= 0) {
+            const double K = sums[1], mean = sums[2] / K;
+            gz += wh * 2.0 * (colh[c] - mean) / K;
+        }
+        gx *= dL;
+        gy *= dL;
+        gz *= dL;
+        if (gworld) {
+            gworld[3 * i] = gx;
+            gworld[3 * i + 1] = gy;
+            gworld[3 * i + 2] = gz;
+        }
+        if (gsensor) {
+            const double *T = d.T;   // rows of [R | t]
+            gsensor[3 * i] = T[0] * gx + T[4] * gy + T[8] * gz;
+            gsensor[3 * i + 1] = T[1] * gx + T[5] * gy + T[9] * gz;
+            gsensor[3 * i + 2] = T[2] * gx + T[6] * gy + T[10] * gz;
+        }
+    } else if (i < n) {
+        if (gworld) gworld[3 * i] = gworld[3 * i + 1] = gworld[3 * i + 2] = 0.0;
+        if (gsensor) gsensor[3 * i] = gsensor[3 * i + 1] = gsensor[3 * i + 2] = 0.0;
+    }
+    const double vals[6] = {gx, gy, gz, py * gz - pz * gy, pz * gx - px * gz, px * gy - py * gx};
+    for (int k = 0; k < 6; ++k) {
+        const double t = block_sum(vals[k], sh);
+        if (threadIdx.x == 0) part6[(int64_t)k * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+inline unsigned nb(int64_t n, int per) { return (unsigned)((n + per - 1) / per > 0 ? (n + per - 1) / per : 1); }
+
+}  // namespace
+
+int64_t diff_partials(int64_t n_points, int64_t X, int64_t ncap) {
+    int64_t a = nb(n_points, RB), b = nb(X, RB), c = nb(ncap, RB);
+    int64_t m = a > b ? a : b;
+    m = m > c ? m : c;
+    return 6 * m + 16;
+}
+
+int64_t diff_rec_bytes() { return (int64_t)sizeof(DiffRec); }
+
+// sums: 8 doubles on the device; part: diff_partials() doubles.
+hipError_t launch_diff_forward(const Geo &g, const DevState *st, int slot, const int32_t *table,
+                               const SlotRec *recs, int64_t ncap, const int32_t *seg_start, const uint32_t *svals,
+                               const double *world, int64_t n, void *dr, double *colh, int32_t *colpt,
+                               int32_t *pmark, double *part, double *sums, double wh, hipStream_t s) {
+    const unsigned ne = nb(ncap, RB), nc = nb(g.X, RB);
+    hipError_t e = launch_fill_i32(pmark, -1, n > 0 ? n : 1, s);
+    if (e != hipSuccess) return e;
+    k_diff_eigen<<<ne, RB, 0, s>>>(st, slot, recs, (DiffRec *)dr, part);
+    k_diff_sum<<<1, RB, 0, s>>>(part, ne, sums + 0);
+    k_diff_columns<<<nc, RB, 0, s>>>(g, st, slot, table, seg_start, svals, world, colh, colpt, pmark, part,
+                                     part + nc);
+    k_diff_sum<<<1, RB, 0, s>>>(part, nc, sums + 1);
+    k_diff_sum<<<1, RB, 0, s>>>(part + nc, nc, sums + 2);
+    k_diff_colvar<<<nc, RB, 0, s>>>(g, colh, colpt, sums, part);
+    k_diff_sum<<<1, RB, 0, s>>>(part, nc, sums + 3);
+    k_diff_loss<<<1, 64, 0, s>>>(sums, wh);
+    return hipGetLastError();
+}
+
+// grad6: 6 doubles on the device (d/d rho, d/d phi).
+hipError_t launch_diff_backward(const Geo &g, const DevState *st, int slot, const ScanDesc *sd,
+                                const double *world, const uint8_t *valid, const int32_t *table, const void *dr,
+                                const int32_t *pmark, const double *colh, const double *sums, double wh, double dL,
+                                int64_t n, double *gsensor, double *gworld, double *part, double *grad6,
+                                hipStream_t s) {
+    const unsigned np = nb(n, RB);
+    k_diff_backward<<<np, RB, 0, s>>>(g, st, slot, sd, world, valid, table, (const DiffRec *)dr, pmark, colh, sums,
+                                      wh, dL, n, gsensor, gworld, part);
+    for (int k = 0; k < 6; ++k) k_diff_sum<<<1, RB, 0, s>>>(part + (int64_t)k * np, np, grad6 + k);
+    return hipGetLastError();
+}

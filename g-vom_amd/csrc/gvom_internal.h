@@ -50,6 +50,9 @@ struct DevState {
     int32_t slot_cells[GVOM_MAX_BUFFER + 2];
     int32_t first_seg[GVOM_MAX_BATCH];
     int32_t inb_end[GVOM_MAX_BATCH];
+    int32_t scan_pos[GVOM_MAX_BATCH];        // first sorted position of each scan's valid keys
+    int32_t scan_valid[GVOM_MAX_BATCH];      // valid keys per scan
+    int32_t chunk_off[GVOM_MAX_BATCH + 1];   // moment chunks before each scan (scan-relative chunking)
     double slot_origin[GVOM_MAX_BUFFER][3];
     double comb_origin[2][3];
     double latest_ego[3];

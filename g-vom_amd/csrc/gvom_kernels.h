@@ -14,7 +14,7 @@ hipError_t launch_prepare(int dtype, const void *in, void *world, uint32_t *keys
 // Exact-rule ray march counting passes into the slot tables (gvom.py:1174-1231).
 hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, const ScanDesc *sd,
                           int S, int64_t NT, const Geo &g, const DevState *st, int32_t *slot_tables,
-                          int block, int rays_per_thread, hipStream_t s);
+                          int block, int rays_per_thread, int use_box, int force_slot, hipStream_t s);
 // Stable LSD radix sort of (key, val) on the low `bits` key bits.  On return *kout/*vout
 // point at whichever buffer pair holds the result.
 hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits,
@@ -27,7 +27,7 @@ hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start,
 // marks + hit/total/min-height records.
 hipError_t launch_moments(int dtype, const void *world, const uint32_t *skeys, const uint32_t *svals,
                           const int32_t *seg_start, const uint32_t *seg_key, int64_t seg_cap,
-                          const ScanDesc *sd, const Geo &g, DevState *st, int32_t *slot_tables,
+                          const ScanDesc *sd, int S, const Geo &g, DevState *st, int32_t *slot_tables,
                           SlotRec *slot_recs, int64_t ncap, Moment *part, Moment *mom, hipStream_t s);
 // Neighbourhood mean/covariance per occupied voxel (gvom.py:1250-1387).
 hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t seg_cap, const Geo &g,
@@ -39,15 +39,17 @@ hipError_t launch_commit_process(DevState *st, int S, int B, hipStream_t s);
 hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_tables,
                               int32_t *comb_tables, int32_t *col_count, hipStream_t s);
 int64_t col_chunks(int64_t X);
-hipError_t launch_scan_columns(int32_t *col_count, int64_t X, int32_t *chunk_tot, hipStream_t s);
+hipError_t launch_scan_columns(int32_t *col_count, int64_t X, int32_t *chunk_tot, DevState *st, hipStream_t s);
+// Cell ids + compact list + inferred height, then the record merge + eigenvalues + heights.
 hipError_t launch_comb_records(const Geo &g, DevState *st, const int32_t *slot_tables,
                                const SlotRec *slot_recs, int64_t ncap, int32_t *comb_tables,
                                CombRec *comb_recs, int64_t ccap, const int32_t *col_base,
-                               const int32_t *chunk_tot, double *height, double *inferred, int32_t *density,
+                               const int32_t *chunk_tot, uint2 *occ_list, double *height, double *inferred,
                                hipStream_t s);
-hipError_t launch_maps2d(const Geo &g, const double *height, const double *inferred,
-                         const int32_t *density, double *slope_x, double *slope_y, double *rough,
-                         double *delta, int32_t *pos, int32_t *neg, int32_t *vis, hipStream_t s);
+hipError_t launch_maps2d(const Geo &g, const DevState *st, const double *height, const double *inferred,
+                         const int32_t *comb_tables, const CombRec *comb_recs, int64_t ccap, double *slope_x,
+                         double *slope_y, double *rough, double *delta, int32_t *pos, int32_t *neg, int32_t *vis,
+                         hipStream_t s);
 hipError_t launch_commit_combine(DevState *st, const int32_t *chunk_tot, int64_t X, hipStream_t s);
 
 // ---- export (gvom_combine.hip) -----------------------------------------------------------

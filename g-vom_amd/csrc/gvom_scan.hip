@@ -153,27 +153,56 @@ __global__ __launch_bounds__(256) void k_prepare(const P *__restrict__ in, P *__
 // |pt| < 2^23 (ego is restricted to |ego/res| < 2^22), and the ego starts inside the
 // map, so a ray leaves the map within max(xy, z) + 1 steps; maxit bounds the loop at
 // that number plus one.
-template <typename P>
+// Near-sensor privatisation: every ray starts at the ego, so the voxels around it are
+// the most contended counters.  With USE_BOX, a block keeps a BOX_XY x BOX_XY x BOX_Z
+// box of counters centred on the ego voxel of its first ray's scan in LDS; steps of
+// that scan inside the box add there, and the block flushes the non-zero counters
+// with one global atomic each at its end.
+constexpr int BOX_XY = 32, BOX_Z = 16;
+constexpr int BOX_N = BOX_XY * BOX_XY * BOX_Z;   // 64 KiB of int32 counters
+
+template <typename P, bool USE_BOX>
 __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
                                                  const uint8_t *__restrict__ valid,
                                                  const ScanDesc *__restrict__ sd, int S, int64_t NT,
                                                  Geo g, const DevState *__restrict__ st,
-                                                 int32_t *__restrict__ tables, int maxit, int rpt) {
+                                                 int32_t *__restrict__ tables, int maxit, int rpt, int force_slot) {
+    extern __shared__ int32_t box[];
     const int lane = threadIdx.x & 63;
     const int next = st->next_slot;
+    // force_slot >= 0 (autotuning) writes every scan into table force_slot instead of its ring slot
+#define RAY_SLOT(s) (force_slot >= 0 ? force_slot : ((next + (s)) % g.B))
+    // the block's box: scan of its first ray, centred on that scan's ego voxel
+    int bs = 0;
+    int64_t bx0 = 0, by0 = 0, bz0 = 0;
+    uint32_t bbase = 0;
+    (void)bbase;
+    if (USE_BOX) {
+        const int64_t i0 = (int64_t)blockIdx.x * rpt * blockDim.x;
+        bs = scan_of(sd, S, i0 < NT ? i0 : (NT > 0 ? NT - 1 : 0));
+        double o[3];
+        scan_origin(sd[bs], g, o);
+        bx0 = (int64_t)floor((double)(float)(sd[bs].ego[0] / g.xy_res) - o[0]) - BOX_XY / 2;
+        by0 = (int64_t)floor((double)(float)(sd[bs].ego[1] / g.xy_res) - o[1]) - BOX_XY / 2;
+        bz0 = (int64_t)floor((double)(float)(sd[bs].ego[2] / g.z_res) - o[2]) - BOX_Z / 2;
+        bbase = (uint32_t)(RAY_SLOT(bs) * g.V);
+        for (int k = threadIdx.x; k < BOX_N; k += blockDim.x) box[k] = 0;
+        __syncthreads();
+    }
     for (int rep = 0; rep < rpt; ++rep) {
         int64_t i = ((int64_t)blockIdx.x * rpt + rep) * blockDim.x + threadIdx.x;
-        bool alive = false;
+        bool alive = false, inbox_scan = false;
         float p0 = 0.f, p1 = 0.f, p2 = 0.f, inc0 = 0.f, inc1 = 0.f, inc2 = 0.f;
         double length = 0.0, rl1 = 0.0, dlen = 0.0, o0 = 0.0, o1 = 0.0, o2 = 0.0;
         uint32_t base = 0;
         if (i < NT && valid[i]) {
             int s = scan_of(sd, S, i);
+            inbox_scan = USE_BOX && s == bs;
             const ScanDesc &d = sd[s];
             double o[3];
             scan_origin(d, g, o);
             o0 = o[0]; o1 = o[1]; o2 = o[2];
-            base = (uint32_t)(((next + s) % g.B) * g.V);
+            base = (uint32_t)(RAY_SLOT(s) * g.V);
             float pt0 = (float)(d.ego[0] / g.xy_res);
             float pt1 = (float)(d.ego[1] / g.xy_res);
             float pt2 = (float)(d.ego[2] / g.z_res);
@@ -205,6 +234,7 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
         for (int it = 0; it < maxit; ++it) {
             if (__ballot(alive) == 0ull) break;
             uint32_t key = 0xFFFFFFFFu - (uint32_t)lane;   // unique per dead lane
+            int lbox = -1;
             if (alive) {
                 p0 += inc0;
                 p1 += inc1;
@@ -215,18 +245,27 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
                 if (fx < 0 || fx >= g.XY || fy < 0 || fy >= g.XY || fz < 0 || fz >= g.Z) {
                     alive = false;
                 } else {
-                    key = base + (uint32_t)lin_of((int64_t)fx, (int64_t)fy, (int64_t)fz, g);
+                    // in-map, so each index fits in 32 bits
+                    const int32_t ix = (int32_t)fx, iy = (int32_t)fy, iz = (int32_t)fz;
+                    key = base + ((uint32_t)ix * (uint32_t)g.XY + (uint32_t)iy) * (uint32_t)g.Z + (uint32_t)iz;
+                    if (USE_BOX && inbox_scan) {
+                        const int32_t lx = ix - (int32_t)bx0, ly = iy - (int32_t)by0, lz = iz - (int32_t)bz0;
+                        if (lx >= 0 && lx < BOX_XY && ly >= 0 && ly < BOX_XY && lz >= 0 && lz < BOX_Z)
+                            lbox = (int)((lx * BOX_XY + ly) * BOX_Z + lz);
+                    }
                 }
             }
-            // merge runs of equal keys among adjacent lanes
-            uint32_t prev = __shfl_up(key, 1);
+            // merge runs of equal keys among adjacent lanes; the previous lane's key comes
+            // from a DPP wave shift right by one (lane 0 receives a dead-lane sentinel)
+            const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)key, 0x138, 0xF, 0xF, false);
             bool is_voxel = key < 0x80000000u;
             bool cont = (lane > 0) && (prev == key) && is_voxel;
             uint64_t same = __ballot(cont);
             if (is_voxel && !cont) {
                 uint64_t rest = (lane == 63) ? 0ull : (same >> (lane + 1));
                 int len = 1 + __builtin_ctzll(~rest);
-                __hip_atomic_fetch_add(tables + key, len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (USE_BOX && lbox >= 0) atomicAdd(&box[lbox], len);
+                else __hip_atomic_fetch_add(tables + key, len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             if (alive) {
                 length += dlen;
@@ -234,11 +273,24 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
             }
         }
     }
+    if (USE_BOX) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < BOX_N; k += blockDim.x) {
+            const int32_t v = box[k];
+            if (v == 0) continue;
+            const int64_t lz = k % BOX_Z, lc = k / BOX_Z, ly = lc % BOX_XY, lx = lc / BOX_XY;
+            const uint32_t key = bbase + (uint32_t)lin_of(bx0 + lx, by0 + ly, bz0 + lz, g);
+            __hip_atomic_fetch_add(tables + key, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+#undef RAY_SLOT
 }
 
 // ------------------------------------------------------------------------------------
 // Radix sort: 8-bit digits, tiles of 2048 keys (256 threads x 8).
 constexpr int RS_TILE = 2048;
+// Points per moment chunk (see k_mom_part).
+constexpr int MCH = 16;
 
 __global__ __launch_bounds__(256) void k_rs_hist(const uint32_t *__restrict__ keys, int64_t n, int shift,
                                                  uint32_t *__restrict__ hist) {
@@ -269,11 +321,13 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t *__restrict__
         // global offset of digit d for this block: keys of smaller digits in all blocks plus
         // keys of digit d in earlier blocks (hist is block-major, so the reads coalesce)
         const int d = threadIdx.x;
+        const int me = (int)blockIdx.x;
         uint32_t before = 0, total = 0;
+#pragma unroll 16
         for (int b = 0; b < nblk; ++b) {
-            uint32_t c = hist[(int64_t)b * 256 + d];
+            const uint32_t c = hist[(int64_t)b * 256 + d];
             total += c;
-            if (b < (int)blockIdx.x) before += c;
+            before += (b < me) ? c : 0u;
         }
         uint32_t v = total;
         for (int off = 1; off < 64; off <<= 1) {
@@ -343,26 +397,44 @@ __device__ __forceinline__ bool is_head(const uint32_t *__restrict__ k, int64_t 
     return key != GVOM_NOKEY && (i == 0 || k[i - 1] != key);
 }
 
-__global__ __launch_bounds__(256) void k_seg_count(const uint32_t *__restrict__ k, int64_t n,
+// Per block of RS_TILE sorted keys: [heads, valid keys, then per scan s (in-map heads,
+// halo heads, valid keys)], W = 2 + 3 * S words.
+__global__ __launch_bounds__(256) void k_seg_count(const uint32_t *__restrict__ k, int64_t n, int S, Geo g,
                                                    uint32_t *__restrict__ blk) {
-    __shared__ uint32_t c;
-    if (threadIdx.x == 0) c = 0;
+    __shared__ uint32_t c[2 + 3 * GVOM_MAX_BATCH];
+    const int W = 2 + 3 * S;
+    if (threadIdx.x < W) c[threadIdx.x] = 0;
     __syncthreads();
     int64_t base = (int64_t)blockIdx.x * RS_TILE;
-    uint32_t mine = 0;
+    uint32_t heads = 0, valid = 0;
     for (int j = 0; j < 8; ++j) {
         int64_t i = base + threadIdx.x * 8 + j;
-        if (i < n && is_head(k, i)) ++mine;
+        if (i >= n) break;
+        const uint32_t key = k[i];
+        if (key == GVOM_NOKEY) continue;
+        ++valid;
+        const int sc = (int)(key >> g.VB);
+        if (S > 1) atomicAdd(&c[4 + 3 * sc], 1u);
+        if (i == 0 || k[i - 1] != key) {
+            ++heads;
+            const int halo = (int)((key >> (g.VB - 1)) & 1u);
+            atomicAdd(&c[2 + 3 * sc + halo], 1u);
+        }
     }
-    // wave reduce then one LDS atomic per wave
-    for (int off = 32; off > 0; off >>= 1) mine += __shfl_down(mine, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&c, mine);
+    for (int off = 32; off > 0; off >>= 1) {
+        heads += __shfl_down(heads, off);
+        valid += __shfl_down(valid, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&c[0], heads);
+        atomicAdd(&c[1], valid);
+    }
     __syncthreads();
-    if (threadIdx.x == 0) blk[blockIdx.x] = c;
+    if (threadIdx.x < W) blk[(int64_t)blockIdx.x * W + threadIdx.x] = c[threadIdx.x];
 }
 
 __global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *__restrict__ k, int64_t n,
-                                                     const uint32_t *__restrict__ blk,
+                                                     const uint32_t *__restrict__ blk, int W,
                                                      int32_t *__restrict__ seg_start,
                                                      uint32_t *__restrict__ seg_key) {
     __shared__ uint32_t wsum[4];
@@ -370,7 +442,7 @@ __global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *__restrict_
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     // heads in earlier blocks
     uint32_t pre = 0;
-    for (int b = t; b < (int)blockIdx.x; b += 256) pre += blk[b];
+    for (int b = t; b < (int)blockIdx.x; b += 256) pre += blk[(int64_t)b * W];
     for (int off = 32; off > 0; off >>= 1) pre += __shfl_down(pre, off);
     if (lane == 0) wpre[w] = pre;
     int64_t base = (int64_t)blockIdx.x * RS_TILE + t * 8;
@@ -402,46 +474,40 @@ __global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *__restrict_
     }
 }
 
-__device__ __forceinline__ int32_t lower_bound_u32(const uint32_t *__restrict__ a, int32_t n, uint32_t v) {
-    int32_t lo = 0, hi = n;
-    while (lo < hi) {
-        int32_t mid = (lo + hi) >> 1;
-        if (a[mid] < v) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// One block: nvalid, end sentinel of the last segment, per-scan [first, in-bounds end).
-__global__ __launch_bounds__(64) void k_seg_bounds(const uint32_t *__restrict__ k, int64_t n,
-                                                   const uint32_t *__restrict__ blk, int nblk,
-                                                   int32_t *__restrict__ seg_start,
-                                                   const uint32_t *__restrict__ seg_key, int S, Geo g,
-                                                   DevState *__restrict__ st) {
+// One block: totals of the per-block counts -> nseg, nvalid, end sentinel of the last
+// segment, per-scan [first segment, end of in-map segments) and slot cell counts.
+// Segments are ordered by scan, and within a scan in-map keys precede halo keys.
+__global__ __launch_bounds__(64) void k_seg_bounds(const uint32_t *__restrict__ blk, int nblk, int S, Geo g,
+                                                   int32_t *__restrict__ seg_start, DevState *__restrict__ st) {
+    __shared__ uint32_t tot[2 + 3 * GVOM_MAX_BATCH];
     const int t = threadIdx.x;
-    uint32_t tot = 0;
-    for (int b = t; b < nblk; b += 64) tot += blk[b];
-    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-    const int32_t nseg = (int32_t)tot;
-    if (t == 0) st->nseg = nseg;
-    if (t == 0) {
-        int64_t lo = 0, hi = n;
-        while (lo < hi) {
-            int64_t mid = (lo + hi) >> 1;
-            if (k[mid] < GVOM_NOKEY) lo = mid + 1;
-            else hi = mid;
-        }
-        st->nvalid = (int32_t)lo;
-        seg_start[nseg] = (int32_t)lo;
+    const int W = 2 + 3 * S;
+    for (int col = t; col < W; col += 64) {
+        uint32_t a = 0;
+#pragma unroll 8
+        for (int b = 0; b < nblk; ++b) a += blk[(int64_t)b * W + col];
+        tot[col] = a;
     }
-    if (t < S) {
-        uint32_t sk = (uint32_t)t << g.VB;
-        int32_t f = lower_bound_u32(seg_key, nseg, sk);
-        int32_t e = lower_bound_u32(seg_key, nseg, sk | (1u << (g.VB - 1)));
-        st->first_seg[t] = f;
-        st->inb_end[t] = e;
-        st->slot_cells[(st->next_slot + t) % g.B] = e - f;
+    __syncthreads();
+    if (t != 0) return;
+    st->nseg = (int32_t)tot[0];
+    st->nvalid = (int32_t)tot[1];
+    seg_start[tot[0]] = (int32_t)tot[1];
+    int32_t f = 0, pos = 0, ch = 0;
+    for (int s = 0; s < S; ++s) {
+        const int32_t in = (int32_t)tot[2 + 3 * s], halo = (int32_t)tot[3 + 3 * s];
+        const int32_t nv = S > 1 ? (int32_t)tot[4 + 3 * s] : (int32_t)tot[1];
+        st->first_seg[s] = f;
+        st->inb_end[s] = f + in;
+        st->slot_cells[(st->next_slot + s) % g.B] = in;
+        st->scan_pos[s] = pos;
+        st->scan_valid[s] = nv;
+        st->chunk_off[s] = ch;
+        f += in + halo;
+        pos += nv;
+        ch += (nv + MCH - 1) / MCH;
     }
+    st->chunk_off[S] = ch;
 }
 
 // ------------------------------------------------------------------------------------
@@ -456,7 +522,6 @@ __global__ __launch_bounds__(64) void k_seg_bounds(const uint32_t *__restrict__ 
 // k_mom_seg is also where the slot record is born: hit = points in the voxel
 // (gvom.py:1170), total = ray passes + hits (1171, 1229), min_h = min local z over the
 // voxel's points (1390-1421), and the table entry becomes OCC | id.
-constexpr int MCH = 16;
 
 __device__ __forceinline__ void key_voxel(uint32_t key, const Geo &g, int &s, bool &halo, int64_t &wx, int64_t &wy,
                                           int64_t &wz) {
@@ -499,12 +564,17 @@ __device__ __forceinline__ void store_moment(Moment *m, const MomAcc &a) {
 template <typename P>
 __global__ __launch_bounds__(256) void k_mom_part(const P *__restrict__ world, const uint32_t *__restrict__ skeys,
                                                   const uint32_t *__restrict__ svals,
-                                                  const ScanDesc *__restrict__ sd, Geo g,
+                                                  const ScanDesc *__restrict__ sd, int S, Geo g,
                                                   const DevState *__restrict__ st, Moment *__restrict__ part) {
-    const int64_t j0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * MCH;
-    const int64_t nv = st->nvalid;
-    if (j0 >= nv) return;
-    const int64_t j1 = j0 + MCH < nv ? j0 + MCH : nv;
+    // chunks are cut relative to each scan's first sorted position, so a scan's sums do not
+    // depend on the other scans of its batch
+    const int32_t t = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= st->chunk_off[S]) return;
+    int sc = 0;
+    while (sc + 1 < S && st->chunk_off[sc + 1] <= t) ++sc;
+    const int64_t sbeg = st->scan_pos[sc], send = sbeg + st->scan_valid[sc];
+    const int64_t j0 = sbeg + (int64_t)(t - st->chunk_off[sc]) * MCH;
+    const int64_t j1 = j0 + MCH < send ? j0 + MCH : send;
     uint32_t cur = GVOM_NOKEY;
     MomAcc a;
     a.zero();
@@ -545,13 +615,15 @@ __global__ __launch_bounds__(256) void k_mom_seg(const int32_t *__restrict__ seg
     const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= st->nseg) return;
     const int32_t a = seg_start[gi], b = seg_start[gi + 1];
+    const int64_t sbeg = st->scan_pos[(int)(seg_key[gi] >> g.VB)];
+    const int64_t ra = a - sbeg, rb = b - sbeg;   // scan-relative positions
     Moment m;
     for (int k = 0; k < 3; ++k) m.S[k] = 0.0;
     for (int k = 0; k < 6; ++k) m.Q[k] = 0.0;
     int32_t n = 0;
     float mh = 1.0f;
-    for (int64_t c = a / MCH; c <= (int64_t)(b - 1) / MCH; ++c) {
-        const int64_t pos = ((c + 1) * MCH < b ? (c + 1) * MCH : b) - 1;
+    for (int64_t c = ra / MCH; c <= (rb - 1) / MCH; ++c) {
+        const int64_t pos = sbeg + ((c + 1) * MCH < rb ? (c + 1) * MCH : rb) - 1;
         const Moment &p = part[pos];
         for (int k = 0; k < 3; ++k) m.S[k] += p.S[k];
         for (int k = 0; k < 6; ++k) m.Q[k] += p.Q[k];
@@ -743,16 +815,28 @@ hipError_t launch_prepare(int dtype, const void *in, void *world, uint32_t *keys
 
 hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, const ScanDesc *sd,
                           int S, int64_t NT, const Geo &g, const DevState *st, int32_t *slot_tables,
-                          int block, int rpt, hipStream_t s) {
+                          int block, int rpt, int use_box, int force_slot, hipStream_t s) {
     if (NT <= 0) return hipSuccess;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
     if (rpt < 1) rpt = 1;
     int maxit = (g.XY > g.Z ? g.XY : g.Z) + 2;
     unsigned nb = nblocks(NT, block * rpt);
-    if (dtype == GVOM_F32)
-        k_raycast<float><<<nb, block, 0, s>>>((const float *)world, valid, sd, S, NT, g, st, slot_tables, maxit, rpt);
-    else
-        k_raycast<double><<<nb, block, 0, s>>>((const double *)world, valid, sd, S, NT, g, st, slot_tables, maxit, rpt);
+    const size_t lds = use_box ? BOX_N * sizeof(int32_t) : 0;
+    if (dtype == GVOM_F32) {
+        if (use_box)
+            k_raycast<float, true><<<nb, block, lds, s>>>((const float *)world, valid, sd, S, NT, g, st, slot_tables,
+                                                          maxit, rpt, force_slot);
+        else
+            k_raycast<float, false><<<nb, block, 0, s>>>((const float *)world, valid, sd, S, NT, g, st, slot_tables,
+                                                         maxit, rpt, force_slot);
+    } else {
+        if (use_box)
+            k_raycast<double, true><<<nb, block, lds, s>>>((const double *)world, valid, sd, S, NT, g, st,
+                                                           slot_tables, maxit, rpt, force_slot);
+        else
+            k_raycast<double, false><<<nb, block, 0, s>>>((const double *)world, valid, sd, S, NT, g, st,
+                                                          slot_tables, maxit, rpt, force_slot);
+    }
     return hipGetLastError();
 }
 
@@ -779,23 +863,24 @@ hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, in
 hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start, uint32_t *seg_key,
                            uint32_t *blk, int S, const Geo &g, DevState *st, hipStream_t s) {
     int nblk = n > 0 ? (int)nblocks(n, RS_TILE) : 0;
+    const int W = 2 + 3 * S;
     if (n > 0) {
-        k_seg_count<<<nblk, 256, 0, s>>>(skeys, n, blk);
-        k_seg_compact<<<nblk, 256, 0, s>>>(skeys, n, blk, seg_start, seg_key);
+        k_seg_count<<<nblk, 256, 0, s>>>(skeys, n, S, g, blk);
+        k_seg_compact<<<nblk, 256, 0, s>>>(skeys, n, blk, W, seg_start, seg_key);
     }
-    k_seg_bounds<<<1, 64, 0, s>>>(skeys, n, blk, nblk, seg_start, seg_key, S, g, st);
+    k_seg_bounds<<<1, 64, 0, s>>>(blk, nblk, S, g, seg_start, st);
     return hipGetLastError();
 }
 
 hipError_t launch_moments(int dtype, const void *world, const uint32_t *skeys, const uint32_t *svals,
                           const int32_t *seg_start, const uint32_t *seg_key, int64_t seg_cap,
-                          const ScanDesc *sd, const Geo &g, DevState *st, int32_t *slot_tables,
+                          const ScanDesc *sd, int S, const Geo &g, DevState *st, int32_t *slot_tables,
                           SlotRec *slot_recs, int64_t ncap, Moment *part, Moment *mom, hipStream_t s) {
-    unsigned nbp = nblocks((seg_cap + MCH - 1) / MCH, 256);
+    unsigned nbp = nblocks((seg_cap + MCH - 1) / MCH + S, 256);
     if (dtype == GVOM_F32)
-        k_mom_part<float><<<nbp, 256, 0, s>>>((const float *)world, skeys, svals, sd, g, st, part);
+        k_mom_part<float><<<nbp, 256, 0, s>>>((const float *)world, skeys, svals, sd, S, g, st, part);
     else
-        k_mom_part<double><<<nbp, 256, 0, s>>>((const double *)world, skeys, svals, sd, g, st, part);
+        k_mom_part<double><<<nbp, 256, 0, s>>>((const double *)world, skeys, svals, sd, S, g, st, part);
     k_mom_seg<<<nblocks(seg_cap, 256), 256, 0, s>>>(seg_start, seg_key, part, g, st, slot_tables, slot_recs, ncap,
                                                     mom);
     return hipGetLastError();

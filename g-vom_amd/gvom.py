@@ -130,6 +130,52 @@ class Gvom:
         nat.check(self._lib.gvom_process_batch(self._h, nat.ptr(pts), nat.ptr(off), S, dt, nat.ptr(ego),
                                                nat.ptr(T), tdt, 0), self._h)
 
+    def process_batch_device(self, dev_points, offsets, egos, transforms=None, dtype=np.float32):
+        """process_batch on a flat point array already in device memory (pointer as int,
+        c_void_p or DeviceArray).  Does not synchronise; capturable in a graph."""
+        ptr = dev_points.ptr if isinstance(dev_points, nat.DeviceArray) else ctypes.c_void_p(int(dev_points))
+        off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+        ego = np.ascontiguousarray(np.asarray(egos, dtype=np.float64).reshape(-1, 3))
+        S = off.shape[0] - 1
+        T, tdt = None, nat.GVOM_NO_TRANSFORM
+        if transforms is not None:
+            T = np.asarray(transforms)
+            if T.dtype not in (np.float32, np.float64):
+                T = T.astype(np.float64)
+            T = np.ascontiguousarray(T).reshape(S, 4, 4)
+            tdt = nat.GVOM_F32 if T.dtype == np.float32 else nat.GVOM_F64
+        dt = nat.GVOM_F32 if np.dtype(dtype) == np.float32 else nat.GVOM_F64
+        self.ego_position = tuple(float(v) for v in ego[-1])
+        nat.check(self._lib.gvom_process_batch(self._h, ptr, nat.ptr(off), S, dt, nat.ptr(ego), nat.ptr(T), tdt, 1),
+                  self._h)
+
+    def autotune(self, dev_points, n, ego_position, transform=None, dtype=np.float32):
+        """Pick the ray-cast launch configuration by timing it on one device-resident scan.
+        Returns (block, rays_per_thread, use_box, best_ms)."""
+        ptr = dev_points.ptr if isinstance(dev_points, nat.DeviceArray) else ctypes.c_void_p(int(dev_points))
+        ego = np.ascontiguousarray(np.asarray(ego_position, dtype=np.float64).reshape(3))
+        T, tdt = _as_transform(transform)
+        dt = nat.GVOM_F32 if np.dtype(dtype) == np.float32 else nat.GVOM_F64
+        chosen = (ctypes.c_int32 * 3)()
+        best = ctypes.c_float()
+        nat.check(self._lib.gvom_autotune(self._h, ptr, int(n), dt, nat.ptr(ego), nat.ptr(T), tdt, chosen,
+                                          ctypes.byref(best)), self._h)
+        return chosen[0], chosen[1], chosen[2], best.value
+
+    def set_raycast_config(self, block=256, rays_per_thread=1, use_box=False):
+        nat.check(self._lib.gvom_set_raycast_config(self._h, int(block), int(rays_per_thread), 1 if use_box else 0),
+                  self._h)
+
+    # -- hipGraph capture ------------------------------------------------------------
+    def graph_begin(self, max_scans=64):
+        """Start recording device-pointer process calls and combine_maps_device calls."""
+        nat.check(self._lib.gvom_graph_begin(self._h, int(max_scans)), self._h)
+
+    def graph_end(self):
+        g = ctypes.c_void_p()
+        nat.check(self._lib.gvom_graph_end(self._h, ctypes.byref(g)), self._h)
+        return GvomGraph(self, g)
+
     # -- gvom.py:222-393 -------------------------------------------------------------
     def combine_maps(self):
         """Combines all maps in the buffer and processes them into 2D maps."""
@@ -247,3 +293,33 @@ class Gvom:
 
     def sync(self):
         nat.check(self._lib.gvom_sync(self._h), self._h)
+
+
+class GvomGraph:
+    """A captured launch sequence; launch() replays it with new ego positions and
+    transforms for the captured scans (capture order)."""
+
+    def __init__(self, owner: Gvom, handle):
+        self.owner = owner
+        self._g = handle
+        self.scans = owner._lib.gvom_graph_scans(handle)
+
+    def launch(self, egos, transforms=None):
+        ego = np.ascontiguousarray(np.asarray(egos, dtype=np.float64).reshape(-1, 3))
+        if ego.shape[0] != self.scans:
+            raise ValueError(f"graph holds {self.scans} scans, got {ego.shape[0]} ego positions")
+        T, tdt = None, nat.GVOM_NO_TRANSFORM
+        if transforms is not None:
+            T = np.asarray(transforms)
+            if T.dtype not in (np.float32, np.float64):
+                T = T.astype(np.float64)
+            T = np.ascontiguousarray(T).reshape(self.scans, 4, 4)
+            tdt = nat.GVOM_F32 if T.dtype == np.float32 else nat.GVOM_F64
+        nat.check(self.owner._lib.gvom_graph_launch(self.owner._h, self._g, nat.ptr(ego), nat.ptr(T), tdt),
+                  self.owner._h)
+
+    def __del__(self):
+        g = getattr(self, "_g", None)
+        if g is not None and g.value:
+            self.owner._lib.gvom_graph_destroy(g)
+            self._g = None

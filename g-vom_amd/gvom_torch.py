@@ -1,0 +1,110 @@
+"""Autograd boundary of the differentiable scan->map loss (torch is used only here).
+
+`MapLoss` owns a dedicated library handle (ring of one slot) and exposes the
+hand-written HIP forward/backward of gvom_diff_forward / gvom_diff_backward:
+
+    L = sum over occupied voxels of lambda_min(neighbourhood covariance)
+      + w_height * variance over columns of the lowest point's height,
+
+with voxel assignment, counts and argmin points fixed by the forward pass.
+`map_loss(points, xi, T0, ego, loss)` is a torch.autograd.Function over float64 CUDA
+points (sensor frame) and a 6-vector xi = (rho, phi) that perturbs the transform as
+T = exp(xi^) T0; its xi-gradient is the left-perturbation gradient at T, exact at xi = 0
+(refinement loops re-anchor T0 every step).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+import gvom
+import gvom_native as nat
+
+
+def hat(w):
+    return np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+
+
+def se3_exp(xi):
+    """SE(3) exponential of xi = (rho, phi) -> 4x4 (float64)."""
+    xi = np.asarray(xi, np.float64)
+    rho, phi = xi[:3], xi[3:]
+    th = float(np.linalg.norm(phi))
+    K = hat(phi)
+    if th < 1e-12:
+        R = np.eye(3) + K
+        V = np.eye(3) + 0.5 * K
+    else:
+        A = np.sin(th) / th
+        B = (1 - np.cos(th)) / th ** 2
+        Cc = (th - np.sin(th)) / th ** 3
+        R = np.eye(3) + A * K + B * K @ K
+        V = np.eye(3) + B * K + Cc * K @ K
+    T = np.eye(4)
+    T[:3, :3] = R
+    T[:3, 3] = V @ rho
+    return T
+
+
+class MapLoss:
+    """Differentiable map loss on single scans, backed by the HIP kernels."""
+
+    def __init__(self, xy_resolution, z_resolution, xy_size, z_size, min_distance=1.0, xy_eigen_dist=1,
+                 z_eigen_dist=1, w_height=0.1):
+        self.g = gvom.Gvom(xy_resolution, z_resolution, xy_size, z_size, 1, min_distance, 0.5, 0.5, 0.3, 2.0, 4.0,
+                           1.0, xy_eigen_dist, z_eigen_dist)
+        self.w_height = float(w_height)
+        self.n = 0
+
+    def forward(self, points_ptr, n, T, ego):
+        """points_ptr: device pointer to n x 3 float64 sensor-frame points."""
+        T = np.ascontiguousarray(np.asarray(T, np.float64).reshape(4, 4))
+        ego = np.ascontiguousarray(np.asarray(ego, np.float64).reshape(3))
+        out = np.zeros(3)
+        nat.check(self.g._lib.gvom_diff_forward(self.g._h, ctypes.c_void_p(int(points_ptr)), int(n), nat.ptr(T),
+                                                nat.ptr(ego), self.w_height, nat.ptr(out)), self.g._h)
+        self.n = int(n)
+        return out
+
+    def backward(self, dL=1.0, grad_points_ptr=None, grad_world_ptr=None):
+        gxi = np.zeros(6)
+        gp = None if grad_points_ptr is None else ctypes.c_void_p(int(grad_points_ptr))
+        gw = None if grad_world_ptr is None else ctypes.c_void_p(int(grad_world_ptr))
+        nat.check(self.g._lib.gvom_diff_backward(self.g._h, float(dL), gp, gw, nat.ptr(gxi)), self.g._h)
+        return gxi
+
+
+def _function():
+    import torch
+
+    class _MapLossFn(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, points, xi, T0, ego, loss):
+            if points.dtype != torch.float64 or not points.is_cuda or not points.is_contiguous():
+                raise ValueError("points must be a contiguous float64 CUDA tensor (N, 3)")
+            T = se3_exp(xi.detach().cpu().numpy()) @ np.asarray(T0, np.float64)
+            torch.cuda.synchronize(points.device)   # the library runs on its own stream
+            out = loss.forward(points.data_ptr(), points.shape[0], T, ego)
+            ctx.loss = loss
+            ctx.save_for_backward(points)
+            return torch.tensor(out[0], dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gL):
+            (points,) = ctx.saved_tensors
+            gp = torch.empty_like(points) if ctx.needs_input_grad[0] else None
+            gxi = ctx.loss.backward(float(gL), None if gp is None else gp.data_ptr())
+            return gp, torch.from_numpy(gxi), None, None, None
+
+    return _MapLossFn
+
+
+_FN = None
+
+
+def map_loss(points, xi, T0, ego, loss: MapLoss):
+    global _FN
+    if _FN is None:
+        _FN = _function()
+    return _FN.apply(points, xi, T0, ego, loss)

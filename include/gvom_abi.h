@@ -57,6 +57,7 @@ typedef struct gvom_params {
 } gvom_params;
 
 typedef struct gvom_handle gvom_handle;
+typedef struct gvom_graph gvom_graph;
 
 /* Replaces Gvom.__init__ (gvom.py:21-101).  Allocates the ring of buffer_size slots,
  * two combined maps and the 2D maps on the current HIP device. */
@@ -170,6 +171,51 @@ int gvom_stage_times(gvom_handle *h, float *ms, int32_t n);
 /* The handle's HIP stream (as void*), for callers that order their own work
  * against it. */
 void *gvom_stream(gvom_handle *h);
+
+/* hipGraph capture of a launch sequence (no reference counterpart).
+ * Between gvom_graph_begin and gvom_graph_end, gvom_process_pointcloud_device,
+ * gvom_process_batch(on_device=1) and gvom_combine_maps_device calls are recorded
+ * instead of run.  Point buffers and point counts are fixed at capture; the ego
+ * positions and transforms of all captured scans (capture order) are passed to every
+ * gvom_graph_launch.  Buffers must be sized beforehand (gvom_reserve); calls that copy
+ * from or to the host are refused while capturing.  max_scans bounds the scans one
+ * sequence may hold. */
+int gvom_graph_begin(gvom_handle *h, int32_t max_scans);
+int gvom_graph_end(gvom_handle *h, gvom_graph **out);
+/* Number of scans recorded in a captured sequence. */
+int gvom_graph_scans(const gvom_graph *g);
+/* Replays the sequence on the handle's stream with new ego positions (scans x 3 doubles)
+ * and transforms (scans x 16 values of t_dtype, or NULL).  Does not synchronise. */
+int gvom_graph_launch(gvom_handle *h, gvom_graph *g, const double *ego, const void *T, int32_t t_dtype);
+void gvom_graph_destroy(gvom_graph *g);
+
+/* Autotuner (no reference counterpart): times the ray-cast kernel on one real scan
+ * (device points, as gvom_process_pointcloud_device) for block sizes {64, 128, 256},
+ * {1, 2, 4} rays per thread and the LDS near-sensor box off/on, without touching the
+ * ring, and keeps the fastest.  chosen = {block, rays per thread, box}; best_ms = its
+ * time.  Synchronises. */
+int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int32_t dtype, const double ego[3],
+                  const void *T, int32_t t_dtype, int32_t chosen[3], float *best_ms);
+/* Sets the ray-cast launch configuration directly. */
+int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t rays_per_thread, int32_t use_box);
+
+/* Differentiable form (no reference counterpart).  Processes one scan like
+ * gvom_process_pointcloud into the next ring slot (points: device array of n x 3
+ * float64 in the sensor frame; T: host 4x4 float64 sensor-to-world; ego: host 3
+ * doubles) and evaluates
+ *   L = sum over occupied voxels of the smallest eigenvalue of the voxel's
+ *       neighbourhood covariance (the per-scan metrics of gvom.py:1250-1387)
+ *     + w_height * variance over columns of the height of their lowest point
+ *       (the height map's lowest voxel, gvom.py:572-576),
+ * with voxel assignment and counts as the forward found them.  out = {L, L_rough,
+ * w_height * Var}.  Synchronises. */
+int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, const double T[16], const double ego[3],
+                      double w_height, double out[3]);
+
+/* Gradient of dL * L from the last gvom_diff_forward: grad_points (device, n x 3 f64,
+ * sensor frame) and grad_world (device, n x 3 f64, world frame) may be NULL;
+ * grad_xi (host, 6 doubles) receives dL/d(rho, phi) for T' = exp(xi^) T.  Synchronises. */
+int gvom_diff_backward(gvom_handle *h, double dL, double *grad_points, double *grad_world, double grad_xi[6]);
 
 #ifdef __cplusplus
 }
