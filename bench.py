@@ -201,10 +201,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no graph, batched or diff legs)")
+    ap.add_argument("--ray-config", default="auto",
+                    help="'auto' (autotune on the first scan) or 'block,rays_per_thread,flags' "
+                         "(flags: bit 0 LDS near-sensor box, bit 1 azimuth-sector ray order)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this path runs on one GPU only; multi-GPU runs are not supported "
-              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print("bench.py: this path runs on one GPU only; runs on more than one GPU are not supported",
+              file=sys.stderr)
         sys.exit(2)
 
     import gvom
@@ -219,6 +222,13 @@ def main():
     g = gvom.Gvom(*wl.map_args)
     g.reserve(max(s.points.shape[0] for s in scans))
     dev = [nat.DeviceArray(s.points) for s in scans]
+    if args.ray_config == "auto":
+        rc = g.autotune(dev[0], scans[0].points.shape[0], scans[0].ego, scans[0].transform)
+        ray_cfg = {"block": rc[0], "rays_per_thread": rc[1], "flags": rc[2], "autotuned_ms": round(rc[3], 4)}
+    else:
+        b, r, f = (int(v) for v in args.ray_config.split(","))
+        g.set_raycast_config(b, r, bool(f & 1), bool(f & 2))
+        ray_cfg = {"block": b, "rays_per_thread": r, "flags": f}
 
     def frame(k):
         s = scans[k % len(scans)]
@@ -277,6 +287,7 @@ def main():
         "plain_eager": {"frames_per_s": round(eager_fps, 2), "ms_per_frame": round(1e3 / eager_fps, 4)},
         "stages_ms": {k: round(v, 5) for k, v in zip(STAGES, stages)},
         "scan": {"cells": ss["cells"], "ray_steps": ss["steps"], "combined_cells": C},
+        "raycast_config": ray_cfg,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes": int(b_dom)},

@@ -20,6 +20,11 @@ hipError_t launch_diff_forward(const Geo &g, const DevState *st, int slot, const
                                const SlotRec *recs, int64_t ncap, const int32_t *seg_start, const uint32_t *svals,
                                const double *world, int64_t n, void *dr, double *colh, int32_t *colpt,
                                int32_t *pmark, double *part, double *sums, double wh, hipStream_t s);
+hipError_t launch_grid_payloads(const Geo &g, const int32_t *pos, const int32_t *neg, const double *rough,
+                                const int32_t *vis, double density, double min_r, double max_r, int8_t *out,
+                                hipStream_t s);
+hipError_t launch_cloud_gather(const uint8_t *data, int64_t n, int64_t width, int64_t point_step, int64_t row_step,
+                               const int32_t off[3], const int32_t type[3], int big, double *xyz, hipStream_t s);
 hipError_t launch_diff_backward(const Geo &g, const DevState *st, int slot, const ScanDesc *sd,
                                 const double *world, const uint8_t *valid, const int32_t *table, const void *dr,
                                 const int32_t *pmark, const double *colh, const double *sums, double wh, double dL,
@@ -97,7 +102,12 @@ struct gvom_handle {
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> set_used;   // bit k set when event k of that set was recorded
     int cur_set = -1;
-    int ray_block = 256, ray_rpt = 1, ray_box = 1;
+    int ray_block = 256, ray_rpt = 1, ray_box = 0, ray_xcd = 0;
+    uint8_t *rsec = nullptr;
+    uint32_t *rorder = nullptr;
+    int8_t *grids = nullptr;   // OccupancyGrid payloads staged for host copies (5 * X bytes)
+    uint8_t *raw = nullptr;    // PointCloud2 bytes staged from the host
+    int64_t rawcap = 0;
 };
 
 #define CKH(h, expr)                                                                 \
@@ -114,7 +124,7 @@ static void free_all(gvom_handle *h) {
                     h->slope_x, h->slope_y, h->rough, h->delta, h->pos, h->neg, h->vis, h->occ_list,
                     h->col_count, h->chunk_tot, h->export_buf, h->pts_in, h->world, h->valid, h->k0, h->v0, h->k1, h->v1,
                     h->hist, h->blk, h->seg_key, h->seg_start, h->mom, h->part, h->sd, h->diff_rec,
-                    h->diff_colh, h->diff_part, h->diff_sums, h->diff_colpt, h->diff_pmark, h->graph_sd};
+                    h->diff_colh, h->diff_part, h->diff_sums, h->diff_colpt, h->diff_pmark, h->graph_sd, h->rsec, h->rorder, h->grids, h->raw};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : h->ev_pool)
@@ -202,6 +212,7 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     CKC(dalloc(&h->rough, X));
     CKC(dalloc(&h->delta, X));
     CKC(dalloc(&h->pos, X));
+    CKC(dalloc(&h->grids, 5 * X));
     CKC(dalloc(&h->neg, X));
     CKC(dalloc(&h->vis, X));
     CKC(dalloc(&h->occ_list, V));
@@ -269,7 +280,7 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
         }
         CKH(h, hipStreamSynchronize(h->stream));
         void *ptrs[] = {h->pts_in, h->world, h->valid, h->k0, h->v0, h->k1, h->v1, h->hist, h->blk,
-                        h->seg_key, h->seg_start, h->mom, h->part};
+                        h->seg_key, h->seg_start, h->mom, h->part, h->rsec, h->rorder};
         for (void *q : ptrs)
             if (q) CKH(h, hipFree(q));
         CKH(h, hipMalloc(&h->pts_in, (size_t)pc * 3 * sizeof(double)));
@@ -285,6 +296,8 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
         CKH(h, dalloc(&h->seg_start, pc + 1));
         CKH(h, dalloc(&h->mom, pc));
         CKH(h, dalloc(&h->part, pc));
+        CKH(h, dalloc(&h->rsec, pc));
+        CKH(h, dalloc(&h->rorder, pc));
         h->ptcap = pc;
     }
     return GVOM_OK;
@@ -380,7 +393,7 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     int rc = ensure_capacity(h, max_scan, NT);
     if (rc) return rc;
     if (!h->slot_recs || !h->pts_in || !h->world || !h->valid || !h->k0 || !h->v0 || !h->k1 || !h->v1 ||
-        !h->hist || !h->blk || !h->seg_key || !h->seg_start || !h->mom || !h->part || h->ptcap < NT || h->ptcap < 1) {
+        !h->hist || !h->blk || !h->seg_key || !h->seg_start || !h->mom || !h->part || !h->rsec || !h->rorder || h->ptcap < NT || h->ptcap < 1) {
         h->err = "internal: work buffers not allocated";
         return GVOM_ESTATE;
     }
@@ -430,10 +443,12 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     }
     stage_event(h, 0);
     CKH(h, launch_clear_slots(h->slot_tables, h->st, S, g, st));
-    CKH(h, launch_prepare(dtype, pts, h->world, h->k0, h->v0, h->valid, sd, S, NT, g, h->st, st));
+    CKH(h, launch_prepare(dtype, pts, h->world, h->k0, h->v0, h->valid, sd, S, NT, g, h->st,
+                          h->ray_xcd ? h->rsec : nullptr, st));
+    if (h->ray_xcd) CKH(h, launch_ray_order(h->rsec, NT, h->st, h->rorder, st));
     stage_event(h, 1);
     CKH(h, launch_raycast(dtype, h->world, h->valid, sd, S, NT, g, h->st, h->slot_tables, h->ray_block,
-                          h->ray_rpt, h->ray_box, -1, st));
+                          h->ray_rpt, h->ray_box, -1, h->ray_xcd ? h->rorder : nullptr, st));
     stage_event(h, 2);
     uint32_t *sk = h->k0, *sv = h->v0;
     CKH(h, radix_sort(h->k0, h->v0, h->k1, h->v1, NT, g.VB + sbits, h->hist, &sk, &sv, st));
@@ -481,6 +496,64 @@ extern "C" int gvom_process_batch(gvom_handle *h, const void *xyz, const int64_t
     if (!h || !offsets || !ego) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
     return process_impl(h, xyz, offsets, n_scans, dtype, ego, T, t_dtype, on_device != 0);
+}
+
+// PointCloud2 input (gvom_ros.py:108): gather x, y, z into float64 points, then the
+// scan stage as for a float64 array.
+extern "C" int gvom_process_pointcloud2(gvom_handle *h, const void *data, const gvom_cloud_layout *L,
+                                        const double ego[3], const void *T, int32_t t_dtype, int32_t on_device) {
+    if (!h || !L || !ego) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    static const int fsize[9] = {0, 1, 1, 2, 2, 4, 4, 4, 8};
+    if (L->height < 0 || L->width < 0 || L->point_step <= 0 || L->row_step < 0) {
+        h->err = "PointCloud2 layout: negative size or non-positive point_step";
+        return GVOM_EINVAL;
+    }
+    const int64_t n = L->height * L->width;
+    if (L->width > 0 && L->row_step < L->width * L->point_step) {
+        h->err = "PointCloud2 layout: row_step < width * point_step";
+        return GVOM_EINVAL;
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (L->datatype[k] < 1 || L->datatype[k] > 8) {
+            h->err = "PointCloud2 layout: x, y, z datatypes must be PointField codes 1..8";
+            return GVOM_EINVAL;
+        }
+        if (L->offset[k] < 0 || L->offset[k] + fsize[L->datatype[k]] > L->point_step) {
+            h->err = "PointCloud2 layout: field outside the point";
+            return GVOM_EINVAL;
+        }
+    }
+    if (n > 0 && !data) {
+        h->err = "null PointCloud2 data";
+        return GVOM_EINVAL;
+    }
+    const int64_t bytes = n > 0 ? (L->height - 1) * L->row_step + L->width * L->point_step : 0;
+    int rc = ensure_capacity(h, n, n);
+    if (rc) return rc;
+    hipStream_t st = h->stream;
+    const uint8_t *src = (const uint8_t *)data;
+    if (!on_device && n > 0) {
+        if (h->capturing) {
+            h->err = "only device-pointer process calls can be captured";
+            return GVOM_ESTATE;
+        }
+        if (bytes > h->rawcap) {
+            CKH(h, hipStreamSynchronize(st));
+            if (h->raw) CKH(h, hipFree(h->raw));
+            h->raw = nullptr;
+            h->rawcap = 0;
+            CKH(h, dalloc(&h->raw, bytes));
+            h->rawcap = bytes;
+        }
+        CKH(h, hipMemcpyAsync(h->raw, data, (size_t)bytes, hipMemcpyHostToDevice, st));
+        src = h->raw;
+    }
+    double *xyz = (double *)h->pts_in;
+    CKH(h, launch_cloud_gather(src, n, L->width, L->point_step, L->row_step, L->offset, L->datatype,
+                               L->is_bigendian ? 1 : 0, xyz, st));
+    int64_t off[2] = {0, n};
+    return process_impl(h, xyz, off, 1, GVOM_F64, ego, T, t_dtype, true);
 }
 
 // The combine stage (gvom.py:222-393).
@@ -539,6 +612,31 @@ extern "C" int gvom_combine_maps_device(gvom_handle *h, double origin_m[3]) {
     std::lock_guard<std::mutex> lk(h->mu);
     int rc = combine_impl(h);
     if (rc) return rc;
+    if (origin_m) origin_metres(h, origin_m);
+    return GVOM_OK;
+}
+
+extern "C" int gvom_occupancy_grids(gvom_handle *h, double density_threshold, double min_roughness,
+                                    double max_roughness, int8_t *out, int32_t on_device, double origin_m[3]) {
+    if (!h || !out) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    bool has = h->have_last != 0;
+    if (h->capturing) {
+        if (!on_device) {
+            h->err = "occupancy grids to host memory cannot be captured; pass a device buffer";
+            return GVOM_ESTATE;
+        }
+        for (int op : h->cap_ops) has = has || op == -1;
+    }
+    if (!has) return GVOM_NO_DATA;
+    const size_t X = (size_t)h->g.X;
+    int8_t *dst = on_device ? out : h->grids;
+    CKH(h, launch_grid_payloads(h->g, h->pos, h->neg, h->rough, h->vis, density_threshold, min_roughness,
+                                max_roughness, dst, h->stream));
+    if (!on_device) {
+        CKH(h, hipMemcpyAsync(out, h->grids, 5 * X, hipMemcpyDeviceToHost, h->stream));
+        CKH(h, hipStreamSynchronize(h->stream));
+    }
     if (origin_m) origin_metres(h, origin_m);
     return GVOM_OK;
 }
@@ -1000,7 +1098,7 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
     int32_t *scratch = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int out = GVOM_OK;
-    int bb = h->ray_block, br = h->ray_rpt, bx = h->ray_box;
+    int bb = h->ray_block, br = h->ray_rpt, bx = h->ray_box, bo = h->ray_xcd;
     float best = 1e30f;
     do {
         if (dalloc(&saved, 1) != hipSuccess || dalloc(&scratch, g.V) != hipSuccess ||
@@ -1011,7 +1109,10 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
         }
         if (hipMemcpyAsync(saved, h->st, sizeof(DevState), hipMemcpyDeviceToDevice, st) != hipSuccess ||
             hipMemcpyAsync(h->sd, &d, sizeof(ScanDesc), hipMemcpyHostToDevice, st) != hipSuccess ||
-            launch_prepare(dtype, points, h->world, h->k0, h->v0, h->valid, h->sd, 1, n, g, h->st, st) != hipSuccess) {
+            launch_fill_i32(&h->st->sec_count[0], 0, 16, st) != hipSuccess ||
+            launch_prepare(dtype, points, h->world, h->k0, h->v0, h->valid, h->sd, 1, n, g, h->st, h->rsec, st) !=
+                hipSuccess ||
+            launch_ray_order(h->rsec, n, h->st, h->rorder, st) != hipSuccess) {
             h->err = "autotune: setup failed";
             out = GVOM_EHIP;
             break;
@@ -1019,13 +1120,14 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
         const int blocks[3] = {64, 128, 256}, rpts[3] = {1, 2, 4};
         for (int ib = 0; ib < 3 && out == GVOM_OK; ++ib)
             for (int ir = 0; ir < 3 && out == GVOM_OK; ++ir)
-                for (int ibox = 0; ibox < 2 && out == GVOM_OK; ++ibox) {
+                for (int icfg = 0; icfg < 4 && out == GVOM_OK; ++icfg) {
+                    const int ibox = icfg & 1, iord = icfg >> 1;
                     float t_min = 1e30f;
                     for (int rep = 0; rep < 3; ++rep) {
                         float t = 0.f;
                         if (launch_fill_i32(scratch, 0, g.V, st) != hipSuccess || hipEventRecord(e0, st) != hipSuccess ||
                             launch_raycast(dtype, h->world, h->valid, h->sd, 1, n, g, h->st, scratch, blocks[ib],
-                                           rpts[ir], ibox, 0, st) != hipSuccess ||
+                                           rpts[ir], ibox, 0, iord ? h->rorder : nullptr, st) != hipSuccess ||
                             hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                             hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
                             h->err = "autotune: timing run failed";
@@ -1039,6 +1141,7 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
                         bb = blocks[ib];
                         br = rpts[ir];
                         bx = ibox;
+                        bo = iord;
                     }
                 }
         if (hipMemcpyAsync(h->st, saved, sizeof(DevState), hipMemcpyDeviceToDevice, st) != hipSuccess ||
@@ -1055,22 +1158,24 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
         h->ray_block = bb;
         h->ray_rpt = br;
         h->ray_box = bx;
+        h->ray_xcd = bo;
         if (chosen) {
             chosen[0] = bb;
             chosen[1] = br;
-            chosen[2] = bx;
+            chosen[2] = bx | (bo << 1);
         }
         if (best_ms) *best_ms = best;
     }
     return out;
 }
 
-extern "C" int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t rays_per_thread, int32_t use_box) {
+extern "C" int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t rays_per_thread, int32_t flags) {
     if (!h || (block != 64 && block != 128 && block != 256) || rays_per_thread < 1 || rays_per_thread > 16)
         return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
     h->ray_block = block;
     h->ray_rpt = rays_per_thread;
-    h->ray_box = use_box ? 1 : 0;
+    h->ray_box = flags & 1;
+    h->ray_xcd = (flags >> 1) & 1;
     return GVOM_OK;
 }

@@ -78,10 +78,36 @@ __device__ void min_eig(double xx, double xy, double xz, double yy, double yz, d
     double nn = n01;
     if (n02 > nn) { c = c02; nn = n02; }
     if (n12 > nn) { c = c12; nn = n12; }
-    if (nn > 0.0) {
+    // largest row of A - lambda I
+    const double q0 = r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2];
+    const double q1 = r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2];
+    const double q2 = r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2];
+    const double *rm = r0;
+    double qm = q0;
+    if (q1 > qm) { rm = r1; qm = q1; }
+    if (q2 > qm) { rm = r2; qm = q2; }
+    if (nn > 1e-20 * qm * qm) {
+        // simple root: A - lambda I has rank 2 and the cross product of two rows spans its
+        // null space
         const double inv = 1.0 / sqrt(nn);
         v[0] = c[0] * inv; v[1] = c[1] * inv; v[2] = c[2] * inv;
+    } else if (qm > 0.0) {
+        // double root (lambda_min = lambda_mid, e.g. collinear neighbourhood points): the
+        // rows span one direction u and every unit vector orthogonal to u is an eigenvector
+        // of lambda_min.  With collinear points every (p - mean) is parallel to u, so any
+        // such choice gives lambda_min the zero gradient it has along its smooth branch.
+        const double iu = 1.0 / sqrt(qm);
+        const double u0 = rm[0] * iu, u1 = rm[1] * iu, u2 = rm[2] * iu;
+        // cross u with the axis it is least aligned with
+        const double a0 = fabs(u0), a1 = fabs(u1), a2 = fabs(u2);
+        double w0, w1, w2;
+        if (a0 <= a1 && a0 <= a2) { w0 = 0.0; w1 = u2; w2 = -u1; }          // u x e0
+        else if (a1 <= a2) { w0 = -u2; w1 = 0.0; w2 = u0; }                 // u x e1
+        else { w0 = u1; w1 = -u0; w2 = 0.0; }                               // u x e2
+        const double iw = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+        v[0] = w0 * iw; v[1] = w1 * iw; v[2] = w2 * iw;
     } else {
+        // A = lambda I: every direction is an eigenvector
         v[0] = 0.0; v[1] = 0.0; v[2] = 1.0;
     }
 }

@@ -56,6 +56,8 @@ struct DevState {
     double slot_origin[GVOM_MAX_BUFFER][3];
     double comb_origin[2][3];
     double latest_ego[3];
+    int32_t sec_count[8];    // valid rays per azimuth sector (XCD-aware ray order)
+    int32_t sec_cursor[8];
 };
 
 // One occupied voxel of a scan slot (reference: hit_count, total_count, metrics f64[10],

@@ -84,12 +84,22 @@ __device__ __forceinline__ bool point_valid(P x, P y, P z, const Geo &g) {
 }
 
 // ------------------------------------------------------------------------------------
+// Azimuth sector (8 x 45 deg) of a ray's xy direction; used only to schedule rays.
+__device__ __forceinline__ int ray_sector(double dx, double dy) {
+    const double a = atan2(dy, dx) + 3.141592653589793;   // [0, 2 pi]
+    int s = (int)(a * (8.0 / 6.283185307179586));
+    return s < 0 ? 0 : (s > 7 ? 7 : s);
+}
+
 template <typename P>
 __global__ __launch_bounds__(256) void k_prepare(const P *__restrict__ in, P *__restrict__ world,
                                                  uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                  uint8_t *__restrict__ valid,
                                                  const ScanDesc *__restrict__ sd, int S, int64_t NT,
-                                                 Geo g, DevState *__restrict__ st) {
+                                                 Geo g, DevState *__restrict__ st, uint8_t *__restrict__ rsec) {
+    __shared__ int32_t scount[8];
+    if (rsec && threadIdx.x < 8) scount[threadIdx.x] = 0;
+    if (rsec) __syncthreads();
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         // Slot origins for combine (gvom.py:208) and the ego the height map uses (gvom.py:110-112).
@@ -106,7 +116,7 @@ __global__ __launch_bounds__(256) void k_prepare(const P *__restrict__ in, P *__
         st->latest_ego[1] = sd[S - 1].ego[1];
         st->latest_ego[2] = sd[S - 1].ego[2];
     }
-    if (i >= NT) return;
+    if (i < NT) {
     int s = scan_of(sd, S, i);
     const ScanDesc &d = sd[s];
     P x = in[3 * i], y = in[3 * i + 1], z = in[3 * i + 2];
@@ -135,6 +145,55 @@ __global__ __launch_bounds__(256) void k_prepare(const P *__restrict__ in, P *__
     }
     keys[i] = key;
     vals[i] = (uint32_t)i;
+    if (rsec) {
+        int sec = 255;
+        if (ok) {
+            sec = ray_sector((double)x - d.ego[0], (double)y - d.ego[1]);
+            atomicAdd(&scount[sec], 1);
+        }
+        rsec[i] = (uint8_t)sec;
+    }
+    }
+    if (rsec) {
+        __syncthreads();
+        if (threadIdx.x < 8 && scount[threadIdx.x]) atomicAdd(&st->sec_count[threadIdx.x], scount[threadIdx.x]);
+    }
+}
+
+// Groups valid rays by azimuth sector: each block appends its rays of sector s, in point
+// order, at a position reserved with one atomic per (block, sector).
+__global__ __launch_bounds__(256) void k_ray_order(const uint8_t *__restrict__ rsec, int64_t NT,
+                                                   DevState *__restrict__ st, uint32_t *__restrict__ order) {
+    __shared__ int32_t cnt[8], base[8];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t < 8) cnt[t] = 0;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + t;
+    const int sec = i < NT ? rsec[i] : 255;
+    // rank within the block among rays of the same sector, in point order
+    int rank = 0;
+    for (int k = 0; k < 8; ++k) {
+        const uint64_t m = __ballot(sec == k);
+        if (sec == k) rank = __popcll(m & ((1ull << lane) - 1ull));
+        __shared__ int32_t wc[4][8];
+        if (lane == 0) wc[w][k] = __popcll(m);
+        __syncthreads();
+        if (sec == k)
+            for (int ww = 0; ww < w; ++ww) rank += wc[ww][k];
+        if (t == 0) {
+            int c = 0;
+            for (int ww = 0; ww < 4; ++ww) c += wc[ww][k];
+            cnt[k] = c;
+        }
+        __syncthreads();
+    }
+    if (t < 8) {
+        int32_t start = 0;
+        for (int k = 0; k < t; ++k) start += st->sec_count[k];
+        base[t] = cnt[t] ? start + atomicAdd(&st->sec_cursor[t], cnt[t]) : 0;
+    }
+    __syncthreads();
+    if (sec < 8) order[base[sec] + rank] = (uint32_t)i;
 }
 
 // ------------------------------------------------------------------------------------
@@ -166,8 +225,20 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
                                                  const uint8_t *__restrict__ valid,
                                                  const ScanDesc *__restrict__ sd, int S, int64_t NT,
                                                  Geo g, const DevState *__restrict__ st,
-                                                 int32_t *__restrict__ tables, int maxit, int rpt, int force_slot) {
+                                                 int32_t *__restrict__ tables, int maxit, int rpt, int force_slot,
+                                                 const uint32_t *__restrict__ order) {
     extern __shared__ int32_t box[];
+    // With `order` (XCD-aware schedule) block b traces chunk b / 8 of azimuth sector b % 8;
+    // workgroups are dispatched to the 8 XCDs round-robin, so each XCD traces one sector.
+    int64_t o_beg = 0, o_len = NT;
+    int64_t chunk = blockIdx.x;
+    if (order) {
+        const int sec = blockIdx.x & 7;
+        chunk = blockIdx.x >> 3;
+        for (int k = 0; k < sec; ++k) o_beg += st->sec_count[k];
+        o_len = st->sec_count[sec];
+    }
+#define RAY_INDEX(j) (order ? ((j) < o_len ? (int64_t)order[o_beg + (j)] : NT) : (j))
     const int lane = threadIdx.x & 63;
     const int next = st->next_slot;
     // force_slot >= 0 (autotuning) writes every scan into table force_slot instead of its ring slot
@@ -178,8 +249,9 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
     uint32_t bbase = 0;
     (void)bbase;
     if (USE_BOX) {
-        const int64_t i0 = (int64_t)blockIdx.x * rpt * blockDim.x;
-        bs = scan_of(sd, S, i0 < NT ? i0 : (NT > 0 ? NT - 1 : 0));
+        int64_t i0 = RAY_INDEX(chunk * rpt * blockDim.x);
+        if (i0 >= NT) i0 = NT > 0 ? NT - 1 : 0;
+        bs = scan_of(sd, S, i0);
         double o[3];
         scan_origin(sd[bs], g, o);
         bx0 = (int64_t)floor((double)(float)(sd[bs].ego[0] / g.xy_res) - o[0]) - BOX_XY / 2;
@@ -190,7 +262,7 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
         __syncthreads();
     }
     for (int rep = 0; rep < rpt; ++rep) {
-        int64_t i = ((int64_t)blockIdx.x * rpt + rep) * blockDim.x + threadIdx.x;
+        const int64_t i = RAY_INDEX((chunk * rpt + rep) * blockDim.x + threadIdx.x);
         bool alive = false, inbox_scan = false;
         float p0 = 0.f, p1 = 0.f, p2 = 0.f, inc0 = 0.f, inc1 = 0.f, inc2 = 0.f;
         double length = 0.0, rl1 = 0.0, dlen = 0.0, o0 = 0.0, o1 = 0.0, o2 = 0.0;
@@ -284,13 +356,14 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
         }
     }
 #undef RAY_SLOT
+#undef RAY_INDEX
 }
 
 // ------------------------------------------------------------------------------------
 // Radix sort: 8-bit digits, tiles of 2048 keys (256 threads x 8).
 constexpr int RS_TILE = 2048;
-// Points per moment chunk (see k_mom_part).
-constexpr int MCH = 16;
+// Points per moment chunk (one wavefront, see k_mom_part).
+constexpr int MCH = 64;
 
 __global__ __launch_bounds__(256) void k_rs_hist(const uint32_t *__restrict__ keys, int64_t n, int shift,
                                                  uint32_t *__restrict__ hist) {
@@ -512,9 +585,9 @@ __global__ __launch_bounds__(64) void k_seg_bounds(const uint32_t *__restrict__ 
 
 // ------------------------------------------------------------------------------------
 // Per-segment moments about the segment's own voxel corner, in two deterministic steps:
-//   k_mom_part: one thread per chunk of MCH sorted points accumulates sequentially and
-//               writes a partial Moment at the last position of every run of equal keys
-//               inside its chunk;
+//   k_mom_part: one wavefront per chunk of MCH sorted points (a lane per point) reduces
+//               every run of equal keys inside the chunk and writes its partial Moment at
+//               the run's last position;
 //   k_mom_seg : one thread per segment adds the partials of the chunks it spans, in
 //               chunk order.
 // A segment of L points costs ceil(L / MCH) + 1 partial loads instead of L serial point
@@ -529,17 +602,17 @@ __device__ __forceinline__ void key_voxel(uint32_t key, const Geo &g, int &s, bo
     const uint32_t vk = key & ((1u << g.VB) - 1u);
     halo = (vk >> (g.VB - 1)) & 1u;
     if (!halo) {
-        int64_t l = vk;
-        wz = l % g.Z;
-        int64_t c = l / g.Z;
-        wy = c % g.XY;
-        wx = c / g.XY;
+        const uint32_t l = vk, Z = (uint32_t)g.Z, XY = (uint32_t)g.XY;
+        const uint32_t c = l / Z;
+        wz = (int64_t)(l - c * Z);
+        wx = (int64_t)(c / XY);
+        wy = (int64_t)(c - (c / XY) * XY);
     } else {
-        int64_t l = vk & ((1u << (g.VB - 1)) - 1u);
-        wz = l % g.Zp - g.ez;
-        int64_t c = l / g.Zp;
-        wy = c % g.XYp - g.ex;
-        wx = c / g.XYp - g.ex;
+        const uint32_t l = vk & ((1u << (g.VB - 1)) - 1u), Zp = (uint32_t)g.Zp, XYp = (uint32_t)g.XYp;
+        const uint32_t c = l / Zp;
+        wz = (int64_t)(l - c * Zp) - g.ez;
+        wx = (int64_t)(c / XYp) - g.ex;
+        wy = (int64_t)(c - (c / XYp) * XYp) - g.ex;
     }
 }
 
@@ -566,44 +639,72 @@ __global__ __launch_bounds__(256) void k_mom_part(const P *__restrict__ world, c
                                                   const uint32_t *__restrict__ svals,
                                                   const ScanDesc *__restrict__ sd, int S, Geo g,
                                                   const DevState *__restrict__ st, Moment *__restrict__ part) {
-    // chunks are cut relative to each scan's first sorted position, so a scan's sums do not
-    // depend on the other scans of its batch
-    const int32_t t = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (t >= st->chunk_off[S]) return;
+    // One wavefront per chunk of MCH = 64 sorted points, one lane per point.  Chunks are cut
+    // relative to each scan's first sorted position, so a scan's sums do not depend on the
+    // other scans of its batch.  A segmented inclusive scan (fixed shuffle tree) leaves each
+    // run's sum in its last lane, which stores it at that point's sorted position.
+    const int lane = threadIdx.x & 63;
+    const int32_t t = (int32_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (t >= st->chunk_off[S]) return;   // uniform across the wavefront
     int sc = 0;
     while (sc + 1 < S && st->chunk_off[sc + 1] <= t) ++sc;
     const int64_t sbeg = st->scan_pos[sc], send = sbeg + st->scan_valid[sc];
-    const int64_t j0 = sbeg + (int64_t)(t - st->chunk_off[sc]) * MCH;
-    const int64_t j1 = j0 + MCH < send ? j0 + MCH : send;
-    uint32_t cur = GVOM_NOKEY;
-    MomAcc a;
-    a.zero();
-    double o0 = 0, o1 = 0, o2 = 0;
-    int64_t wx = 0, wy = 0, wz = 0;
-    for (int64_t j = j0; j < j1; ++j) {
-        const uint32_t key = skeys[j];
-        if (key != cur) {
-            if (cur != GVOM_NOKEY) store_moment(part + (j - 1), a);
-            cur = key;
-            a.zero();
-            int s;
-            bool halo;
-            key_voxel(key, g, s, halo, wx, wy, wz);
-            double o[3];
-            scan_origin(sd[s], g, o);
-            o0 = o[0]; o1 = o[1]; o2 = o[2];
-        }
+    const int64_t j = sbeg + (int64_t)(t - st->chunk_off[sc]) * MCH + lane;
+    const bool in = j < send;
+    const uint32_t key = in ? skeys[j] : GVOM_NOKEY;
+    const uint32_t prev = __shfl_up(key, 1);
+    int f = (lane == 0 || key != prev) ? 1 : 0;   // run head
+    double v[9];
+    int32_t cnt = 0;
+    float mh = 1.0f;
+    for (int k = 0; k < 9; ++k) v[k] = 0.0;
+    if (in) {
+        int s;
+        bool halo;
+        int64_t wx, wy, wz;
+        key_voxel(key, g, s, halo, wx, wy, wz);
+        double o[3];
+        scan_origin(sd[s], g, o);
         const int64_t i = svals[j];
-        double r0 = (((double)world[3 * i] / g.xy_res) - o0) - (double)wx;
-        double r1 = (((double)world[3 * i + 1] / g.xy_res) - o1) - (double)wy;
-        double r2 = (((double)world[3 * i + 2] / g.z_res) - o2) - (double)wz;
-        a.S0 += r0; a.S1 += r1; a.S2 += r2;
-        a.Qxx += r0 * r0; a.Qxy += r0 * r1; a.Qxz += r0 * r2;
-        a.Qyy += r1 * r1; a.Qyz += r1 * r2; a.Qzz += r2 * r2;
-        a.n += 1;
-        a.mh = py_minf(a.mh, (float)r2);
+        const double r0 = (((double)world[3 * i] / g.xy_res) - o[0]) - (double)wx;
+        const double r1 = (((double)world[3 * i + 1] / g.xy_res) - o[1]) - (double)wy;
+        const double r2 = (((double)world[3 * i + 2] / g.z_res) - o[2]) - (double)wz;
+        v[0] = r0; v[1] = r1; v[2] = r2;
+        v[3] = r0 * r0; v[4] = r0 * r1; v[5] = r0 * r2;
+        v[6] = r1 * r1; v[7] = r1 * r2; v[8] = r2 * r2;
+        cnt = 1;
+        mh = py_minf(1.0f, (float)r2);
     }
-    store_moment(part + (j1 - 1), a);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        double pv[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) pv[k] = __shfl_up(v[k], off);
+        const int32_t pc = __shfl_up(cnt, off);
+        const float pm = __shfl_up(mh, off);
+        const int pf = __shfl_up(f, off);
+        if (lane >= off) {
+            if (!f) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) v[k] = pv[k] + v[k];
+                cnt += pc;
+                mh = py_minf(pm, mh);
+            }
+            f |= pf;
+        }
+    }
+    // last lane of its run inside the chunk: the next lane starts a run (padding lanes carry
+    // GVOM_NOKEY, so the last valid lane always ends one)
+    const uint32_t next = __shfl_down(key, 1);
+    const bool last = lane == 63 || next != key;
+    if (in && last) {
+        MomAcc a;
+        a.S0 = v[0]; a.S1 = v[1]; a.S2 = v[2];
+        a.Qxx = v[3]; a.Qxy = v[4]; a.Qxz = v[5]; a.Qyy = v[6]; a.Qyz = v[7]; a.Qzz = v[8];
+        a.n = cnt;
+        a.mh = mh;
+        store_moment(part + j, a);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_mom_seg(const int32_t *__restrict__ seg_start,
@@ -689,66 +790,93 @@ __global__ __launch_bounds__(256) void k_metrics(const uint32_t *__restrict__ se
                                                  const DevState *__restrict__ st,
                                                  const int32_t *__restrict__ tables,
                                                  SlotRec *__restrict__ recs, int64_t ncap) {
-    const int64_t gi = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
     const int l32 = threadIdx.x & 31;
     const int32_t nseg = st->nseg;
-    if (gi >= nseg) return;   // uniform across the 32-lane group
-    const uint32_t key = seg_key[gi];
-    const uint32_t vk = key & ((1u << g.VB) - 1u);
-    if ((vk >> (g.VB - 1)) & 1u) return;   // halo segment: no record
-    const int s = (int)(key >> g.VB);
-    const int slot = (st->next_slot + s) % g.B;
-    const int32_t fs = st->first_seg[s], ie = st->inb_end[s];
-    const int32_t *table = tables + (int64_t)slot * g.V;
-    const int64_t uz = vk % g.Z, uc = vk / g.Z, uy = uc % g.XY, ux = uc / g.XY;
+    const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> 5;
     const int nx = 2 * g.ex + 1, nz = 2 * g.ez + 1, nn = nx * nx * nz;
+    // grid-stride over segments; each 32-lane group handles one occupied voxel at a time
+    for (int64_t gi = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 5; gi < nseg; gi += groups) {
+        const uint32_t key = seg_key[gi];
+        const uint32_t vk = key & ((1u << g.VB) - 1u);
+        if ((vk >> (g.VB - 1)) & 1u) continue;   // halo segment: no record (uniform in the group)
+        const int s = (int)(key >> g.VB);
+        const int slot = (st->next_slot + s) % g.B;
+        const int32_t fs = st->first_seg[s], ie = st->inb_end[s];
+        const int32_t *table = tables + (int64_t)slot * g.V;
+        const uint32_t Z = (uint32_t)g.Z, XY = (uint32_t)g.XY;
+        const uint32_t uc = vk / Z;
+        const int64_t uz = vk - uc * Z, ux = uc / XY, uy = uc - (uc / XY) * XY;
 
-    double n = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
-    for (int k = l32; k < nn; k += 32) {
-        const int dz = k % nz - g.ez, dy = (k / nz) % nx - g.ex, dx = k / (nz * nx) - g.ex;
-        int64_t w = neighbour_segment(ux + dx, uy + dy, uz + dz, s, g, table, fs, ie, nseg, seg_key);
-        if (w < 0) continue;
-        const Moment &M = mom[w];
-        double nw = (double)M.n;
-        m0 += M.S[0] + nw * (double)dx;
-        m1 += M.S[1] + nw * (double)dy;
-        m2 += M.S[2] + nw * (double)dz;
-        n += nw;
+        // first neighbour of this lane (nn <= 32 for the default eigen distances): its
+        // segment is looked up once and reused by the second pass
+        int64_t w0 = -1;
+        int dx0 = 0, dy0 = 0, dz0 = 0;
+        if (l32 < nn) {
+            dz0 = l32 % nz - g.ez;
+            dy0 = (l32 / nz) % nx - g.ex;
+            dx0 = l32 / (nz * nx) - g.ex;
+            w0 = neighbour_segment(ux + dx0, uy + dy0, uz + dz0, s, g, table, fs, ie, nseg, seg_key);
+        }
+        double n = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+        for (int k = l32; k < nn; k += 32) {
+            int dx = dx0, dy = dy0, dz = dz0;
+            int64_t w = w0;
+            if (k != l32) {
+                dz = k % nz - g.ez;
+                dy = (k / nz) % nx - g.ex;
+                dx = k / (nz * nx) - g.ex;
+                w = neighbour_segment(ux + dx, uy + dy, uz + dz, s, g, table, fs, ie, nseg, seg_key);
+            }
+            if (w < 0) continue;
+            const Moment &M = mom[w];
+            double nw = (double)M.n;
+            m0 += M.S[0] + nw * (double)dx;
+            m1 += M.S[1] + nw * (double)dy;
+            m2 += M.S[2] + nw * (double)dz;
+            n += nw;
+        }
+        n = red32(n);
+        m0 = red32(m0) / n;
+        m1 = red32(m1) / n;
+        m2 = red32(m2) / n;
+        double cxx = 0, cxy = 0, cxz = 0, cyy = 0, cyz = 0, czz = 0;
+        for (int k = l32; k < nn; k += 32) {
+            int dx = dx0, dy = dy0, dz = dz0;
+            int64_t w = w0;
+            if (k != l32) {
+                dz = k % nz - g.ez;
+                dy = (k / nz) % nx - g.ex;
+                dx = k / (nz * nx) - g.ex;
+                w = neighbour_segment(ux + dx, uy + dy, uz + dz, s, g, table, fs, ie, nseg, seg_key);
+            }
+            if (w < 0) continue;
+            const Moment &M = mom[w];
+            double nw = (double)M.n;
+            double a0 = (double)dx - m0, a1 = (double)dy - m1, a2 = (double)dz - m2;
+            cxx += M.Q[0] + 2.0 * M.S[0] * a0 + nw * a0 * a0;
+            cxy += M.Q[1] + M.S[0] * a1 + a0 * M.S[1] + nw * a0 * a1;
+            cxz += M.Q[2] + M.S[0] * a2 + a0 * M.S[2] + nw * a0 * a2;
+            cyy += M.Q[3] + 2.0 * M.S[1] * a1 + nw * a1 * a1;
+            cyz += M.Q[4] + M.S[1] * a2 + a1 * M.S[2] + nw * a1 * a2;
+            czz += M.Q[5] + 2.0 * M.S[2] * a2 + nw * a2 * a2;
+        }
+        cxx = red32(cxx); cxy = red32(cxy); cxz = red32(cxz);
+        cyy = red32(cyy); cyz = red32(cyz); czz = red32(czz);
+        if (l32 == 0) {
+            SlotRec *r = recs + (int64_t)slot * ncap + (gi - fs);
+            r->m[0] = m0;
+            r->m[1] = m1;
+            r->m[2] = m2;
+            bool pos = n > 0;   // gvom.py:1383-1385
+            r->m[3] = pos ? cxx / n : 0.0;
+            r->m[4] = pos ? cxy / n : 0.0;
+            r->m[5] = pos ? cxz / n : 0.0;
+            r->m[6] = pos ? cyy / n : 0.0;
+            r->m[7] = pos ? cyz / n : 0.0;
+            r->m[8] = pos ? czz / n : 0.0;
+            r->m[9] = n;
+        }
     }
-    n = red32(n);
-    m0 = red32(m0) / n;
-    m1 = red32(m1) / n;
-    m2 = red32(m2) / n;
-    double cxx = 0, cxy = 0, cxz = 0, cyy = 0, cyz = 0, czz = 0;
-    for (int k = l32; k < nn; k += 32) {
-        const int dz = k % nz - g.ez, dy = (k / nz) % nx - g.ex, dx = k / (nz * nx) - g.ex;
-        int64_t w = neighbour_segment(ux + dx, uy + dy, uz + dz, s, g, table, fs, ie, nseg, seg_key);
-        if (w < 0) continue;
-        const Moment &M = mom[w];
-        double nw = (double)M.n;
-        double a0 = (double)dx - m0, a1 = (double)dy - m1, a2 = (double)dz - m2;
-        cxx += M.Q[0] + 2.0 * M.S[0] * a0 + nw * a0 * a0;
-        cxy += M.Q[1] + M.S[0] * a1 + a0 * M.S[1] + nw * a0 * a1;
-        cxz += M.Q[2] + M.S[0] * a2 + a0 * M.S[2] + nw * a0 * a2;
-        cyy += M.Q[3] + 2.0 * M.S[1] * a1 + nw * a1 * a1;
-        cyz += M.Q[4] + M.S[1] * a2 + a1 * M.S[2] + nw * a1 * a2;
-        czz += M.Q[5] + 2.0 * M.S[2] * a2 + nw * a2 * a2;
-    }
-    cxx = red32(cxx); cxy = red32(cxy); cxz = red32(cxz);
-    cyy = red32(cyy); cyz = red32(cyz); czz = red32(czz);
-    if (l32 != 0) return;
-    SlotRec *r = recs + (int64_t)slot * ncap + (gi - fs);
-    r->m[0] = m0;
-    r->m[1] = m1;
-    r->m[2] = m2;
-    bool pos = n > 0;   // gvom.py:1383-1385
-    r->m[3] = pos ? cxx / n : 0.0;
-    r->m[4] = pos ? cxy / n : 0.0;
-    r->m[5] = pos ? cxz / n : 0.0;
-    r->m[6] = pos ? cyy / n : 0.0;
-    r->m[7] = pos ? cyz / n : 0.0;
-    r->m[8] = pos ? czz / n : 0.0;
-    r->m[9] = n;
 }
 
 __global__ void k_commit_process(DevState *st, int S, int B) {
@@ -767,6 +895,7 @@ __global__ __launch_bounds__(256) void k_clear_slots(int32_t *__restrict__ table
                                                      int S, int B, int64_t V) {
     const int64_t per = (V + 3) / 4;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 16) (&const_cast<DevState *>(st)->sec_count[0])[i] = 0;   // sec_count and sec_cursor
     if (i >= per * S) return;
     const int s = (int)(i / per);
     const int slot = (st->next_slot + s) % B;
@@ -778,6 +907,12 @@ __global__ __launch_bounds__(256) void k_clear_slots(int32_t *__restrict__ table
         for (int k = 0; k < 4; ++k)
             if (j + k < V) t[j + k] = 0;
     }
+}
+
+// One word per thread, for buffers that are not 16-byte aligned (fields inside DevState).
+__global__ __launch_bounds__(256) void k_fill_i32_words(int32_t *__restrict__ p, int32_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 __global__ __launch_bounds__(256) void k_fill_i32(int32_t *__restrict__ p, int32_t v, int64_t n) {
@@ -796,46 +931,57 @@ inline unsigned nblocks(int64_t n, int per) { return (unsigned)((n + per - 1) / 
 // ---- launchers ----------------------------------------------------------------------
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    bool aligned = ((uintptr_t)p % 16) == 0;
-    if (!aligned) return hipErrorInvalidValue;
-    k_fill_i32<<<nblocks((n + 3) / 4, 256), 256, 0, s>>>(p, v, n);
+    if (((uintptr_t)p % 16) == 0)
+        k_fill_i32<<<nblocks((n + 3) / 4, 256), 256, 0, s>>>(p, v, n);
+    else
+        k_fill_i32_words<<<nblocks(n, 256), 256, 0, s>>>(p, v, n);
     return hipGetLastError();
 }
 
 hipError_t launch_prepare(int dtype, const void *in, void *world, uint32_t *keys, uint32_t *vals,
                           uint8_t *valid, const ScanDesc *sd, int S, int64_t NT, const Geo &g,
-                          DevState *st, hipStream_t s) {
+                          DevState *st, uint8_t *rsec, hipStream_t s) {
     unsigned nb = nblocks(NT, 256);
     if (dtype == GVOM_F32)
-        k_prepare<float><<<nb, 256, 0, s>>>((const float *)in, (float *)world, keys, vals, valid, sd, S, NT, g, st);
+        k_prepare<float><<<nb, 256, 0, s>>>((const float *)in, (float *)world, keys, vals, valid, sd, S, NT, g, st,
+                                             rsec);
     else
-        k_prepare<double><<<nb, 256, 0, s>>>((const double *)in, (double *)world, keys, vals, valid, sd, S, NT, g, st);
+        k_prepare<double><<<nb, 256, 0, s>>>((const double *)in, (double *)world, keys, vals, valid, sd, S, NT, g, st,
+                                              rsec);
+    return hipGetLastError();
+}
+
+hipError_t launch_ray_order(const uint8_t *rsec, int64_t NT, DevState *st, uint32_t *order, hipStream_t s) {
+    if (NT <= 0) return hipSuccess;
+    k_ray_order<<<nblocks(NT, 256), 256, 0, s>>>(rsec, NT, st, order);
     return hipGetLastError();
 }
 
 hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, const ScanDesc *sd,
                           int S, int64_t NT, const Geo &g, const DevState *st, int32_t *slot_tables,
-                          int block, int rpt, int use_box, int force_slot, hipStream_t s) {
+                          int block, int rpt, int use_box, int force_slot, const uint32_t *order,
+                          hipStream_t s) {
     if (NT <= 0) return hipSuccess;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
     if (rpt < 1) rpt = 1;
     int maxit = (g.XY > g.Z ? g.XY : g.Z) + 2;
     unsigned nb = nblocks(NT, block * rpt);
+    if (order) nb *= 8;   // 8 sectors, each sized for every ray in the worst case
     const size_t lds = use_box ? BOX_N * sizeof(int32_t) : 0;
     if (dtype == GVOM_F32) {
         if (use_box)
             k_raycast<float, true><<<nb, block, lds, s>>>((const float *)world, valid, sd, S, NT, g, st, slot_tables,
-                                                          maxit, rpt, force_slot);
+                                                          maxit, rpt, force_slot, order);
         else
             k_raycast<float, false><<<nb, block, 0, s>>>((const float *)world, valid, sd, S, NT, g, st, slot_tables,
-                                                         maxit, rpt, force_slot);
+                                                         maxit, rpt, force_slot, order);
     } else {
         if (use_box)
             k_raycast<double, true><<<nb, block, lds, s>>>((const double *)world, valid, sd, S, NT, g, st,
-                                                           slot_tables, maxit, rpt, force_slot);
+                                                           slot_tables, maxit, rpt, force_slot, order);
         else
             k_raycast<double, false><<<nb, block, 0, s>>>((const double *)world, valid, sd, S, NT, g, st,
-                                                          slot_tables, maxit, rpt, force_slot);
+                                                          slot_tables, maxit, rpt, force_slot, order);
     }
     return hipGetLastError();
 }
@@ -876,7 +1022,7 @@ hipError_t launch_moments(int dtype, const void *world, const uint32_t *skeys, c
                           const int32_t *seg_start, const uint32_t *seg_key, int64_t seg_cap,
                           const ScanDesc *sd, int S, const Geo &g, DevState *st, int32_t *slot_tables,
                           SlotRec *slot_recs, int64_t ncap, Moment *part, Moment *mom, hipStream_t s) {
-    unsigned nbp = nblocks((seg_cap + MCH - 1) / MCH + S, 256);
+    unsigned nbp = nblocks(((seg_cap + MCH - 1) / MCH + S) * 64, 256);
     if (dtype == GVOM_F32)
         k_mom_part<float><<<nbp, 256, 0, s>>>((const float *)world, skeys, svals, sd, S, g, st, part);
     else
@@ -889,7 +1035,10 @@ hipError_t launch_moments(int dtype, const void *world, const uint32_t *skeys, c
 hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t seg_cap, const Geo &g,
                           const DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap,
                           hipStream_t s) {
-    k_metrics<<<nblocks(seg_cap * 32, 256), 256, 0, s>>>(seg_key, mom, g, st, slot_tables, slot_recs, ncap);
+    // grid-stride: enough 32-lane groups to fill the chip, never more than segments can need
+    int64_t nb = nblocks(seg_cap * 32, 256);
+    if (nb > 2048) nb = 2048;
+    k_metrics<<<(unsigned)nb, 256, 0, s>>>(seg_key, mom, g, st, slot_tables, slot_recs, ncap);
     return hipGetLastError();
 }
 

@@ -45,6 +45,24 @@ def _as_transform(transform):
     return T, (nat.GVOM_F32 if T.dtype == np.float32 else nat.GVOM_F64)
 
 
+def cloud_layout(cloud):
+    """gvom_cloud_layout of a PointCloud2-like message: the x, y, z PointFields."""
+    fields = {f.name: f for f in cloud.fields}
+    for name in ("x", "y", "z"):
+        if name not in fields:
+            raise ValueError(f"PointCloud2 has no '{name}' field")
+    lay = nat.CloudLayout()
+    lay.height = int(cloud.height)
+    lay.width = int(cloud.width)
+    lay.point_step = int(cloud.point_step)
+    lay.row_step = int(cloud.row_step)
+    for k, name in enumerate(("x", "y", "z")):
+        lay.offset[k] = int(fields[name].offset)
+        lay.datatype[k] = int(fields[name].datatype)
+    lay.is_bigendian = 1 if cloud.is_bigendian else 0
+    return lay
+
+
 class Gvom:
     """Scan -> voxel map -> 2D cost maps (reference class: scripts/gvom.py:8)."""
 
@@ -107,6 +125,33 @@ class Gvom:
         nat.check(self._lib.gvom_process_pointcloud_device(self._h, ptr, int(n), dt, nat.ptr(ego), nat.ptr(T), tdt),
                   self._h)
 
+    def process_pointcloud2(self, cloud, ego_position, transform=None, device_data=None):
+        """The ROS node's input path (gvom_ros.py:108-109) on a sensor_msgs/PointCloud2
+        (or any object with height, width, point_step, row_step, is_bigendian, fields
+        (name, offset, datatype) and data): x, y, z are read from the message bytes on the
+        device, as float64, and non-finite points are dropped, the same slot that
+        Process_pointcloud(ros_numpy.point_cloud2.pointcloud2_to_xyz_array(cloud), ...)
+        makes.  device_data: the message bytes already in device memory (then
+        cloud.data is not read and the call does not synchronise)."""
+        lay = cloud_layout(cloud)
+        ego = np.ascontiguousarray(np.asarray(ego_position, dtype=np.float64).reshape(3))
+        T, tdt = _as_transform(transform)
+        self.ego_position = tuple(float(v) for v in ego)
+        if device_data is not None:
+            ptr = device_data.ptr if isinstance(device_data, nat.DeviceArray) else ctypes.c_void_p(int(device_data))
+            on_dev = 1
+            keep = None
+        else:
+            keep = np.frombuffer(memoryview(cloud.data).cast("B"), dtype=np.uint8)
+            need = (lay.height - 1) * lay.row_step + lay.width * lay.point_step if lay.height * lay.width else 0
+            if keep.size < need:
+                raise ValueError(f"PointCloud2 data holds {keep.size} bytes, layout needs {need}")
+            ptr = keep.ctypes.data_as(ctypes.c_void_p)
+            on_dev = 0
+        nat.check(self._lib.gvom_process_pointcloud2(self._h, ptr, ctypes.byref(lay), nat.ptr(ego), nat.ptr(T), tdt,
+                                                     on_dev), self._h)
+        del keep
+
     def reserve(self, max_points, max_batch_points=0):
         nat.check(self._lib.gvom_reserve(self._h, int(max_points), int(max_batch_points)), self._h)
 
@@ -151,7 +196,7 @@ class Gvom:
 
     def autotune(self, dev_points, n, ego_position, transform=None, dtype=np.float32):
         """Pick the ray-cast launch configuration by timing it on one device-resident scan.
-        Returns (block, rays_per_thread, use_box, best_ms)."""
+        Returns (block, rays_per_thread, flags (bit 0 LDS box, bit 1 sector order), best_ms)."""
         ptr = dev_points.ptr if isinstance(dev_points, nat.DeviceArray) else ctypes.c_void_p(int(dev_points))
         ego = np.ascontiguousarray(np.asarray(ego_position, dtype=np.float64).reshape(3))
         T, tdt = _as_transform(transform)
@@ -162,9 +207,9 @@ class Gvom:
                                           ctypes.byref(best)), self._h)
         return chosen[0], chosen[1], chosen[2], best.value
 
-    def set_raycast_config(self, block=256, rays_per_thread=1, use_box=False):
-        nat.check(self._lib.gvom_set_raycast_config(self._h, int(block), int(rays_per_thread), 1 if use_box else 0),
-                  self._h)
+    def set_raycast_config(self, block=256, rays_per_thread=1, use_box=False, sector_order=False):
+        flags = (1 if use_box else 0) | (2 if sector_order else 0)
+        nat.check(self._lib.gvom_set_raycast_config(self._h, int(block), int(rays_per_thread), flags), self._h)
 
     # -- hipGraph capture ------------------------------------------------------------
     def graph_begin(self, max_scans=64):
@@ -198,6 +243,33 @@ class Gvom:
         origin = np.zeros(3, np.float64)
         rc = nat.check(self._lib.gvom_combine_maps_device(self._h, nat.ptr(origin)), self._h)
         return None if rc == nat.GVOM_NO_DATA else origin
+
+    # -- gvom_ros.py:137-164 ---------------------------------------------------------
+    def occupancy_grids(self, density_threshold=50, min_roughness=-10, max_roughness=0, device_out=None):
+        """OccupancyGrid payloads of the latest combined maps, as the ROS node publishes
+        them (gvom_ros.py:137-164): a dict topic -> int8 array of xy_size^2 bytes in the
+        message's order (x fastest), plus "origin" (metres; the node publishes x, y and
+        z = 0).  Topics: hard_obstacle_map, soft_obstacle_map, ground_certainty_map (the
+        node also publishes it as all_ground_certainty_map), negative_obstacle_map,
+        roughness_map.  With device_out (a device pointer to 5 * xy_size^2 bytes) the
+        payloads are written there without synchronising and only the origin is
+        returned.  None before the first combine."""
+        X = self.xy_size * self.xy_size
+        origin = np.zeros(3, np.float64)
+        if device_out is not None:
+            ptr = device_out.ptr if isinstance(device_out, nat.DeviceArray) else ctypes.c_void_p(int(device_out))
+            rc = nat.check(self._lib.gvom_occupancy_grids(self._h, float(density_threshold), float(min_roughness),
+                                                          float(max_roughness), ptr, 1, nat.ptr(origin)), self._h)
+            return None if rc == nat.GVOM_NO_DATA else origin
+        out = np.empty(5 * X, np.int8)
+        rc = nat.check(self._lib.gvom_occupancy_grids(self._h, float(density_threshold), float(min_roughness),
+                                                      float(max_roughness), nat.ptr(out), 0, nat.ptr(origin)),
+                       self._h)
+        if rc == nat.GVOM_NO_DATA:
+            return None
+        grids = {name: out[k * X:(k + 1) * X] for k, name in enumerate(nat.GRID_NAMES)}
+        grids["origin"] = origin
+        return grids
 
     # -- gvom.py:395-442 -------------------------------------------------------------
     def make_debug_voxel_map(self):

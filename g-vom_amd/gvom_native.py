@@ -16,6 +16,8 @@ GVOM_NO_DATA = 1
 GVOM_F32 = 0
 GVOM_F64 = 1
 GVOM_NO_TRANSFORM = -1
+GRID_NAMES = ("hard_obstacle_map", "soft_obstacle_map", "ground_certainty_map", "negative_obstacle_map",
+              "roughness_map")
 
 # Every symbol the header declares, with (restype, argtypes).
 _P = ctypes.c_void_p
@@ -33,6 +35,8 @@ SIGNATURES = {
     "gvom_combine_maps_device": (ctypes.c_int, [_P, _P]),
     "gvom_device_maps": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "gvom_sync": (ctypes.c_int, [_P]),
+    "gvom_process_pointcloud2": (ctypes.c_int, [_P, _P, _P, _P, _P, _I32, _I32]),
+    "gvom_occupancy_grids": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P, _I32, _P]),
     "gvom_debug_voxel_map": (ctypes.c_int, [_P, _P, _I64, _P]),
     "gvom_debug_height_map": (ctypes.c_int, [_P, _P]),
     "gvom_debug_inferred_height_map": (ctypes.c_int, [_P, _P]),
@@ -68,6 +72,16 @@ class Params(ctypes.Structure):
         ("robot_height", ctypes.c_double), ("robot_radius", ctypes.c_double),
         ("ground_to_lidar_height", ctypes.c_double),
         ("xy_eigen_dist", ctypes.c_int32), ("z_eigen_dist", ctypes.c_int32),
+    ]
+
+
+class CloudLayout(ctypes.Structure):
+    """gvom_cloud_layout (include/gvom_abi.h): where x, y, z sit in PointCloud2 data."""
+    _fields_ = [
+        ("height", ctypes.c_int64), ("width", ctypes.c_int64),
+        ("point_step", ctypes.c_int64), ("row_step", ctypes.c_int64),
+        ("offset", ctypes.c_int32 * 3), ("datatype", ctypes.c_int32 * 3),
+        ("is_bigendian", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 

@@ -118,6 +118,52 @@ int gvom_combine_maps_device(gvom_handle *h, double origin_m[3]);
 int gvom_device_maps(gvom_handle *h, int32_t **positive_obstacle, int32_t **negative_obstacle,
                      double **roughness, int32_t **visibility);
 
+/* Memory layout of a sensor_msgs/PointCloud2 message's data (the fields the node's
+ * input conversion reads).  Point (r, c) starts at byte r * row_step + c * point_step;
+ * offset / datatype are the x, y, z PointFields (datatype: PointField codes 1 INT8 ..
+ * 8 FLOAT64). */
+typedef struct {
+    int64_t height, width;
+    int64_t point_step, row_step;
+    int32_t offset[3];
+    int32_t datatype[3];
+    int32_t is_bigendian;
+    int32_t reserved;
+} gvom_cloud_layout;
+
+/* Replaces the node's input path (gvom_ros.py:108-109): ros_numpy's
+ * pointcloud2_to_xyz_array (x, y, z as float64, rows with a non-finite coordinate
+ * removed) followed by Process_pointcloud.  The x, y, z fields are gathered into float64
+ * on the device; a point with a non-finite coordinate is dropped by the scan stage,
+ * which leaves the slot identical to processing the compacted float64 array.  data is
+ * the message's byte buffer (host memory, or device memory with on_device != 0: then
+ * no host copy is made and the call may be captured).  ego and T as for
+ * gvom_process_pointcloud.  (ros_numpy assumes row_step == width * point_step; padded
+ * rows are addressed by row_step here.) */
+int gvom_process_pointcloud2(gvom_handle *h, const void *data, const gvom_cloud_layout *layout,
+                             const double ego[3], const void *T, int32_t t_dtype, int32_t on_device);
+
+/* Replaces the payload formatting of the reference's ROS node (gvom_ros.py:137-164):
+ * the int8 OccupancyGrid data it publishes from the latest combined maps, made on the
+ * device.  out receives 5 payloads of xy_size^2 bytes, in the order GVOM_GRID_HARD,
+ * _SOFT, _CERTAINTY, _NEGATIVE, _ROUGHNESS, each in the message's order (x fastest:
+ * byte x + y*xy_size is map cell [x, y]).  Values follow the node's numpy expressions,
+ * including its float64 -> int8 cast as on x86-64 (truncate to int32, keep the low
+ * byte; so the default roughness range lands in [-128, 127] by wrapping).  The node
+ * publishes the ground-certainty payload twice (ground_certainty_map and
+ * all_ground_certainty_map).  on_device != 0: out is a device pointer, the call is
+ * queued on the handle's stream and may be captured; otherwise out is host memory and
+ * the call synchronises.  origin_m (may be NULL) receives the grid origin in metres
+ * (info.origin.position x, y; the node sets z = 0).  Returns GVOM_NO_DATA before the
+ * first combine. */
+#define GVOM_GRID_HARD 0
+#define GVOM_GRID_SOFT 1
+#define GVOM_GRID_CERTAINTY 2
+#define GVOM_GRID_NEGATIVE 3
+#define GVOM_GRID_ROUGHNESS 4
+int gvom_occupancy_grids(gvom_handle *h, double density_threshold, double min_roughness, double max_roughness,
+                         int8_t *out, int32_t on_device, double origin_m[3]);
+
 /* Waits for all work queued on the handle's stream. */
 int gvom_sync(gvom_handle *h);
 
@@ -172,7 +218,7 @@ int gvom_stage_times(gvom_handle *h, float *ms, int32_t n);
  * against it. */
 void *gvom_stream(gvom_handle *h);
 
-/* hipGraph capture of a launch sequence (no reference counterpart; SURVEY N2).
+/* hipGraph capture of a launch sequence (no reference counterpart).
  * Between gvom_graph_begin and gvom_graph_end, gvom_process_pointcloud_device,
  * gvom_process_batch(on_device=1) and gvom_combine_maps_device calls are recorded
  * instead of run.  Point buffers and point counts are fixed at capture; the ego
@@ -191,13 +237,14 @@ void gvom_graph_destroy(gvom_graph *g);
 
 /* Autotuner (no reference counterpart): times the ray-cast kernel on one real scan
  * (device points, as gvom_process_pointcloud_device) for block sizes {64, 128, 256},
- * {1, 2, 4} rays per thread and the LDS near-sensor box off/on, without touching the
- * ring, and keeps the fastest.  chosen = {block, rays per thread, box}; best_ms = its
- * time.  Synchronises. */
+ * {1, 2, 4} rays per thread, the LDS near-sensor box off/on and the XCD-aware
+ * (azimuth-sector) ray order off/on, without touching the ring, and keeps the fastest.
+ * chosen = {block, rays per thread, flags (bit 0 box, bit 1 sector order)}; best_ms =
+ * its time.  Synchronises. */
 int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int32_t dtype, const double ego[3],
                   const void *T, int32_t t_dtype, int32_t chosen[3], float *best_ms);
-/* Sets the ray-cast launch configuration directly. */
-int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t rays_per_thread, int32_t use_box);
+/* Sets the ray-cast launch configuration directly (flags as for gvom_autotune). */
+int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t rays_per_thread, int32_t flags);
 
 /* Differentiable form (no reference counterpart).  Processes one scan like
  * gvom_process_pointcloud into the next ring slot (points: device array of n x 3

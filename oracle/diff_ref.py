@@ -73,8 +73,9 @@ def assign(world, valid, origin, xy_res, z_res, XY, Z, ex=1, ez=1) -> Assignment
     return Assignment(pair_u, pair_i, u_xyz, col_pts, np.asarray(origin, np.float64), res)
 
 
-def loss(world: torch.Tensor, a: Assignment, w_h=0.1):
-    """world: (N, 3) float64 tensor (requires_grad for gradients)."""
+def loss(world: torch.Tensor, a: Assignment, w_h=0.1, covariances=False):
+    """world: (N, 3) float64 tensor (requires_grad for gradients).  Returns (L, sum of
+    lambda_min, height term), plus the (U, 3, 3) covariances when asked."""
     res = torch.tensor(a.res, dtype=torch.float64)
     org = torch.tensor(a.origin, dtype=torch.float64)
     q = world / res - org
@@ -90,4 +91,6 @@ def loss(world: torch.Tensor, a: Assignment, w_h=0.1):
     lam = torch.linalg.eigvalsh(C)[:, 0]
     h = world[torch.from_numpy(a.col_pts), 2]
     var = ((h - h.mean()) ** 2).mean() if len(h) else torch.zeros((), dtype=torch.float64)
+    if covariances:
+        return lam.sum() + w_h * var, lam.sum(), w_h * var, C
     return lam.sum() + w_h * var, lam.sum(), w_h * var
