@@ -41,6 +41,24 @@ STAGES = ["clear+prepare", "raycast", "sort+segments", "moments+metrics", "combi
           "records+columns", "maps2d"]
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by a plain copy's reading of the same
+    counters), or None when no summary for it exists."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.strip().strip("|").split("|")
+        name = parts[0].strip() if parts else ""
+        if len(parts) >= 2 and (name == kernel or name.startswith(kernel + "<")):
+            try:
+                return int(float(parts[-1].strip()))
+            except ValueError:
+                return None
+    return None
+
+
 def scan_stats(g, slot, V):
     """Ray steps and cells of one slot, from the exported state."""
     st = g.slot_state(slot)
@@ -139,6 +157,49 @@ def batched_leg(reps):
             "speedup_vs_plain": round(t_p / t_b, 2)}
 
 
+def host_api_leg(wl, scans, reps):
+    """Full host-API frames/s: host float32 points in (H2D of 12N bytes), the four 2D maps
+    out (D2H), as the reference's callers use Process_pointcloud + combine_maps."""
+    import gvom
+    g = gvom.Gvom(*wl.map_args)
+    for k, s in enumerate(scans):
+        g.Process_pointcloud(s.points, s.ego, s.transform)
+        g.combine_maps()
+    t0 = time.perf_counter()
+    for k in range(reps):
+        s = scans[k % len(scans)]
+        g.Process_pointcloud(s.points, s.ego, s.transform)
+        g.combine_maps()
+    el = time.perf_counter() - t0
+    return {"frames_per_s": round(reps / el, 1), "ms_per_frame": round(1e3 * el / reps, 4), "frames": reps}
+
+
+def atomics_leg(g, steps, t_ray_ms):
+    """Ceiling of non-returning int32 atomics (scattered / hot / XCD-local / workgroup
+    scope) next to the ray cast's rate of table updates (ray steps per second)."""
+    out = {}
+    for pat, name in ((0, "scattered"), (1, "hot_64_words"), (2, "xcd_local_scattered"), (3, "wg_scope_scattered")):
+        ms, n = g.probe_atomics(pat, 1 << 25)
+        out[name + "_G_per_s"] = round(n / (ms * 1e-3) / 1e9, 2)
+    out["raycast_steps_G_per_s"] = round(steps / (t_ray_ms * 1e-3) / 1e9, 2) if t_ray_ms > 0 else None
+    return out
+
+
+def copy_calibration(mib, reps=5):
+    """Device-to-device copies of a known size, for scaling HBM byte counters."""
+    import ctypes
+    import gvom_native as nat
+    hip = nat._hip_runtime()
+    n = mib << 20
+    a = nat.DeviceArray(np.zeros(n, np.uint8))
+    b = nat.DeviceArray(np.ones(n, np.uint8))
+    hip.hipDeviceSynchronize()
+    for _ in range(reps):
+        hip.hipMemcpy(a.ptr, b.ptr, ctypes.c_size_t(n), 3)
+    hip.hipDeviceSynchronize()
+    print(json.dumps({"copy_calibration_bytes": n, "copies": reps}), flush=True)
+
+
 def diff_leg(reps):
     import gvom_native as nat
     import gvom_synth
@@ -201,6 +262,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no graph, batched or diff legs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (HBM counter calibration)")
     ap.add_argument("--ray-config", default="auto",
                     help="'auto' (autotune on the first scan) or 'block,rays_per_thread,flags' "
                          "(flags: bit 0 LDS near-sensor box, bit 1 azimuth-sector ray order)")
@@ -209,6 +272,10 @@ def main():
         print("bench.py: this path runs on one GPU only; runs on more than one GPU are not supported",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     import gvom
     import gvom_native as nat
@@ -289,11 +356,14 @@ def main():
         "scan": {"cells": ss["cells"], "ray_steps": ss["steps"], "combined_cells": C},
         "raycast_config": ray_cfg,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic({"raycast": "k_raycast", "combine_tables": "k_comb_tables"}[dom]),
                      "algorithmic_bytes": int(b_dom)},
         "gen_seconds": round(gen_s, 1),
     }
     if not args.quick:
+        out["host_api"] = host_api_leg(wl, scans, max(8, args.steps // 4))
+        out["atomics"] = atomics_leg(g, ss["steps"], stages[1])
         out["batched"] = batched_leg(max(1, args.steps // 24))
         out["fwd_bwd"] = diff_leg(max(5, args.steps // 8))
     if not args.no_cpu:

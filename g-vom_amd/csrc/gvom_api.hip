@@ -25,6 +25,7 @@ hipError_t launch_grid_payloads(const Geo &g, const int32_t *pos, const int32_t 
                                 hipStream_t s);
 hipError_t launch_cloud_gather(const uint8_t *data, int64_t n, int64_t width, int64_t point_step, int64_t row_step,
                                const int32_t off[3], const int32_t type[3], int big, double *xyz, hipStream_t s);
+hipError_t launch_probe_atomics(int32_t *table, uint32_t V, int blocks, int per_thread, int pattern, hipStream_t s);
 hipError_t launch_diff_backward(const Geo &g, const DevState *st, int slot, const ScanDesc *sd,
                                 const double *world, const uint8_t *valid, const int32_t *table, const void *dr,
                                 const int32_t *pmark, const double *colh, const double *sums, double wh, double dL,
@@ -638,6 +639,40 @@ extern "C" int gvom_occupancy_grids(gvom_handle *h, double density_threshold, do
         CKH(h, hipStreamSynchronize(h->stream));
     }
     if (origin_m) origin_metres(h, origin_m);
+    return GVOM_OK;
+}
+
+extern "C" int gvom_probe_atomics(gvom_handle *h, int32_t pattern, int64_t n_atomics, float *ms) {
+    if (!h || !ms || pattern < 0 || pattern > 3 || n_atomics <= 0) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (h->capturing) {
+        h->err = "cannot probe while capturing";
+        return GVOM_ESTATE;
+    }
+    const int blocks = 2048;
+    int64_t per = n_atomics / ((int64_t)blocks * 256);
+    if (per < 1) per = 1;
+    if (per > (1 << 20)) per = 1 << 20;
+    const uint32_t V = (uint32_t)h->g.V;
+    int32_t *table = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = dalloc(&table, (int64_t)V);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(table, 0, (size_t)V * 4, h->stream);
+    if (e == hipSuccess) e = launch_probe_atomics(table, V, blocks, (int)per, pattern, h->stream);   // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+    if (e == hipSuccess) e = launch_probe_atomics(table, V, blocks, (int)per, pattern, h->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (table) (void)hipFree(table);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    CKH(h, e);
+    ms[0] = t;
+    ms[1] = (float)((double)per * blocks * 256);
     return GVOM_OK;
 }
 

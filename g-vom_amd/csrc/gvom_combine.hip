@@ -62,42 +62,57 @@ __device__ __forceinline__ void col_xy(int64_t c, const Geo &g, int64_t &x, int6
     y = c - x * g.XY;
 }
 
+// Wave-uniform int32 in a scalar register.
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 __global__ __launch_bounds__(256) void k_comb_tables(Geo g, const DevState *__restrict__ st,
                                                      const int32_t *__restrict__ slot_tables,
                                                      int32_t *__restrict__ comb_tables,
                                                      int32_t *__restrict__ col_count) {
+    // Everything but the z index is uniform across the wavefront: column coordinates,
+    // each source's shift and column base live in scalar registers, so a source word costs
+    // one vector compare and one load (SGPR base + lane offset).  All offsets fit in 32 bits
+    // (V * (B + 1) < 2^31).
     const int lane = threadIdx.x & 63;
-    const int64_t c0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * CPW;
-    if (c0 >= g.X) return;
-    const int parity = st->parity, cur = 1 - parity;
-    const int filled = st->filled_mask, have_last = st->have_last;
-    const double *co = st->slot_origin[st->last_slot];   // combined origin = newest slot's (gvom.py:229)
-    int64_t cx[CPW], cy[CPW];
-    bool cin[CPW];
+    const int32_t XY = g.XY, Z = g.Z, X = (int32_t)g.X, V = (int32_t)g.V;
+    const int32_t c0 = uni((int32_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * CPW);
+    if (c0 >= X) return;
+    const int parity = uni(st->parity), cur = 1 - parity;
+    const int filled = uni(st->filled_mask), have_last = uni(st->have_last);
+    const int last_slot = uni(st->last_slot);
+    const double *co = st->slot_origin[last_slot];   // combined origin = newest slot's (gvom.py:229)
+    int32_t cx[CPW], cy[CPW];
+    {
+        const int32_t x0 = uni(c0 / XY), y0 = uni(c0 - (c0 / XY) * XY);
+        int32_t x = x0, y = y0;
 #pragma unroll
-    for (int j = 0; j < CPW; ++j) {
-        cin[j] = c0 + j < g.X;
-        col_xy(c0 + j, g, cx[j], cy[j]);
+        for (int j = 0; j < CPW; ++j) {
+            cx[j] = x;
+            cy[j] = y;
+            if (++y == XY) { y = 0; ++x; }
+        }
     }
-    const int32_t *ltab = comb_tables + (int64_t)parity * g.V;
+    const int32_t *ltab = comb_tables + (int64_t)parity * V;
     uint32_t count[CPW] = {0, 0, 0, 0};
-    for (int z0 = 0; z0 < g.Z; z0 += 64) {
+    for (int z0 = 0; z0 < Z; z0 += 64) {
         const int z = z0 + lane;
-        const bool zin = z < g.Z;
+        const bool zin = z < Z;
         // previous combined map
         int32_t vl[CPW];
         bool okl[CPW];
         {
-            const int64_t dx = have_last ? (int64_t)(co[0] - st->comb_origin[parity][0]) : 0;
-            const int64_t dy = have_last ? (int64_t)(co[1] - st->comb_origin[parity][1]) : 0;
-            const int dz = have_last ? (int)(co[2] - st->comb_origin[parity][2]) : 0;
+            const int32_t dx = have_last ? uni((int32_t)(co[0] - st->comb_origin[parity][0])) : 0;
+            const int32_t dy = have_last ? uni((int32_t)(co[1] - st->comb_origin[parity][1])) : 0;
+            const int32_t dz = have_last ? uni((int32_t)(co[2] - st->comb_origin[parity][2])) : 0;
             const int zl = z + dz;
+            const bool zok = have_last && zin && zl >= 0 && zl < Z;
 #pragma unroll
             for (int j = 0; j < CPW; ++j) {
-                const int64_t ux = cx[j] + dx, uy = cy[j] + dy;
-                okl[j] = have_last && cin[j] && zin && ux >= 0 && ux < g.XY && uy >= 0 && uy < g.XY && zl >= 0 &&
-                         zl < g.Z;
-                vl[j] = okl[j] ? ltab[(ux * g.XY + uy) * g.Z + zl] : 0;
+                const int32_t ux = cx[j] + dx, uy = cy[j] + dy;
+                const bool cok = c0 + j < X && ux >= 0 && ux < XY && uy >= 0 && uy < XY;   // uniform
+                const int32_t *col = ltab + uni(cok ? (ux * XY + uy) * Z : 0);
+                okl[j] = cok && zok;
+                vl[j] = okl[j] ? col[zl] : 0;
             }
         }
         uint32_t mask[CPW] = {0, 0, 0, 0}, P[CPW] = {0, 0, 0, 0};
@@ -108,16 +123,18 @@ __global__ __launch_bounds__(256) void k_comb_tables(Geo g, const DevState *__re
                 const int s = s0 + k;
                 const bool use = s < g.B && ((filled >> s) & 1);
                 const double *so = st->slot_origin[use ? s : 0];
-                const int64_t dx = (int64_t)(co[0] - so[0]), dy = (int64_t)(co[1] - so[1]);
-                const int zs = z + (int)(co[2] - so[2]);
-                const int32_t *tab = slot_tables + (int64_t)(use ? s : 0) * g.V;
+                const int32_t dx = uni((int32_t)(co[0] - so[0])), dy = uni((int32_t)(co[1] - so[1]));
+                const int32_t dz = uni((int32_t)(co[2] - so[2]));
+                const int zs = z + dz;
+                const bool zok = use && zin && zs >= 0 && zs < Z;
+                const int32_t *tab = slot_tables + (int64_t)(use ? s : 0) * V;
 #pragma unroll
                 for (int j = 0; j < CPW; ++j) {
-                    const int64_t ux = cx[j] + dx, uy = cy[j] + dy;
-                    const bool ok = use && cin[j] && zin && ux >= 0 && ux < g.XY && uy >= 0 && uy < g.XY && zs >= 0 &&
-                                    zs < g.Z;
+                    const int32_t ux = cx[j] + dx, uy = cy[j] + dy;
+                    const bool cok = c0 + j < X && ux >= 0 && ux < XY && uy >= 0 && uy < XY;   // uniform
+                    const int32_t *col = tab + uni(cok ? (ux * XY + uy) * Z : 0);
                     // unused sources and off-map cells read as "no data" (0)
-                    v[k][j] = ok ? tab[(ux * g.XY + uy) * g.Z + zs] : 0;
+                    v[k][j] = (cok && zok) ? col[zs] : 0;
                 }
             }
 #pragma unroll
@@ -128,6 +145,7 @@ __global__ __launch_bounds__(256) void k_comb_tables(Geo g, const DevState *__re
                     else P[j] += (uint32_t)v[k][j];
                 }
         }
+        int32_t *out = comb_tables + (int64_t)cur * V + c0 * Z;
 #pragma unroll
         for (int j = 0; j < CPW; ++j) {
             uint32_t m = mask[j], p = P[j];
@@ -140,11 +158,11 @@ __global__ __launch_bounds__(256) void k_comb_tables(Geo g, const DevState *__re
                     p += (uint32_t)vl[j];
                 }
             }
-            if (zin && cin[j]) comb_tables[(int64_t)cur * g.V + (c0 + j) * g.Z + z] = m ? (int32_t)(GVOM_OCC | m) : (int32_t)p;
+            if (zin && c0 + j < X) out[j * Z + z] = m ? (int32_t)(GVOM_OCC | m) : (int32_t)p;
             count[j] += __popcll(__ballot(m != 0));
         }
     }
-    if (lane < CPW && cin[lane]) {
+    if (lane < CPW && c0 + lane < X) {
         uint32_t cnt = count[0];
 #pragma unroll
         for (int j = 1; j < CPW; ++j)
@@ -593,14 +611,28 @@ __global__ __launch_bounds__(256) void k_maps2d(Geo g, const DevState *__restric
             const int32_t *col = comb_tables + (int64_t)cur * g.V + c0 * g.Z;
             const CombRec *recs = comb_recs + (int64_t)cur * ccap;
             double dens = 0.0, nn = 0.0;
-            for (int z = lo; z <= hi; ++z) {
-                const int32_t t = col[z];
-                if (t >= 0) continue;
-                const CombRec &r = recs[(uint32_t)t & MASK31];
-                if (r.hit > 10) {
-                    nn += (double)r.total;
-                    dens += (double)r.hit;
+            // eight voxels per round: all table words, then all records, are in flight
+            // together; the sums still run in z order
+            for (int z0 = lo; z0 <= hi; z0 += 8) {
+                int32_t t[8], rh[8], rt[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) t[k] = (z0 + k <= hi) ? col[z0 + k] : 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    rh[k] = -1;
+                    rt[k] = 0;
+                    if (t[k] < 0) {
+                        const CombRec &r = recs[(uint32_t)t[k] & MASK31];
+                        rh[k] = r.hit;
+                        rt[k] = r.total;
+                    }
                 }
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (rh[k] > 10) {
+                        nn += (double)rt[k];
+                        dens += (double)rh[k];
+                    }
             }
             if (nn > 0.0) dens /= nn;
             pv = (int32_t)(dens * 100.0);

@@ -892,20 +892,26 @@ __global__ void k_commit_process(DevState *st, int S, int B) {
 // Zero the slot tables of the S slots this batch writes (slot choice read on device).
 // Each thread clears 4 consecutive words of one table.
 __global__ __launch_bounds__(256) void k_clear_slots(int32_t *__restrict__ tables, const DevState *__restrict__ st,
-                                                     int S, int B, int64_t V) {
-    const int64_t per = (V + 3) / 4;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 16) (&const_cast<DevState *>(st)->sec_count[0])[i] = 0;   // sec_count and sec_cursor
-    if (i >= per * S) return;
-    const int s = (int)(i / per);
-    const int slot = (st->next_slot + s) % B;
+                                                     int B, int32_t V) {
+    // grid (ceil(V / 4096), S): block (b, s) clears words [4096 b, 4096 (b + 1)) of the table
+    // of the batch's s-th slot; a wavefront stores 1 KiB contiguous per round
+    const int s = blockIdx.y;
+    if (blockIdx.x == 0 && s == 0 && threadIdx.x < 16)
+        (&const_cast<DevState *>(st)->sec_count[0])[threadIdx.x] = 0;   // sec_count and sec_cursor
+    const int slot = __builtin_amdgcn_readfirstlane((st->next_slot + s) % B);
     int32_t *t = tables + (int64_t)slot * V;
-    const int64_t j = (i - (int64_t)s * per) * 4;
+    const int32_t base = (int32_t)blockIdx.x * 4096;
     if ((V & 3) == 0) {
-        *reinterpret_cast<int4 *>(t + j) = make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int32_t j = base + (k * 256 + (int32_t)threadIdx.x) * 4;
+            if (j < V) *reinterpret_cast<int4 *>(t + j) = make_int4(0, 0, 0, 0);
+        }
     } else {
-        for (int k = 0; k < 4; ++k)
-            if (j + k < V) t[j + k] = 0;
+        for (int k = 0; k < 16; ++k) {
+            const int32_t j = base + k * 256 + (int32_t)threadIdx.x;
+            if (j < V) t[j] = 0;
+        }
     }
 }
 
@@ -1043,7 +1049,8 @@ hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t se
 }
 
 hipError_t launch_clear_slots(int32_t *slot_tables, const DevState *st, int S, const Geo &g, hipStream_t s) {
-    k_clear_slots<<<nblocks((g.V + 3) / 4 * S, 256), 256, 0, s>>>(slot_tables, st, S, g.B, g.V);
+    k_clear_slots<<<dim3((unsigned)((g.V + 4095) / 4096), (unsigned)S), 256, 0, s>>>(slot_tables, st, g.B,
+                                                                                    (int32_t)g.V);
     return hipGetLastError();
 }
 

@@ -207,6 +207,14 @@ class Gvom:
                                           ctypes.byref(best)), self._h)
         return chosen[0], chosen[1], chosen[2], best.value
 
+    def probe_atomics(self, pattern=0, n_atomics=1 << 24):
+        """Time non-returning int32 atomic adds (pattern: 0 scattered, 1 64 hot words,
+        2 scattered within the issuing XCD's region, 3 scattered at workgroup scope).
+        Returns (ms, atomics issued)."""
+        out = (ctypes.c_float * 2)()
+        nat.check(self._lib.gvom_probe_atomics(self._h, int(pattern), int(n_atomics), out), self._h)
+        return out[0], int(out[1])
+
     def set_raycast_config(self, block=256, rays_per_thread=1, use_box=False, sector_order=False):
         flags = (1 if use_box else 0) | (2 if sector_order else 0)
         nat.check(self._lib.gvom_set_raycast_config(self._h, int(block), int(rays_per_thread), flags), self._h)

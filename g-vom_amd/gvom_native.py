@@ -36,6 +36,7 @@ SIGNATURES = {
     "gvom_device_maps": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "gvom_sync": (ctypes.c_int, [_P]),
     "gvom_process_pointcloud2": (ctypes.c_int, [_P, _P, _P, _P, _P, _I32, _I32]),
+    "gvom_probe_atomics": (ctypes.c_int, [_P, _I32, _I64, _P]),
     "gvom_occupancy_grids": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P, _I32, _P]),
     "gvom_debug_voxel_map": (ctypes.c_int, [_P, _P, _I64, _P]),
     "gvom_debug_height_map": (ctypes.c_int, [_P, _P]),

@@ -164,6 +164,13 @@ int gvom_process_pointcloud2(gvom_handle *h, const void *data, const gvom_cloud_
 int gvom_occupancy_grids(gvom_handle *h, double density_threshold, double min_roughness, double max_roughness,
                          int8_t *out, int32_t on_device, double origin_m[3]);
 
+/* Atomic-throughput probe (no reference counterpart; a measurement aid): times one launch
+ * of non-returning int32 atomic adds into a scratch table of V words on the handle's
+ * stream.  pattern 0 scattered (device scope), 1 64 hot words, 2 scattered within the
+ * region of the issuing XCD, 3 scattered with workgroup scope (timing only).  ms[0] =
+ * kernel time in ms, ms[1] = atomics issued.  Synchronises; allocates and frees the table. */
+int gvom_probe_atomics(gvom_handle *h, int32_t pattern, int64_t n_atomics, float ms[2]);
+
 /* Waits for all work queued on the handle's stream. */
 int gvom_sync(gvom_handle *h);
 

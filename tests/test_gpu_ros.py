@@ -163,3 +163,34 @@ def test_node_timer_matches_oracle_path(cfg2):
     assert hm.dtype.names == ("x", "y", "z", "roughness", "slope_x", "slope_y", "slope", "obstacles")
     np.testing.assert_array_equal(hm["obstacles"], pos.flatten("F"))
     print("node timer payloads identical to the oracle path", flush=True)
+
+
+def test_pointcloud2_layout_errors(cfg2):
+    """Malformed layouts are refused with GVOM_EINVAL before anything reaches the device,
+    and leave the ring untouched."""
+    import gvom
+    import gvom_native as nat
+    g = gvom.Gvom(*cfg2.map_args)
+    s = cfg2.steps[0][0]
+    msg, _ = _message(s, *LAYOUTS["f32"])
+    before = g.ring_state()
+
+    def bad(**change):
+        m = pack_cloud(np.zeros((8, 3)), 2, 4, {"x": (0, FLOAT32), "y": (4, FLOAT32), "z": (8, FLOAT32)})
+        for k, v in change.items():
+            setattr(m, k, v)
+        return m
+
+    cases = [bad(point_step=0), bad(row_step=4), bad(height=-1)]
+    m = bad()
+    m.fields[2].datatype = 9
+    cases.append(m)
+    m = bad()
+    m.fields[0].offset = m.point_step - 2
+    cases.append(m)
+    for m in cases:
+        with pytest.raises(nat.GvomError):
+            g.process_pointcloud2(m, s.ego)
+    assert g.ring_state() == before
+    g.process_pointcloud2(msg, s.ego, s.transform.astype(np.float64))
+    assert g.ring_state() != before
