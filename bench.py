@@ -149,12 +149,25 @@ def batched_leg(reps):
             p.combine_maps_device()
     p.sync()
     t_p = (time.perf_counter() - t0) / reps
+    # the reference's calling pattern: host arrays in, host maps out, one call per scan
+    hcall = gvom.Gvom(*wl.map_args)
+    for s in scans[:8]:
+        hcall.Process_pointcloud(s.points, s.ego, s.transform)
+        hcall.combine_maps()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for s in scans:
+            hcall.Process_pointcloud(s.points, s.ego, s.transform)
+            hcall.combine_maps()
+    t_h = (time.perf_counter() - t0) / reps
     return {"workload": "3 sensors (128/64/16 x 1024, 10-40% dropout) x 8 steps, 256x256x64, buffer 8; "
                         "one hipGraph launch = 8 x (3-scan batch + combine_maps)",
             "scans_per_launch": len(scans), "points_per_launch": int(n_pts),
             "scans_per_s": round(len(scans) / t_b, 1), "ms_per_launch": round(1e3 * t_b, 3),
             "plain_frames_per_s": round(len(scans) / t_p, 1),
-            "speedup_vs_plain": round(t_p / t_b, 2)}
+            "speedup_vs_plain": round(t_p / t_b, 2),
+            "host_api_frames_per_s": round(len(scans) / t_h, 1),
+            "speedup_vs_host_api": round(t_h / t_b, 2)}
 
 
 def host_api_leg(wl, scans, reps):
@@ -178,7 +191,8 @@ def atomics_leg(g, steps, t_ray_ms):
     """Ceiling of non-returning int32 atomics (scattered / hot / XCD-local / workgroup
     scope) next to the ray cast's rate of table updates (ray steps per second)."""
     out = {}
-    for pat, name in ((0, "scattered"), (1, "hot_64_words"), (2, "xcd_local_scattered"), (3, "wg_scope_scattered")):
+    for pat, name in ((0, "scattered"), (1, "hot_64_words"), (2, "xcd_local_scattered"), (3, "wg_scope_scattered"),
+                      (4, "wave_on_64_consecutive_words"), (5, "wave_on_8_lines")):
         ms, n = g.probe_atomics(pat, 1 << 25)
         out[name + "_G_per_s"] = round(n / (ms * 1e-3) / 1e9, 2)
     out["raycast_steps_G_per_s"] = round(steps / (t_ray_ms * 1e-3) / 1e9, 2) if t_ray_ms > 0 else None
@@ -245,13 +259,34 @@ def diff_leg(reps):
         Tk = se3_exp(-step) @ Tk
     L1 = loss.forward(dev.ptr.value, pts.shape[0], Tk, s.ego)[0]
     e1 = err(Tk)
+
+    # the same perturbation as a sensor extrinsic shared by 4 frames at known vehicle poses
+    # (the frames' transforms): misalignment is observable across frames
+    from gvom_torch import refine_extrinsic
+    wl4 = gvom_synth.workload(2, frames=4)
+    frames = [st[0] for st in wl4.steps]
+    E_true = np.eye(4)
+    E0 = gvom_synth.perturb_transform(np.eye(4, dtype=np.float32), np.random.default_rng(205)).astype(np.float64)
+    loss4 = MapLoss(0.4, 0.2, 256, 64)
+    E1, Ls = refine_extrinsic([f.points.astype(np.float64) for f in frames],
+                              [f.transform.astype(np.float64) for f in frames], E0, loss4, steps=20)
+
+    def err_e(Ea):
+        dR = Ea[:3, :3] @ E_true[:3, :3].T
+        ang = np.degrees(np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1)))
+        return float(ang), float(np.linalg.norm(Ea[:3, 3] - E_true[:3, 3]))
     return {"workload": "one 64x1024 scan (65,536 rays, 5% dropout), 256x256x64; L = sum lambda_min + "
                         "0.1 * column-height variance",
             "points": int(pts.shape[0]), "fwd_ms": round(1e3 * t_f, 3), "fwd_bwd_ms": round(1e3 * t_fb, 3),
             "bwd_over_fwd": round(t_fb / t_f, 3),
             "refine_20_steps": {"loss": [round(float(L0), 4), round(float(L1), 4)],
                                 "rot_err_deg": [round(e0[0], 4), round(e1[0], 4)],
-                                "trans_err_m": [round(e0[1], 4), round(e1[1], 4)]}}
+                                "trans_err_m": [round(e0[1], 4), round(e1[1], 4)]},
+            "refine_extrinsic_4_frames_20_steps": {
+                "workload": "4 frames of 64x1024 scans at known vehicle poses, shared perturbed extrinsic",
+                "loss": [round(Ls[0], 4), round(Ls[-1], 4)],
+                "rot_err_deg": [round(err_e(E0)[0], 4), round(err_e(E1)[0], 4)],
+                "trans_err_m": [round(err_e(E0)[1], 4), round(err_e(E1)[1], 4)]}}
 
 
 def main():
@@ -266,7 +301,8 @@ def main():
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration)")
     ap.add_argument("--ray-config", default="auto",
                     help="'auto' (autotune on the first scan) or 'block,rays_per_thread,flags' "
-                         "(flags: bit 0 LDS near-sensor box, bit 1 azimuth-sector ray order)")
+                         "(flags: bit 0 LDS near-sensor box, bit 1 azimuth-sector ray order, bit 2 "
+                         "tiled scratch)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU only; runs on more than one GPU are not supported",
@@ -294,7 +330,7 @@ def main():
         ray_cfg = {"block": rc[0], "rays_per_thread": rc[1], "flags": rc[2], "autotuned_ms": round(rc[3], 4)}
     else:
         b, r, f = (int(v) for v in args.ray_config.split(","))
-        g.set_raycast_config(b, r, bool(f & 1), bool(f & 2))
+        g.set_raycast_config(b, r, bool(f & 1), bool(f & 2), bool(f & 4))
         ray_cfg = {"block": b, "rays_per_thread": r, "flags": f}
 
     def frame(k):

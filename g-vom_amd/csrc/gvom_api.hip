@@ -109,6 +109,10 @@ struct gvom_handle {
     int8_t *grids = nullptr;   // OccupancyGrid payloads staged for host copies (5 * X bytes)
     uint8_t *raw = nullptr;    // PointCloud2 bytes staged from the host
     int64_t rawcap = 0;
+    // tiled ray-cast scratch: min(B, 32) zeroed tables in the 4 x 4 x 1 segment layout
+    int ray_tiled = 0;
+    int32_t *tscratch = nullptr;
+    int tscratch_scans = 0;
 };
 
 #define CKH(h, expr)                                                                 \
@@ -125,7 +129,7 @@ static void free_all(gvom_handle *h) {
                     h->slope_x, h->slope_y, h->rough, h->delta, h->pos, h->neg, h->vis, h->occ_list,
                     h->col_count, h->chunk_tot, h->export_buf, h->pts_in, h->world, h->valid, h->k0, h->v0, h->k1, h->v1,
                     h->hist, h->blk, h->seg_key, h->seg_start, h->mom, h->part, h->sd, h->diff_rec,
-                    h->diff_colh, h->diff_part, h->diff_sums, h->diff_colpt, h->diff_pmark, h->graph_sd, h->rsec, h->rorder, h->grids, h->raw};
+                    h->diff_colh, h->diff_part, h->diff_sums, h->diff_colpt, h->diff_pmark, h->graph_sd, h->rsec, h->rorder, h->grids, h->raw, h->tscratch};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : h->ev_pool)
@@ -343,6 +347,27 @@ static void stage_event(gvom_handle *h, int k) {
         h->set_used[h->cur_set] |= 1 << k;
 }
 
+// Tiled ray-cast scratch: one zeroed table per scan a batch can hold (min(B, 32)); the
+// conversion kernel leaves it zeroed after every use.
+static int ensure_tile_scratch(gvom_handle *h) {
+    if (h->tscratch) return GVOM_OK;
+    const Geo &g = h->g;
+    const int n = g.B < GVOM_MAX_BATCH ? g.B : GVOM_MAX_BATCH;
+    const int64_t words = tile_scratch_words(g);
+    if (words * n >= ((int64_t)1 << 31)) {
+        h->err = "map too large for the tiled ray cast";
+        return GVOM_EINVAL;
+    }
+    if (h->capturing) {
+        h->err = "the tiled ray cast's scratch must exist before graph capture (process one scan first)";
+        return GVOM_ESTATE;
+    }
+    CKH(h, dalloc(&h->tscratch, words * n));
+    CKH(h, hipMemsetAsync(h->tscratch, 0, (size_t)words * n * sizeof(int32_t), h->stream));
+    h->tscratch_scans = n;
+    return GVOM_OK;
+}
+
 // The scan stage for S scans (gvom.py:105-220 for each).
 static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets, int S, int dtype,
                         const double *ego, const void *T, int t_dtype, bool on_device) {
@@ -417,6 +442,11 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
         d.offset = offsets[s];
         d.count = offsets[s + 1] - offsets[s];
     }
+    const bool tiled = h->ray_tiled != 0;
+    if (tiled) {
+        int rc_t = ensure_tile_scratch(h);
+        if (rc_t) return rc_t;
+    }
     hipStream_t st = h->stream;
     const size_t esz = dtype == GVOM_F32 ? 4 : 8;
     const void *pts = xyz;
@@ -443,13 +473,17 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
         CKH(h, hipMemcpyAsync(h->sd, sdh, sizeof(ScanDesc) * S, hipMemcpyHostToDevice, st));
     }
     stage_event(h, 0);
-    CKH(h, launch_clear_slots(h->slot_tables, h->st, S, g, st));
+    const bool xcd = h->ray_xcd != 0;
+    // the tiled path's conversion writes every word of the slot tables
+    if (!tiled || xcd) CKH(h, launch_clear_slots(h->slot_tables, h->st, S, g, st));
     CKH(h, launch_prepare(dtype, pts, h->world, h->k0, h->v0, h->valid, sd, S, NT, g, h->st,
-                          h->ray_xcd ? h->rsec : nullptr, st));
-    if (h->ray_xcd) CKH(h, launch_ray_order(h->rsec, NT, h->st, h->rorder, st));
+                          xcd ? h->rsec : nullptr, st));
+    if (xcd) CKH(h, launch_ray_order(h->rsec, NT, h->st, h->rorder, st));
     stage_event(h, 1);
-    CKH(h, launch_raycast(dtype, h->world, h->valid, sd, S, NT, g, h->st, h->slot_tables, h->ray_block,
-                          h->ray_rpt, h->ray_box, -1, h->ray_xcd ? h->rorder : nullptr, st));
+    CKH(h, launch_raycast(dtype, h->world, h->valid, sd, S, NT, g, h->st, tiled ? h->tscratch : h->slot_tables,
+                          h->ray_block, h->ray_rpt, tiled ? 0 : h->ray_box, tiled ? 1 : 0, -1,
+                          xcd ? h->rorder : nullptr, st));
+    if (tiled) CKH(h, launch_tile_convert(g, h->st, S, -1, h->tscratch, h->slot_tables, st));
     stage_event(h, 2);
     uint32_t *sk = h->k0, *sv = h->v0;
     CKH(h, radix_sort(h->k0, h->v0, h->k1, h->v1, NT, g.VB + sbits, h->hist, &sk, &sv, st));
@@ -643,7 +677,7 @@ extern "C" int gvom_occupancy_grids(gvom_handle *h, double density_threshold, do
 }
 
 extern "C" int gvom_probe_atomics(gvom_handle *h, int32_t pattern, int64_t n_atomics, float *ms) {
-    if (!h || !ms || pattern < 0 || pattern > 3 || n_atomics <= 0) return GVOM_EINVAL;
+    if (!h || !ms || pattern < 0 || pattern > 5 || n_atomics <= 0) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
     if (h->capturing) {
         h->err = "cannot probe while capturing";
@@ -1133,9 +1167,13 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
     int32_t *scratch = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int out = GVOM_OK;
-    int bb = h->ray_block, br = h->ray_rpt, bx = h->ray_box, bo = h->ray_xcd;
+    int bb = h->ray_block, br = h->ray_rpt, bx = h->ray_box, bo = h->ray_xcd, bt = h->ray_tiled;
     float best = 1e30f;
     do {
+        if (ensure_tile_scratch(h) != GVOM_OK) {
+            out = GVOM_EHIP;
+            break;
+        }
         if (dalloc(&saved, 1) != hipSuccess || dalloc(&scratch, g.V) != hipSuccess ||
             hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
             h->err = "autotune: allocation failed";
@@ -1152,19 +1190,26 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
             out = GVOM_EHIP;
             break;
         }
+        // each timed region holds what the configuration needs per scan: the table clear and
+        // the cast into the table, or the cast into the tiled scratch and its conversion
         const int blocks[3] = {64, 128, 256}, rpts[3] = {1, 2, 4};
         for (int ib = 0; ib < 3 && out == GVOM_OK; ++ib)
             for (int ir = 0; ir < 3 && out == GVOM_OK; ++ir)
-                for (int icfg = 0; icfg < 4 && out == GVOM_OK; ++icfg) {
-                    const int ibox = icfg & 1, iord = icfg >> 1;
+                for (int icfg = 0; icfg < 6 && out == GVOM_OK; ++icfg) {
+                    // icfg: 0-3 table cast (bit 0 box, bit 1 sector order); 4-5 tiled (+ order)
+                    const int itile = icfg >= 4, ibox = itile ? 0 : (icfg & 1), iord = itile ? (icfg & 1) : (icfg >> 1);
                     float t_min = 1e30f;
                     for (int rep = 0; rep < 3; ++rep) {
                         float t = 0.f;
-                        if (launch_fill_i32(scratch, 0, g.V, st) != hipSuccess || hipEventRecord(e0, st) != hipSuccess ||
-                            launch_raycast(dtype, h->world, h->valid, h->sd, 1, n, g, h->st, scratch, blocks[ib],
-                                           rpts[ir], ibox, 0, iord ? h->rorder : nullptr, st) != hipSuccess ||
-                            hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                            hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+                        bool ok = hipEventRecord(e0, st) == hipSuccess;
+                        if (ok && !itile) ok = launch_fill_i32(scratch, 0, g.V, st) == hipSuccess;
+                        ok = ok && launch_raycast(dtype, h->world, h->valid, h->sd, 1, n, g, h->st,
+                                                  itile ? h->tscratch : scratch, blocks[ib], rpts[ir], ibox, itile, 0,
+                                                  iord ? h->rorder : nullptr, st) == hipSuccess;
+                        if (ok && itile) ok = launch_tile_convert(g, h->st, 1, 0, h->tscratch, scratch, st) == hipSuccess;
+                        ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+                             hipEventElapsedTime(&t, e0, e1) == hipSuccess;
+                        if (!ok) {
                             h->err = "autotune: timing run failed";
                             out = GVOM_EHIP;
                             break;
@@ -1177,6 +1222,7 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
                         br = rpts[ir];
                         bx = ibox;
                         bo = iord;
+                        bt = itile;
                     }
                 }
         if (hipMemcpyAsync(h->st, saved, sizeof(DevState), hipMemcpyDeviceToDevice, st) != hipSuccess ||
@@ -1194,10 +1240,11 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
         h->ray_rpt = br;
         h->ray_box = bx;
         h->ray_xcd = bo;
+        h->ray_tiled = bt;
         if (chosen) {
             chosen[0] = bb;
             chosen[1] = br;
-            chosen[2] = bx | (bo << 1);
+            chosen[2] = bx | (bo << 1) | (bt << 2);
         }
         if (best_ms) *best_ms = best;
     }
@@ -1205,12 +1252,14 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
 }
 
 extern "C" int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t rays_per_thread, int32_t flags) {
-    if (!h || (block != 64 && block != 128 && block != 256) || rays_per_thread < 1 || rays_per_thread > 16)
+    if (!h || (block != 64 && block != 128 && block != 256) || rays_per_thread < 1 || rays_per_thread > 16 ||
+        (flags & 1 && flags & 4))
         return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
     h->ray_block = block;
     h->ray_rpt = rays_per_thread;
     h->ray_box = flags & 1;
     h->ray_xcd = (flags >> 1) & 1;
+    h->ray_tiled = (flags >> 2) & 1;
     return GVOM_OK;
 }

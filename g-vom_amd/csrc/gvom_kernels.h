@@ -11,12 +11,17 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s);
 hipError_t launch_prepare(int dtype, const void *in, void *world, uint32_t *keys, uint32_t *vals,
                           uint8_t *valid, const ScanDesc *sd, int S, int64_t NT, const Geo &g,
                           DevState *st, uint8_t *rsec, hipStream_t s);
+// Tiled ray-cast scratch (4 x 4 x 1 voxels per 64-byte segment): words per scan, and the
+// conversion into the slot tables (which also zeroes the scratch).
+int64_t tile_scratch_words(const Geo &g);
+hipError_t launch_tile_convert(const Geo &g, const DevState *st, int S, int force_slot, int32_t *scratch,
+                               int32_t *tables, hipStream_t s);
 // Groups valid rays by azimuth sector for the XCD-aware ray schedule.
 hipError_t launch_ray_order(const uint8_t *rsec, int64_t NT, DevState *st, uint32_t *order, hipStream_t s);
 // Exact-rule ray march counting passes into the slot tables (gvom.py:1174-1231).
 hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, const ScanDesc *sd,
                           int S, int64_t NT, const Geo &g, const DevState *st, int32_t *slot_tables,
-                          int block, int rays_per_thread, int use_box, int force_slot,
+                          int block, int rays_per_thread, int use_box, int tiled, int force_slot,
                           const uint32_t *order, hipStream_t s);
 // Stable LSD radix sort of (key, val) on the low `bits` key bits.  On return *kout/*vout
 // point at whichever buffer pair holds the result.

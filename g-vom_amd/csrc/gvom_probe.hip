@@ -8,6 +8,10 @@
 //              XCDs round-robin) stays in one part of the table
 //   pattern 3  as 0 with workgroup-scope atomics (a timing point only: such adds are not
 //              atomic across workgroups of different XCDs)
+//   pattern 4  each wavefront instruction: its 64 lanes on 64 consecutive words at a random
+//              place (two 128-byte lines)
+//   pattern 5  each wavefront instruction: 8 random 128-byte lines, 8 lanes on consecutive
+//              words of each
 #include <hip/hip_runtime.h>
 
 #include "gvom_internal.h"
@@ -20,13 +24,20 @@ __global__ __launch_bounds__(256) void k_probe_atomics(int32_t *__restrict__ tab
     uint32_t h = tid * 2654435761u + 12345u;
     const uint32_t region = V / 8u;
     const uint32_t rbase = (blockIdx.x & 7u) * region;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t hw = (blockIdx.x * 4u + (threadIdx.x >> 6)) * 2654435761u + 777u;   // per wavefront
     for (int k = 0; k < per_thread; ++k) {
         h = h * 1664525u + 1013904223u;
+        hw = hw * 1664525u + 1013904223u;
         const uint32_t r = h >> 3;
         uint32_t idx;
         if (pattern == 1) idx = r & 63u;
         else if (pattern == 2) idx = rbase + r % region;
-        else idx = r % V;
+        else if (pattern == 4) idx = ((hw >> 3) % (V / 64u)) * 64u + lane;
+        else if (pattern == 5) {
+            const uint32_t g8 = lane >> 3, hl = (hw ^ (g8 * 0x9E3779B9u)) * 2246822519u;
+            idx = ((hl >> 3) % (V / 32u)) * 32u + (lane & 7u);
+        } else idx = r % V;
         if (pattern == 3)
             __hip_atomic_fetch_add(table + idx, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         else

@@ -217,10 +217,116 @@ __global__ __launch_bounds__(256) void k_ray_order(const uint8_t *__restrict__ r
 // box of counters centred on the ego voxel of its first ray's scan in LDS; steps of
 // that scan inside the box add there, and the block flushes the non-zero counters
 // with one global atomic each at its end.
+// One ray's stepping state (gvom.py:1174-1213 typing: f32 position / slope, f64 length).
+struct Ray {
+    float p0, p1, p2, inc0, inc1, inc2;
+    double length, rl1, dlen, o0, o1, o2;
+    bool alive;
+};
+
+// Ray from the scan's ego to world point i (gvom.py:1174-1208).
+template <typename P>
+__device__ __forceinline__ void ray_init(Ray &r, const ScanDesc &d, const Geo &g, const P *__restrict__ world,
+                                         int64_t i) {
+    double o[3];
+    scan_origin(d, g, o);
+    r.o0 = o[0]; r.o1 = o[1]; r.o2 = o[2];
+    float pt0 = (float)(d.ego[0] / g.xy_res);
+    float pt1 = (float)(d.ego[1] / g.xy_res);
+    float pt2 = (float)(d.ego[2] / g.z_res);
+    float e0 = (float)((double)world[3 * i] / g.xy_res);
+    float e1 = (float)((double)world[3 * i + 1] / g.xy_res);
+    float e2 = (float)((double)world[3 * i + 2] / g.z_res);
+    float s0 = e0 - pt0, s1 = e1 - pt1, s2 = e2 - pt2;
+    float rl = sqrtf(s0 * s0 + s1 * s1 + s2 * s2);
+    s0 = s0 / rl;
+    s1 = s1 / rl;
+    s2 = s2 / rl;
+    float a0 = fabsf(s0), a1 = fabsf(s1), a2 = fabsf(s2);
+    float smax = py_maxf(a0, py_maxf(a1, a2));
+    int si = 0;
+    if (smax == a1) si = 1;
+    if (smax == a2) si = 2;
+    float sd_ = si == 0 ? s0 : (si == 1 ? s1 : s2);
+    float ad = fabsf(sd_);
+    float dir = sd_ / ad;
+    // per-axis increments: +-1 on the dominant axis, slope/|slope_dom| elsewhere
+    r.inc0 = si == 0 ? dir : s0 / ad;
+    r.inc1 = si == 1 ? dir : s1 / ad;
+    r.inc2 = si == 2 ? dir : s2 / ad;
+    r.p0 = pt0; r.p1 = pt1; r.p2 = pt2;
+    r.length = 0.0;
+    r.rl1 = (double)rl - 1.0;
+    r.dlen = fabs(1.0 / (double)sd_);
+    r.alive = r.length < r.rl1;
+}
+
+// One step of an alive ray (gvom.py:1209-1231): returns false when the voxel it enters is
+// off the map (the ray is then dead), else its coordinates; advances length.
+__device__ __forceinline__ bool ray_advance(Ray &r, const Geo &g, int32_t &ix, int32_t &iy, int32_t &iz) {
+    r.p0 += r.inc0;
+    r.p1 += r.inc1;
+    r.p2 += r.inc2;
+    const double fx = floor((double)r.p0 - r.o0);
+    const double fy = floor((double)r.p1 - r.o1);
+    const double fz = floor((double)r.p2 - r.o2);
+    if (fx < 0 || fx >= g.XY || fy < 0 || fy >= g.XY || fz < 0 || fz >= g.Z) {
+        r.alive = false;
+        return false;
+    }
+    r.length += r.dlen;
+    r.alive = r.length < r.rl1;
+    // in-map, so each index fits in 32 bits
+    ix = (int32_t)fx;
+    iy = (int32_t)fy;
+    iz = (int32_t)fz;
+    return true;
+}
+
+__device__ __forceinline__ uint32_t lin_index(int32_t ix, int32_t iy, int32_t iz, const Geo &g) {
+    return ((uint32_t)ix * (uint32_t)g.XY + (uint32_t)iy) * (uint32_t)g.Z + (uint32_t)iz;
+}
+
+// Run merge across a wavefront: lanes whose keys are equal to the previous lane's
+// continue that lane's run (the previous lane's key comes from a DPP wave shift right by
+// one; lane 0 receives a dead-lane sentinel).  Returns the run length at the head lane of
+// each run of voxel keys (< 2^31), 0 elsewhere.
+__device__ __forceinline__ int run_length(uint32_t key, int lane) {
+    const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)key, 0x138, 0xF, 0xF, false);
+    const bool is_voxel = key < 0x80000000u;
+    const bool cont = (lane > 0) && (prev == key) && is_voxel;
+    const uint64_t same = __ballot(cont);
+    if (!is_voxel || cont) return 0;
+    const uint64_t rest = (lane == 63) ? 0ull : (same >> (lane + 1));
+    return 1 + __builtin_ctzll(~rest);
+}
+
+// Ray-cast scratch layout: 4 x 4 columns x 1 z per 64-byte segment (tile t = (x/4 * ty + y/4)
+// * z_size + z, word 16 t + 4 (x%4) + y%4).  A wavefront's run heads at one step lie on an
+// arc of nearby voxels; device-scope atomics of one instruction that fall in the same 64-byte
+// segment travel as one request, and the segment holds more of the arc's voxels in this
+// layout than in the z-contiguous table (measured on the headline scan: 0.76M instead of
+// 2.1M segments for 2.16M runs).  k_tile_convert moves the counts into the slot tables.
+struct TileGeo {
+    int32_t tx, ty;    // tiles along x, y
+    int64_t vt;        // words per scan
+};
+__host__ __device__ __forceinline__ TileGeo tile_geo(const Geo &g) {
+    TileGeo t;
+    t.tx = (g.XY + 3) / 4;
+    t.ty = (g.XY + 3) / 4;
+    t.vt = (int64_t)t.tx * t.ty * g.Z * 16;
+    return t;
+}
+__device__ __forceinline__ uint32_t tiled_index(int32_t ix, int32_t iy, int32_t iz, const Geo &g, int32_t ty) {
+    return ((((uint32_t)(ix >> 2) * (uint32_t)ty + (uint32_t)(iy >> 2)) * (uint32_t)g.Z + (uint32_t)iz) << 4) +
+           ((uint32_t)(ix & 3) << 2) + (uint32_t)(iy & 3);
+}
+
 constexpr int BOX_XY = 32, BOX_Z = 16;
 constexpr int BOX_N = BOX_XY * BOX_XY * BOX_Z;   // 64 KiB of int32 counters
 
-template <typename P, bool USE_BOX>
+template <typename P, bool USE_BOX, bool TILED>
 __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
                                                  const uint8_t *__restrict__ valid,
                                                  const ScanDesc *__restrict__ sd, int S, int64_t NT,
@@ -241,6 +347,7 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
 #define RAY_INDEX(j) (order ? ((j) < o_len ? (int64_t)order[o_beg + (j)] : NT) : (j))
     const int lane = threadIdx.x & 63;
     const int next = st->next_slot;
+    const TileGeo tg = tile_geo(g);
     // force_slot >= 0 (autotuning) writes every scan into table force_slot instead of its ring slot
 #define RAY_SLOT(s) (force_slot >= 0 ? force_slot : ((next + (s)) % g.B))
     // the block's box: scan of its first ray, centred on that scan's ego voxel
@@ -263,63 +370,28 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
     }
     for (int rep = 0; rep < rpt; ++rep) {
         const int64_t i = RAY_INDEX((chunk * rpt + rep) * blockDim.x + threadIdx.x);
-        bool alive = false, inbox_scan = false;
-        float p0 = 0.f, p1 = 0.f, p2 = 0.f, inc0 = 0.f, inc1 = 0.f, inc2 = 0.f;
-        double length = 0.0, rl1 = 0.0, dlen = 0.0, o0 = 0.0, o1 = 0.0, o2 = 0.0;
+        bool inbox_scan = false;
+        Ray r;
+        r.alive = false;
         uint32_t base = 0;
         if (i < NT && valid[i]) {
-            int s = scan_of(sd, S, i);
+            const int s = scan_of(sd, S, i);
             inbox_scan = USE_BOX && s == bs;
-            const ScanDesc &d = sd[s];
-            double o[3];
-            scan_origin(d, g, o);
-            o0 = o[0]; o1 = o[1]; o2 = o[2];
-            base = (uint32_t)(RAY_SLOT(s) * g.V);
-            float pt0 = (float)(d.ego[0] / g.xy_res);
-            float pt1 = (float)(d.ego[1] / g.xy_res);
-            float pt2 = (float)(d.ego[2] / g.z_res);
-            float e0 = (float)((double)world[3 * i] / g.xy_res);
-            float e1 = (float)((double)world[3 * i + 1] / g.xy_res);
-            float e2 = (float)((double)world[3 * i + 2] / g.z_res);
-            float s0 = e0 - pt0, s1 = e1 - pt1, s2 = e2 - pt2;
-            float rl = sqrtf(s0 * s0 + s1 * s1 + s2 * s2);
-            s0 = s0 / rl;
-            s1 = s1 / rl;
-            s2 = s2 / rl;
-            float a0 = fabsf(s0), a1 = fabsf(s1), a2 = fabsf(s2);
-            float smax = py_maxf(a0, py_maxf(a1, a2));
-            int si = 0;
-            if (smax == a1) si = 1;
-            if (smax == a2) si = 2;
-            float sd_ = si == 0 ? s0 : (si == 1 ? s1 : s2);
-            float ad = fabsf(sd_);
-            float dir = sd_ / ad;
-            // per-axis increments: +-1 on the dominant axis, slope/|slope_dom| elsewhere
-            inc0 = si == 0 ? dir : s0 / ad;
-            inc1 = si == 1 ? dir : s1 / ad;
-            inc2 = si == 2 ? dir : s2 / ad;
-            p0 = pt0; p1 = pt1; p2 = pt2;
-            rl1 = (double)rl - 1.0;
-            dlen = fabs(1.0 / (double)sd_);
-            alive = length < rl1;
+            // TILED: the s-th scratch table of the batch; else the scan's slot table
+            base = TILED ? (uint32_t)(s * tg.vt) : (uint32_t)(RAY_SLOT(s) * g.V);
+            ray_init<P>(r, sd[s], g, world, i);
         }
         for (int it = 0; it < maxit; ++it) {
-            if (__ballot(alive) == 0ull) break;
+            if (__ballot(r.alive) == 0ull) break;
             uint32_t key = 0xFFFFFFFFu - (uint32_t)lane;   // unique per dead lane
             int lbox = -1;
-            if (alive) {
-                p0 += inc0;
-                p1 += inc1;
-                p2 += inc2;
-                double fx = floor((double)p0 - o0);
-                double fy = floor((double)p1 - o1);
-                double fz = floor((double)p2 - o2);
-                if (fx < 0 || fx >= g.XY || fy < 0 || fy >= g.XY || fz < 0 || fz >= g.Z) {
-                    alive = false;
-                } else {
-                    // in-map, so each index fits in 32 bits
-                    const int32_t ix = (int32_t)fx, iy = (int32_t)fy, iz = (int32_t)fz;
-                    key = base + ((uint32_t)ix * (uint32_t)g.XY + (uint32_t)iy) * (uint32_t)g.Z + (uint32_t)iz;
+            if (r.alive) {
+                int32_t ix, iy, iz;
+                const bool in = ray_advance(r, g, ix, iy, iz);
+                if (in && TILED) {
+                    key = base + tiled_index(ix, iy, iz, g, tg.ty);
+                } else if (in) {
+                    key = base + lin_index(ix, iy, iz, g);
                     if (USE_BOX && inbox_scan) {
                         const int32_t lx = ix - (int32_t)bx0, ly = iy - (int32_t)by0, lz = iz - (int32_t)bz0;
                         if (lx >= 0 && lx < BOX_XY && ly >= 0 && ly < BOX_XY && lz >= 0 && lz < BOX_Z)
@@ -327,21 +399,10 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
                     }
                 }
             }
-            // merge runs of equal keys among adjacent lanes; the previous lane's key comes
-            // from a DPP wave shift right by one (lane 0 receives a dead-lane sentinel)
-            const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)key, 0x138, 0xF, 0xF, false);
-            bool is_voxel = key < 0x80000000u;
-            bool cont = (lane > 0) && (prev == key) && is_voxel;
-            uint64_t same = __ballot(cont);
-            if (is_voxel && !cont) {
-                uint64_t rest = (lane == 63) ? 0ull : (same >> (lane + 1));
-                int len = 1 + __builtin_ctzll(~rest);
+            const int len = run_length(key, lane);
+            if (len > 0) {
                 if (USE_BOX && lbox >= 0) atomicAdd(&box[lbox], len);
                 else __hip_atomic_fetch_add(tables + key, len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            if (alive) {
-                length += dlen;
-                alive = length < rl1;
             }
         }
     }
@@ -365,60 +426,79 @@ constexpr int RS_TILE = 2048;
 // Points per moment chunk (one wavefront, see k_mom_part).
 constexpr int MCH = 64;
 
-__global__ __launch_bounds__(256) void k_rs_hist(const uint32_t *__restrict__ keys, int64_t n, int shift,
-                                                 uint32_t *__restrict__ hist) {
+// Tiles of RS_TILE keys, RS_THREADS threads (2 keys each): more wavefronts per tile keep
+// the short passes of a ~10^5-key sort from running on a quarter of the chip.
+constexpr int RS_THREADS = 1024;
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_ROWS = RS_TILE / RS_THREADS;   // 64-key rows per wavefront
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t *__restrict__ keys, int64_t n, int shift,
+                                                        uint32_t *__restrict__ hist) {
     __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    if (threadIdx.x < 256) h[threadIdx.x] = 0;
     __syncthreads();
-    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        int64_t i = base + k * 256 + threadIdx.x;
+    for (int k = 0; k < RS_ROWS; ++k) {
+        const int64_t i = base + k * RS_THREADS + threadIdx.x;
         if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
     }
     __syncthreads();
-    hist[(int64_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];   // block-major
+    if (threadIdx.x < 256) hist[(int64_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];   // block-major
 }
 
-__global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t *__restrict__ kin,
-                                                    const uint32_t *__restrict__ vin,
-                                                    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                    int64_t n, int shift, const uint32_t *__restrict__ hist,
-                                                    int nblk) {
-    __shared__ uint32_t wcnt[4][256];
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *__restrict__ kin,
+                                                           const uint32_t *__restrict__ vin,
+                                                           uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                           int64_t n, int shift, const uint32_t *__restrict__ hist,
+                                                           int nblk) {
+    __shared__ uint32_t wcnt[RS_WAVES][256];
+    __shared__ uint32_t ptot[4][256], pbef[4][256];
     __shared__ uint32_t gbase[256];
     __shared__ uint32_t wtot[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int k = threadIdx.x; k < 1024; k += 256) (&wcnt[0][0])[k] = 0;
+    for (int k = threadIdx.x; k < RS_WAVES * 256; k += RS_THREADS) (&wcnt[0][0])[k] = 0;
     {
         // global offset of digit d for this block: keys of smaller digits in all blocks plus
-        // keys of digit d in earlier blocks (hist is block-major, so the reads coalesce)
-        const int d = threadIdx.x;
+        // keys of digit d in earlier blocks.  Four threads per digit each add every fourth
+        // block's count (hist is block-major, so each round of loads coalesces).
+        const int d = threadIdx.x & 255, q = threadIdx.x >> 8;
         const int me = (int)blockIdx.x;
         uint32_t before = 0, total = 0;
-#pragma unroll 16
-        for (int b = 0; b < nblk; ++b) {
+#pragma unroll 8
+        for (int b = q; b < nblk; b += 4) {
             const uint32_t c = hist[(int64_t)b * 256 + d];
             total += c;
             before += (b < me) ? c : 0u;
         }
+        ptot[q][d] = total;
+        pbef[q][d] = before;
+    }
+    __syncthreads();
+    if (threadIdx.x < 256) {
+        const int d = threadIdx.x;
+        const uint32_t total = ptot[0][d] + ptot[1][d] + ptot[2][d] + ptot[3][d];
+        const uint32_t before = pbef[0][d] + pbef[1][d] + pbef[2][d] + pbef[3][d];
         uint32_t v = total;
         for (int off = 1; off < 64; off <<= 1) {
             uint32_t u = __shfl_up(v, off);
             if (lane >= off) v += u;
         }
         if (lane == 63) wtot[w] = v;
-        __syncthreads();
-        uint32_t wo = 0;
-        for (int ww = 0; ww < w; ++ww) wo += wtot[ww];
-        gbase[d] = wo + v - total + before;
+        ptot[0][d] = v - total + before;   // digit-local part, wave offsets added below
     }
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * RS_TILE + w * 512;
+    if (threadIdx.x < 256) {
+        uint32_t wo = 0;
+        for (int ww = 0; ww < w; ++ww) wo += wtot[ww];
+        gbase[threadIdx.x] = wo + ptot[0][threadIdx.x];
+    }
+    // rank every key inside its wavefront's rows (stable: rows in key order, waves in order)
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE + w * (RS_ROWS * 64);
     const uint64_t lt = (1ull << lane) - 1ull;
-    uint32_t kk[8], vv[8], lo[8];
+    uint32_t kk[RS_ROWS], vv[RS_ROWS], lo[RS_ROWS];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
+    for (int r = 0; r < RS_ROWS; ++r) {
         int64_t i = base + r * 64 + lane;
         bool in = i < n;
         uint32_t key = in ? kin[i] : 0u;
@@ -440,11 +520,11 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t *__restrict__
         lo[r] = prior + rank;
     }
     __syncthreads();
-    {
+    if (threadIdx.x < 256) {
         const int d = threadIdx.x;
         uint32_t a = 0;
 #pragma unroll
-        for (int ww = 0; ww < 4; ++ww) {
+        for (int ww = 0; ww < RS_WAVES; ++ww) {
             uint32_t c = wcnt[ww][d];
             wcnt[ww][d] = a;
             a += c;
@@ -452,7 +532,7 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t *__restrict__
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
+    for (int r = 0; r < RS_ROWS; ++r) {
         int64_t i = base + r * 64 + lane;
         if (i < n) {
             uint32_t d = (kk[r] >> shift) & 0xFFu;
@@ -964,33 +1044,72 @@ hipError_t launch_ray_order(const uint8_t *rsec, int64_t NT, DevState *st, uint3
 }
 
 hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, const ScanDesc *sd,
-                          int S, int64_t NT, const Geo &g, const DevState *st, int32_t *slot_tables,
-                          int block, int rpt, int use_box, int force_slot, const uint32_t *order,
+                          int S, int64_t NT, const Geo &g, const DevState *st, int32_t *tables,
+                          int block, int rpt, int use_box, int tiled, int force_slot, const uint32_t *order,
                           hipStream_t s) {
     if (NT <= 0) return hipSuccess;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
+    if (tiled && use_box) return hipErrorInvalidValue;
     if (rpt < 1) rpt = 1;
     int maxit = (g.XY > g.Z ? g.XY : g.Z) + 2;
     unsigned nb = nblocks(NT, block * rpt);
     if (order) nb *= 8;   // 8 sectors, each sized for every ray in the worst case
     const size_t lds = use_box ? BOX_N * sizeof(int32_t) : 0;
+#define RAY_LAUNCH(P, BOX, TL)                                                                              \
+    k_raycast<P, BOX, TL><<<nb, block, BOX ? lds : 0, s>>>((const P *)world, valid, sd, S, NT, g, st, tables, \
+                                                           maxit, rpt, force_slot, order)
     if (dtype == GVOM_F32) {
-        if (use_box)
-            k_raycast<float, true><<<nb, block, lds, s>>>((const float *)world, valid, sd, S, NT, g, st, slot_tables,
-                                                          maxit, rpt, force_slot, order);
-        else
-            k_raycast<float, false><<<nb, block, 0, s>>>((const float *)world, valid, sd, S, NT, g, st, slot_tables,
-                                                         maxit, rpt, force_slot, order);
+        if (tiled) RAY_LAUNCH(float, false, true);
+        else if (use_box) RAY_LAUNCH(float, true, false);
+        else RAY_LAUNCH(float, false, false);
     } else {
-        if (use_box)
-            k_raycast<double, true><<<nb, block, lds, s>>>((const double *)world, valid, sd, S, NT, g, st,
-                                                           slot_tables, maxit, rpt, force_slot, order);
-        else
-            k_raycast<double, false><<<nb, block, 0, s>>>((const double *)world, valid, sd, S, NT, g, st,
-                                                          slot_tables, maxit, rpt, force_slot, order);
+        if (tiled) RAY_LAUNCH(double, false, true);
+        else if (use_box) RAY_LAUNCH(double, true, false);
+        else RAY_LAUNCH(double, false, false);
     }
+#undef RAY_LAUNCH
     return hipGetLastError();
 }
+
+// Tiled scratch -> slot tables, and the scratch back to zero: one block per 4 x 4 tile
+// column of one scan of the batch.  The tile column's z_size * 16 words are contiguous in
+// the scratch; each of its 4 x-rows is 4 * z_size contiguous words of the slot table.
+__global__ __launch_bounds__(256) void k_tile_convert(Geo g, const DevState *__restrict__ st, int force_slot,
+                                                      int32_t *__restrict__ scratch, int32_t *__restrict__ tables) {
+    __shared__ int32_t sh[256 * 16];   // [z][16] for z_size <= 256
+    const TileGeo tg = tile_geo(g);
+    const int tcol = blockIdx.x, s = blockIdx.y;
+    const int txi = tcol / tg.ty, tyi = tcol - (tcol / tg.ty) * tg.ty;
+    const int Z = g.Z;
+    int32_t *src = scratch + (int64_t)s * tg.vt + (int64_t)tcol * Z * 16;
+    for (int k = threadIdx.x; k < Z * 16; k += 256) {
+        sh[k] = src[k];
+        src[k] = 0;
+    }
+    __syncthreads();
+    const int slot = force_slot >= 0 ? force_slot : (st->next_slot + s) % g.B;
+    int32_t *tab = tables + (int64_t)slot * g.V;
+    // x-row a (0..3): columns (x, y0 .. y0 + ny - 1), ny * Z contiguous words
+    const int y0 = tyi * 4, ny = (g.XY - y0) < 4 ? (g.XY - y0) : 4;
+    for (int a = 0; a < 4; ++a) {
+        const int x = txi * 4 + a;
+        if (x >= g.XY) break;
+        int32_t *row = tab + ((int64_t)x * g.XY + y0) * Z;
+        for (int k = threadIdx.x; k < ny * Z; k += 256) {
+            const int b = k / Z, z = k - (k / Z) * Z;
+            row[k] = sh[z * 16 + a * 4 + b];
+        }
+    }
+}
+
+hipError_t launch_tile_convert(const Geo &g, const DevState *st, int S, int force_slot, int32_t *scratch,
+                               int32_t *tables, hipStream_t s) {
+    const TileGeo tg = tile_geo(g);
+    k_tile_convert<<<dim3((unsigned)(tg.tx * tg.ty), (unsigned)S), 256, 0, s>>>(g, st, force_slot, scratch, tables);
+    return hipGetLastError();
+}
+
+int64_t tile_scratch_words(const Geo &g) { return tile_geo(g).vt; }
 
 int64_t radix_hist_entries(int64_t n) { return 256 * (int64_t)nblocks(n, RS_TILE) + 256; }
 
@@ -1002,8 +1121,8 @@ hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, in
     int nblk = (int)nblocks(n, RS_TILE);
     uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
     for (int shift = 0; shift < bits; shift += 8) {
-        k_rs_hist<<<nblk, 256, 0, s>>>(ka, n, shift, hist);
-        k_rs_scatter<<<nblk, 256, 0, s>>>(ka, va, kb, vb, n, shift, hist, nblk);
+        k_rs_hist<<<nblk, RS_THREADS, 0, s>>>(ka, n, shift, hist);
+        k_rs_scatter<<<nblk, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, hist, nblk);
         uint32_t *tk = ka; ka = kb; kb = tk;
         uint32_t *tv = va; va = vb; vb = tv;
     }

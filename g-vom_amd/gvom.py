@@ -195,8 +195,9 @@ class Gvom:
                   self._h)
 
     def autotune(self, dev_points, n, ego_position, transform=None, dtype=np.float32):
-        """Pick the ray-cast launch configuration by timing it on one device-resident scan.
-        Returns (block, rays_per_thread, flags (bit 0 LDS box, bit 1 sector order), best_ms)."""
+        """Pick the ray-cast variant by timing it on one device-resident scan.  Returns
+        (block, rays_per_thread, flags, best_ms); flags bit 0 LDS box, bit 1 sector order,
+        bit 2 tiled scratch."""
         ptr = dev_points.ptr if isinstance(dev_points, nat.DeviceArray) else ctypes.c_void_p(int(dev_points))
         ego = np.ascontiguousarray(np.asarray(ego_position, dtype=np.float64).reshape(3))
         T, tdt = _as_transform(transform)
@@ -209,14 +210,16 @@ class Gvom:
 
     def probe_atomics(self, pattern=0, n_atomics=1 << 24):
         """Time non-returning int32 atomic adds (pattern: 0 scattered, 1 64 hot words,
-        2 scattered within the issuing XCD's region, 3 scattered at workgroup scope).
+        2 scattered within the issuing XCD's region, 3 scattered at workgroup scope, 4 a
+        wavefront's lanes on 64 consecutive words, 5 a wavefront on 8 lines of 8 lanes).
         Returns (ms, atomics issued)."""
         out = (ctypes.c_float * 2)()
         nat.check(self._lib.gvom_probe_atomics(self._h, int(pattern), int(n_atomics), out), self._h)
         return out[0], int(out[1])
 
-    def set_raycast_config(self, block=256, rays_per_thread=1, use_box=False, sector_order=False):
-        flags = (1 if use_box else 0) | (2 if sector_order else 0)
+    def set_raycast_config(self, block=256, rays_per_thread=1, use_box=False, sector_order=False, tiled=False):
+        """Ray-cast launch shape and variant (tiled scratch excludes the LDS box)."""
+        flags = (1 if use_box else 0) | (2 if sector_order else 0) | (4 if tiled else 0)
         nat.check(self._lib.gvom_set_raycast_config(self._h, int(block), int(rays_per_thread), flags), self._h)
 
     # -- hipGraph capture ------------------------------------------------------------

@@ -108,3 +108,51 @@ def map_loss(points, xi, T0, ego, loss: MapLoss):
     if _FN is None:
         _FN = _function()
     return _FN.apply(points, xi, T0, ego, loss)
+
+
+def refine_extrinsic(scan_points, poses, E0, loss, steps=20, lr=(0.01, 0.01, 0.01, 0.002, 0.002, 0.002), ego=None):
+    """Refine a sensor extrinsic E (sensor -> vehicle) from several scans taken at known
+    vehicle poses P_i (vehicle -> world), by Adam on the map loss of their union.
+
+    scan_points: list of (N_i, 3) float64 arrays in the sensor frame; poses: list of 4x4;
+    E0: starting 4x4 extrinsic.  Each step maps every scan with W_i = P_i E p, runs the
+    HIP forward/backward on the union (identity transform) for dL/dW, and chains it to a
+    left perturbation E <- exp(xi) E: dL/drho = sum R_i^T g, dL/dphi = sum q x (R_i^T g)
+    with q = E p.  Returns (E, [loss per step])."""
+    E = np.asarray(E0, np.float64).copy()
+    P = [np.asarray(p, np.float64) for p in poses]
+    if ego is None:
+        ego = np.mean([p[:3, 3] for p in P], axis=0)
+    lr = np.asarray(lr, np.float64)
+    m = np.zeros(6)
+    v = np.zeros(6)
+    n = int(sum(s.shape[0] for s in scan_points))
+    dev = nat.DeviceArray(np.zeros((n, 3)))
+    gdev = nat.DeviceArray(np.zeros((n, 3)))
+    hip = nat._hip_runtime()
+    losses = []
+    eye = np.eye(4)
+    for k in range(1, steps + 2):
+        qs = [s @ E[:3, :3].T + E[:3, 3] for s in scan_points]                  # vehicle frame
+        W = np.concatenate([q @ p[:3, :3].T + p[:3, 3] for q, p in zip(qs, P)])
+        W = np.ascontiguousarray(W)
+        hip.hipMemcpy(dev.ptr, W.ctypes.data_as(ctypes.c_void_p), W.nbytes, 1)
+        out = loss.forward(dev.ptr.value, n, eye, ego)
+        losses.append(float(out[0]))
+        if k == steps + 1:
+            break
+        loss.backward(1.0, None, gdev.ptr.value)
+        gw = np.empty((n, 3))
+        hip.hipMemcpy(gw.ctypes.data_as(ctypes.c_void_p), gdev.ptr, gw.nbytes, 2)
+        grad = np.zeros(6)
+        o = 0
+        for q, p in zip(qs, P):
+            gl = gw[o:o + q.shape[0]] @ p[:3, :3]                                # R_i^T g per point
+            grad[:3] += gl.sum(0)
+            grad[3:] += np.cross(q, gl).sum(0)
+            o += q.shape[0]
+        m = 0.9 * m + 0.1 * grad
+        v = 0.999 * v + 0.001 * grad * grad
+        step = lr * (m / (1 - 0.9 ** k)) / (np.sqrt(v / (1 - 0.999 ** k)) + 1e-12)
+        E = se3_exp(-step) @ E
+    return E, losses

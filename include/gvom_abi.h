@@ -167,7 +167,8 @@ int gvom_occupancy_grids(gvom_handle *h, double density_threshold, double min_ro
 /* Atomic-throughput probe (no reference counterpart; a measurement aid): times one launch
  * of non-returning int32 atomic adds into a scratch table of V words on the handle's
  * stream.  pattern 0 scattered (device scope), 1 64 hot words, 2 scattered within the
- * region of the issuing XCD, 3 scattered with workgroup scope (timing only).  ms[0] =
+ * region of the issuing XCD, 3 scattered with workgroup scope (timing only), 4 each
+ * wavefront's 64 lanes on 64 consecutive words, 5 each wavefront on 8 lines of 8 lanes.  ms[0] =
  * kernel time in ms, ms[1] = atomics issued.  Synchronises; allocates and frees the table. */
 int gvom_probe_atomics(gvom_handle *h, int32_t pattern, int64_t n_atomics, float ms[2]);
 
@@ -242,15 +243,19 @@ int gvom_graph_scans(const gvom_graph *g);
 int gvom_graph_launch(gvom_handle *h, gvom_graph *g, const double *ego, const void *T, int32_t t_dtype);
 void gvom_graph_destroy(gvom_graph *g);
 
-/* Autotuner (no reference counterpart): times the ray-cast kernel on one real scan
- * (device points, as gvom_process_pointcloud_device) for block sizes {64, 128, 256},
- * {1, 2, 4} rays per thread, the LDS near-sensor box off/on and the XCD-aware
- * (azimuth-sector) ray order off/on, without touching the ring, and keeps the fastest.
- * chosen = {block, rays per thread, flags (bit 0 box, bit 1 sector order)}; best_ms =
- * its time.  Synchronises. */
+/* Autotuner (no reference counterpart): times the ray cast on one real scan (device
+ * points, as gvom_process_pointcloud_device), without touching the ring, for block sizes
+ * {64, 128, 256}, {1, 2, 4} rays per thread and the variants
+ *   - cast into the slot table (after its clear), LDS near-sensor box off/on;
+ *   - cast into a tiled scratch table (4 x 4 x 1 voxels per 64-byte segment, so the run
+ *     heads of one wavefront step share fewer atomic requests) and its conversion;
+ *   - each with the XCD-aware (azimuth-sector) ray order off/on,
+ * and keeps the fastest.  chosen = {block, rays per thread, flags (bit 0 box, bit 1 sector
+ * order, bit 2 tiled)}; best_ms = its time.  Synchronises. */
 int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int32_t dtype, const double ego[3],
                   const void *T, int32_t t_dtype, int32_t chosen[3], float *best_ms);
-/* Sets the ray-cast launch configuration directly (flags as for gvom_autotune). */
+/* Sets the ray-cast configuration directly (flags as for gvom_autotune; the box and the
+ * tiled scratch exclude each other). */
 int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t rays_per_thread, int32_t flags);
 
 /* Differentiable form (no reference counterpart).  Processes one scan like

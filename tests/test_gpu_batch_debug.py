@@ -135,3 +135,33 @@ def test_graph_replay_matches_eager(cfg4):
             if a_s is not None:
                 _state_equal(a_s, b_s, f"replay {r} slot {slot}")
         print(f"graph replay {r} ok", flush=True)
+
+
+@pytest.mark.parametrize("flags", [4, 6, 1, 2])
+def test_ray_cast_variants_equal(cfg4, flags):
+    """Every ray-cast variant the autotuner chooses from (tiled scratch, sector order, LDS
+    box) leaves slot tables and records bit-identical to the plain cast, for single scans
+    and ragged 3-scan batches."""
+    import gvom
+    wl = cfg4
+    a = gvom.Gvom(*wl.map_args)
+    b = gvom.Gvom(*wl.map_args)
+    b.set_raycast_config(128, 2, bool(flags & 1), bool(flags & 2), bool(flags & 4))
+    for k, step in enumerate(wl.steps):
+        if k == 0:
+            for s in step:
+                a.Process_pointcloud(s.points, s.ego, s.transform)
+                b.Process_pointcloud(s.points, s.ego, s.transform)
+        else:
+            pts = np.concatenate([s.points for s in step])
+            off = np.cumsum([0] + [s.points.shape[0] for s in step])
+            egos = np.stack([s.ego for s in step])
+            Ts = np.stack([s.transform for s in step])
+            a.process_batch(pts, off, egos, Ts)
+            b.process_batch(pts, off, egos, Ts)
+        for slot in range(wl.map_args[4]):
+            sa, sb = a.slot_state(slot), b.slot_state(slot)
+            assert (sa is None) == (sb is None)
+            if sa is not None:
+                _state_equal(sa, sb, f"flags {flags} step {k} slot {slot}")
+        print(f"ray variant {flags}: step {k} slots identical", flush=True)
