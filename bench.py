@@ -37,7 +37,7 @@ import numpy as np  # noqa: E402
 
 METRIC = "scan->map frames/s at 131k pts, 256x256x64 voxels; fwd+bwd ms; % HBM BW"
 HBM_PEAK_GBS = 8000.0
-STAGES = ["clear+prepare", "raycast", "sort+segments", "moments+metrics", "combine_tables",
+STAGES = ["clear+prepare", "raycast", "sort+segments_beyond_raycast", "moments+metrics", "combine_tables",
           "records+columns", "maps2d"]
 
 

@@ -47,6 +47,10 @@ struct gvom_handle {
     Geo g;
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream for the sort / segment / chunk-partial chain, which does not depend on
+    // the ray cast; forked after k_prepare and joined before the segment sums
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::mutex mu;
     std::string err;
 
@@ -134,6 +138,9 @@ static void free_all(gvom_handle *h) {
         if (p) (void)hipFree(p);
     for (auto &e : h->ev_pool)
         if (e) (void)hipEventDestroy(e);
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->side) (void)hipStreamDestroy(h->side);
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
@@ -203,6 +210,9 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     } while (0)
     CKC(hipGetDevice(&h->device));
     CKC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    CKC(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+    CKC(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    CKC(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
     CKC(dalloc(&h->st, 1));
     CKC(hipMemset(h->st, 0, sizeof(DevState)));
     CKC(dalloc(&h->slot_tables, V * g.B));
@@ -479,21 +489,28 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     CKH(h, launch_prepare(dtype, pts, h->world, h->k0, h->v0, h->valid, sd, S, NT, g, h->st,
                           xcd ? h->rsec : nullptr, st));
     if (xcd) CKH(h, launch_ray_order(h->rsec, NT, h->st, h->rorder, st));
+    // fork: keys, segments and chunk partials on the side stream, concurrent with the cast
+    CKH(h, hipEventRecord(h->ev_fork, st));
+    CKH(h, hipStreamWaitEvent(h->side, h->ev_fork, 0));
+    uint32_t *sk = h->k0, *sv = h->v0;
+    CKH(h, radix_sort(h->k0, h->v0, h->k1, h->v1, NT, g.VB + sbits, h->hist, &sk, &sv, h->side));
+    h->sorted_vals = sv;
+    CKH(h, launch_segments(sk, NT, h->seg_start, h->seg_key, h->blk, S, g, h->st, h->side));
+    CKH(h, launch_mom_part(dtype, h->world, sk, sv, NT, sd, S, g, h->st, h->part, h->side));
+    CKH(h, hipEventRecord(h->ev_join, h->side));
     stage_event(h, 1);
     CKH(h, launch_raycast(dtype, h->world, h->valid, sd, S, NT, g, h->st, tiled ? h->tscratch : h->slot_tables,
                           h->ray_block, h->ray_rpt, tiled ? 0 : h->ray_box, tiled ? 1 : 0, -1,
                           xcd ? h->rorder : nullptr, st));
     if (tiled) CKH(h, launch_tile_convert(g, h->st, S, -1, h->tscratch, h->slot_tables, st));
     stage_event(h, 2);
-    uint32_t *sk = h->k0, *sv = h->v0;
-    CKH(h, radix_sort(h->k0, h->v0, h->k1, h->v1, NT, g.VB + sbits, h->hist, &sk, &sv, st));
-    h->sorted_vals = sv;
-    CKH(h, launch_segments(sk, NT, h->seg_start, h->seg_key, h->blk, S, g, h->st, st));
+    // join: the segment sums read the pass counts and the partials
+    CKH(h, hipStreamWaitEvent(st, h->ev_join, 0));
     stage_event(h, 3);
-    CKH(h, launch_moments(dtype, h->world, sk, sv, h->seg_start, h->seg_key, NT, sd, S, g, h->st, h->slot_tables,
-                          h->slot_recs, h->ncap, h->part, h->mom, st));
-    CKH(h, launch_metrics(h->seg_key, h->mom, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, st));
-    CKH(h, launch_commit_process(h->st, S, g.B, st));
+    CKH(h, launch_mom_seg(h->seg_start, h->seg_key, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, h->part,
+                          h->mom, st));
+    // k_metrics' last block also advances the ring on the device
+    CKH(h, launch_metrics(h->seg_key, h->mom, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, S, st));
     stage_event(h, 4);
 
     for (int s = 0; s < S; ++s) {
@@ -607,7 +624,6 @@ static int combine_impl(gvom_handle *h) {
     stage_event(h, 7);
     CKH(h, launch_maps2d(g, h->st, h->height, h->inferred, h->comb_tables, h->comb_recs, h->ccap, h->slope_x,
                          h->slope_y, h->rough, h->delta, h->pos, h->neg, h->vis, st));
-    CKH(h, launch_commit_combine(h->st, h->chunk_tot, g.X, st));
     stage_event(h, 8);
     h->parity = cur;
     h->have_last = 1;

@@ -190,8 +190,15 @@ __global__ __launch_bounds__(1024) void k_col_scan(int32_t *__restrict__ cnt, in
         st->comb_origin[cur][2] = co[2];
     }
     const int64_t i = (int64_t)blockIdx.x * COLSCAN + 4 * t;
+    // four counts per thread; past the last column they read as 0 and are not written
     int4 v = make_int4(0, 0, 0, 0);
-    if (i < X) v = *reinterpret_cast<const int4 *>(cnt + i);   // X is a multiple of 16
+    if (i + 3 < X) {
+        v = *reinterpret_cast<const int4 *>(cnt + i);
+    } else if (i < X) {
+        v.x = cnt[i];
+        if (i + 1 < X) v.y = cnt[i + 1];
+        if (i + 2 < X) v.z = cnt[i + 2];
+    }
     const int32_t sum = v.x + v.y + v.z + v.w;
     int32_t incl = sum;
     for (int off = 1; off < 64; off <<= 1) {
@@ -202,13 +209,17 @@ __global__ __launch_bounds__(1024) void k_col_scan(int32_t *__restrict__ cnt, in
     __syncthreads();
     int32_t pre = incl - sum;
     for (int ww = 0; ww < w; ++ww) pre += wsum[ww];
-    if (i < X) {
-        int4 o;
-        o.x = pre;
-        o.y = pre + v.x;
-        o.z = o.y + v.y;
-        o.w = o.z + v.z;
+    int4 o;
+    o.x = pre;
+    o.y = pre + v.x;
+    o.z = o.y + v.y;
+    o.w = o.z + v.z;
+    if (i + 3 < X) {
         *reinterpret_cast<int4 *>(cnt + i) = o;
+    } else if (i < X) {
+        cnt[i] = o.x;
+        if (i + 1 < X) cnt[i + 1] = o.y;
+        if (i + 2 < X) cnt[i + 2] = o.z;
     }
     if (t == 1023) chunk_tot[blockIdx.x] = pre + sum;
 }
@@ -375,6 +386,7 @@ __global__ __launch_bounds__(256) void k_comb_merge(Geo g, const DevState *__res
     uint32_t C = 0;
     for (int64_t k = lane; k < nchunk; k += 64) C += (uint32_t)chunk_tot[k];
     for (int off = 32; off > 0; off >>= 1) C += __shfl_xor(C, off);
+    if ((int64_t)C > ccap) C = (uint32_t)ccap;   // the compact list and records hold ccap cells
     CombRec *crec = comb_recs + (int64_t)cur * ccap;
     const CombRec *lrec = comb_recs + (int64_t)parity * ccap;
     const int32_t *ltab = comb_tables + (int64_t)parity * g.V;
@@ -427,6 +439,22 @@ __global__ __launch_bounds__(256) void k_comb_merge(Geo g, const DevState *__res
     }
 }
 
+// Commit of the combine (gvom.py:276-316): cell count of the new map (sum of the column-scan
+// chunk totals) and the ping-pong flip, which k_maps2d and later readers see.  One wavefront.
+__global__ void k_commit_combine(DevState *st, const int32_t *__restrict__ chunk_tot, int64_t nchunk) {
+    if (blockIdx.x != 0) return;
+    const int lane = threadIdx.x & 63;
+    uint32_t tot = 0;
+    for (int64_t k = lane; k < nchunk; k += 64) tot += (uint32_t)chunk_tot[k];
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    if (threadIdx.x != 0) return;
+    const int cur = 1 - st->parity;
+    st->comb_cells[cur] = (int32_t)tot;
+    st->comb_total = (int32_t)tot;
+    st->parity = cur;
+    st->have_last = 1;
+}
+
 // ------------------------------------------------------------------------------------
 constexpr int TILE = 16;
 constexpr int HALO = 15;
@@ -440,7 +468,11 @@ __device__ __forceinline__ int first_in(uint64_t m, int lo, int hi) {
     return r ? lo + __builtin_ctzll(r) : -1;
 }
 
-__global__ __launch_bounds__(256) void k_maps2d(Geo g, const DevState *__restrict__ st,
+// One block of 512 threads per 16 x 16 tile: threads 0-255 run the slope fit and the guessed
+// height of their column, threads 256-511 walk the same columns' positive-obstacle windows
+// (dependent table and record loads), so the two latency chains overlap; the first half
+// then writes the maps.
+__global__ __launch_bounds__(512) void k_maps2d(Geo g, const DevState *__restrict__ st,
                                                 const double *__restrict__ height,
                                                 const double *__restrict__ inferred,
                                                 const int32_t *__restrict__ comb_tables,
@@ -452,33 +484,87 @@ __global__ __launch_bounds__(256) void k_maps2d(Geo g, const DevState *__restric
     __shared__ double H[LT][LT];
     __shared__ uint64_t rowm[LT];   // bit ly set when H[lx][ly] is observed (rows: fixed x)
     __shared__ uint64_t colm[LT];   // bit lx set when H[lx][ly] is observed (columns: fixed y)
+    __shared__ int32_t winpv[TILE * TILE];   // window density result per column of the tile
     const int XY = g.XY;
     const int bx = blockIdx.y * TILE, by = blockIdx.x * TILE;   // tile origin (x rows, y fastest)
-    for (int k = threadIdx.x; k < LT * LT; k += 256) {
+    for (int k = threadIdx.x; k < LT * LT; k += 512) {
         int lx = k / LT, ly = k % LT;
         int gx = bx - HALO + lx, gy = by - HALO + ly;
         H[lx][ly] = (gx >= 0 && gx < XY && gy >= 0 && gy < XY) ? height[(int64_t)gx * XY + gy] : -1000.0;
     }
     __syncthreads();
-    if (threadIdx.x < LT) {
-        const int lx = threadIdx.x;
-        uint64_t m = 0;
-        for (int ly = 0; ly < LT; ++ly)
-            if (H[lx][ly] > -1000) m |= 1ull << ly;
-        rowm[lx] = m;
-    } else if (threadIdx.x >= 64 && threadIdx.x < 64 + LT) {
-        const int ly = threadIdx.x - 64;
-        uint64_t m = 0;
-        for (int lx = 0; lx < LT; ++lx)
-            if (H[lx][ly] > -1000) m |= 1ull << lx;
-        colm[ly] = m;
+    {
+        // observed masks by ballot: one row and one column per wavefront and round
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        for (int r = wv; r < LT; r += 8) {
+            bool ro = false, co = false;
+            if (lane < LT) {
+                ro = H[r][lane] > -1000;
+                co = H[lane][r] > -1000;
+            }
+            const uint64_t mr = __ballot(ro), mc = __ballot(co);
+            if (lane == 0) {
+                rowm[r] = mr;
+                colm[r] = mc;
+            }
+        }
     }
     __syncthreads();
-    const int tx = threadIdx.x / TILE, ty = threadIdx.x % TILE;
+    const int cell = threadIdx.x & 255, half = threadIdx.x >> 8;
+    const int tx = cell / TILE, ty = cell % TILE;
     const int x0 = bx + tx, y0 = by + ty;
-    if (x0 >= XY || y0 >= XY) return;
+    const bool act = x0 < XY && y0 < XY;
     const int64_t c0 = (int64_t)x0 * XY + y0;
 #define HAT(gx, gy) H[(gx) - bx + HALO][(gy) - by + HALO]
+    if (half == 1) {
+        // __make_positive_obstacle_map's density (gvom.py:530-555): occupied voxels with
+        // hit > 10 in [floor((h + pos)/zr - oz) + 1, floor((h + robot_h)/zr - oz)]; the slope
+        // test that overrides it is applied by the first half
+        int32_t pvw = 0;
+        if (act) {
+            const double h0 = HAT(x0, y0);
+            const int cur = st->parity;   // committed by k_commit_combine
+            const double oz = st->comb_origin[cur][2];
+            const double flo = floor(((h0 + g.pos_thr) / g.z_res) - oz);
+            const double fhi = floor(((h0 + g.robot_h) / g.z_res) - oz);
+            if (flo + 1 >= 0 && flo + 1 < g.Z && fhi >= 0 && fhi < g.Z) {
+                const int lo = (int)flo + 1, hi = (int)fhi;
+                const int32_t *col = comb_tables + (int64_t)cur * g.V + c0 * g.Z;
+                const CombRec *recs = comb_recs + (int64_t)cur * ccap;
+                double dens = 0.0, nn = 0.0;
+                // eight voxels per round: all table words, then all records, are in flight
+                // together; the sums still run in z order
+                for (int z0 = lo; z0 <= hi; z0 += 8) {
+                    int32_t t[8], rh[8], rt[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) t[k] = (z0 + k <= hi) ? col[z0 + k] : 0;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        rh[k] = -1;
+                        rt[k] = 0;
+                        if (t[k] < 0) {
+                            const CombRec &r = recs[(uint32_t)t[k] & MASK31];
+                            rh[k] = r.hit;
+                            rt[k] = r.total;
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k)
+                        if (rh[k] > 10) {
+                            nn += (double)rt[k];
+                            dens += (double)rh[k];
+                        }
+                }
+                if (nn > 0.0) dens /= nn;
+                pvw = (int32_t)(dens * 100.0);
+            }
+        }
+        winpv[cell] = pvw;
+        __syncthreads();
+        return;
+    }
+    // first half: lanes outside the map (partial tiles) compute on in-tile LDS only and
+    // write nothing, so every lane reaches the one barrier below
 
     // __calculate_slope (gvom.py:718-805)
     double sx = 0.0, sy = 0.0, rg = -1.0;
@@ -551,7 +637,7 @@ __global__ __launch_bounds__(256) void k_maps2d(Geo g, const DevState *__restric
     // the map are never observed, matching the reference's bounds `continue`).
     double dh_out = 0.0;
     const double h0 = HAT(x0, y0);
-    const double inf0 = inferred[c0];
+    const double inf0 = act ? inferred[c0] : -1000.0;
     if (!(h0 > -1000) && !(inf0 == -1000.0)) {
         const int lx0 = x0 - bx + HALO, ly0 = y0 - by + HALO;
         bool xpd = false, xnd = false, ypd = false, ynd = false;
@@ -593,70 +679,21 @@ __global__ __launch_bounds__(256) void k_maps2d(Geo g, const DevState *__restric
         if (dh > 0) dh_out = dh;
     }
 #undef HAT
-    slope_x[c0] = sx;
-    slope_y[c0] = sy;
-    rough[c0] = rg;
-    delta[c0] = dh_out;
-    // __make_positive_obstacle_map (gvom.py:516-555): slope test, else the density of
-    // occupied voxels with hit > 10 in [floor((h + pos)/zr - oz) + 1, floor((h + robot_h)/zr - oz)]
-    int32_t pv = 100;
-    if (!(sqrt(sx * sx + sy * sy) >= g.slope_thr)) {
-        pv = 0;
-        const int cur = 1 - st->parity;
-        const double oz = st->comb_origin[cur][2];
-        const double flo = floor(((h0 + g.pos_thr) / g.z_res) - oz);
-        const double fhi = floor(((h0 + g.robot_h) / g.z_res) - oz);
-        if (flo + 1 >= 0 && flo + 1 < g.Z && fhi >= 0 && fhi < g.Z) {
-            const int lo = (int)flo + 1, hi = (int)fhi;
-            const int32_t *col = comb_tables + (int64_t)cur * g.V + c0 * g.Z;
-            const CombRec *recs = comb_recs + (int64_t)cur * ccap;
-            double dens = 0.0, nn = 0.0;
-            // eight voxels per round: all table words, then all records, are in flight
-            // together; the sums still run in z order
-            for (int z0 = lo; z0 <= hi; z0 += 8) {
-                int32_t t[8], rh[8], rt[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) t[k] = (z0 + k <= hi) ? col[z0 + k] : 0;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    rh[k] = -1;
-                    rt[k] = 0;
-                    if (t[k] < 0) {
-                        const CombRec &r = recs[(uint32_t)t[k] & MASK31];
-                        rh[k] = r.hit;
-                        rt[k] = r.total;
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    if (rh[k] > 10) {
-                        nn += (double)rt[k];
-                        dens += (double)rh[k];
-                    }
-            }
-            if (nn > 0.0) dens /= nn;
-            pv = (int32_t)(dens * 100.0);
-        }
+    if (act) {
+        slope_x[c0] = sx;
+        slope_y[c0] = sy;
+        rough[c0] = rg;
+        delta[c0] = dh_out;
     }
+    // __make_positive_obstacle_map (gvom.py:516-555): slope test, else the window density
+    __syncthreads();
+    const int32_t pv = (sqrt(sx * sx + sy * sy) >= g.slope_thr) ? 100 : winpv[cell];
     // __make_negative_obstacle_map (506-512); __make_visability_map (445-452)
-    pos[c0] = pv;
-    neg[c0] = (dh_out > g.neg_thr) ? 100 : 0;
-    vis[c0] = (h0 > -1000) ? 1 : 0;
-}
-
-// One wavefront: combined cell count = sum of the column-scan chunk totals; flip parity.
-__global__ void k_commit_combine(DevState *st, const int32_t *__restrict__ chunk_tot, int64_t nchunk) {
-    if (blockIdx.x != 0) return;
-    const int lane = threadIdx.x & 63;
-    uint32_t tot = 0;
-    for (int64_t k = lane; k < nchunk; k += 64) tot += (uint32_t)chunk_tot[k];
-    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-    if (threadIdx.x != 0) return;
-    const int cur = 1 - st->parity;
-    st->comb_cells[cur] = (int32_t)tot;
-    st->comb_total = (int32_t)tot;
-    st->parity = cur;
-    st->have_last = 1;
+    if (act) {
+        pos[c0] = pv;
+        neg[c0] = (dh_out > g.neg_thr) ? 100 : 0;
+        vis[c0] = (h0 > -1000) ? 1 : 0;
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -754,6 +791,7 @@ hipError_t launch_comb_records(const Geo &g, DevState *st, const int32_t *slot_t
                                                                  height, inferred);
     k_comb_merge<<<2048, 256, 0, s>>>(g, st, slot_tables, slot_recs, ncap, comb_tables, comb_recs, ccap, occ_list,
                                       chunk_tot, col_chunks(g.X), height);
+    k_commit_combine<<<1, 64, 0, s>>>(st, chunk_tot, col_chunks(g.X));
     return hipGetLastError();
 }
 
@@ -762,15 +800,11 @@ hipError_t launch_maps2d(const Geo &g, const DevState *st, const double *height,
                          double *slope_y, double *rough, double *delta, int32_t *pos, int32_t *neg, int32_t *vis,
                          hipStream_t s) {
     dim3 grid((unsigned)((g.XY + TILE - 1) / TILE), (unsigned)((g.XY + TILE - 1) / TILE));
-    k_maps2d<<<grid, 256, 0, s>>>(g, st, height, inferred, comb_tables, comb_recs, ccap, slope_x, slope_y, rough,
+    k_maps2d<<<grid, 512, 0, s>>>(g, st, height, inferred, comb_tables, comb_recs, ccap, slope_x, slope_y, rough,
                                   delta, pos, neg, vis);
     return hipGetLastError();
 }
 
-hipError_t launch_commit_combine(DevState *st, const int32_t *chunk_tot, int64_t X, hipStream_t s) {
-    k_commit_combine<<<1, 64, 0, s>>>(st, chunk_tot, col_chunks(X));
-    return hipGetLastError();
-}
 
 hipError_t launch_export_table(const Geo &g, const int32_t *table, int32_t *out_ref, hipStream_t s) {
     k_export_table<<<nblocks(g.V, 256), 256, 0, s>>>(g, table, out_ref);

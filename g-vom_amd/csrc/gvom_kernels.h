@@ -33,15 +33,17 @@ hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start,
                            uint32_t *blk, int S, const Geo &g, DevState *st, hipStream_t s);
 // Per-segment moments (chunk partials, then per-segment sums) + slot table occupancy
 // marks + hit/total/min-height records.
-hipError_t launch_moments(int dtype, const void *world, const uint32_t *skeys, const uint32_t *svals,
-                          const int32_t *seg_start, const uint32_t *seg_key, int64_t seg_cap,
-                          const ScanDesc *sd, int S, const Geo &g, DevState *st, int32_t *slot_tables,
-                          SlotRec *slot_recs, int64_t ncap, Moment *part, Moment *mom, hipStream_t s);
+hipError_t launch_mom_part(int dtype, const void *world, const uint32_t *skeys, const uint32_t *svals,
+                           int64_t seg_cap, const ScanDesc *sd, int S, const Geo &g, DevState *st, Moment *part,
+                           hipStream_t s);
+hipError_t launch_mom_seg(const int32_t *seg_start, const uint32_t *seg_key, int64_t seg_cap, const Geo &g,
+                          DevState *st, int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap, const Moment *part,
+                          Moment *mom, hipStream_t s);
 // Neighbourhood mean/covariance per occupied voxel (gvom.py:1250-1387).
 hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t seg_cap, const Geo &g,
-                          const DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap,
+                          DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap, int S,
                           hipStream_t s);
-hipError_t launch_commit_process(DevState *st, int S, int B, hipStream_t s);
+
 
 // ---- combine stage (gvom_combine.hip) ----------------------------------------------------
 hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_tables,
@@ -58,7 +60,7 @@ hipError_t launch_maps2d(const Geo &g, const DevState *st, const double *height,
                          const int32_t *comb_tables, const CombRec *comb_recs, int64_t ccap, double *slope_x,
                          double *slope_y, double *rough, double *delta, int32_t *pos, int32_t *neg, int32_t *vis,
                          hipStream_t s);
-hipError_t launch_commit_combine(DevState *st, const int32_t *chunk_tot, int64_t X, hipStream_t s);
+
 
 // ---- export (gvom_combine.hip) -----------------------------------------------------------
 // Internal z-contiguous table -> reference layout and encoding.
@@ -69,3 +71,4 @@ hipError_t launch_debug_voxels(const Geo &g, const DevState *st, const int32_t *
 hipError_t launch_debug_height(const Geo &g, const DevState *st, int parity, const double *height,
                                const double *rough, const double *sx, const double *sy,
                                const double *delta, float *out7, float *out3, hipStream_t s);
+

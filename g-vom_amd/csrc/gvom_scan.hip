@@ -627,15 +627,16 @@ __global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *__restrict_
     }
 }
 
-// One block: totals of the per-block counts -> nseg, nvalid, end sentinel of the last
-// segment, per-scan [first segment, end of in-map segments) and slot cell counts.
-// Segments are ordered by scan, and within a scan in-map keys precede halo keys.
-__global__ __launch_bounds__(64) void k_seg_bounds(const uint32_t *__restrict__ blk, int nblk, int S, Geo g,
-                                                   int32_t *__restrict__ seg_start, DevState *__restrict__ st) {
+// Totals of the per-block counts -> nseg, nvalid, end sentinel of the last segment,
+// per-scan [first segment, end of in-map segments) and slot cell counts, by one block
+// (blockDim >= 64).  Segments are ordered by scan, and within a scan in-map keys precede
+// halo keys.
+__device__ void seg_bounds_body(const uint32_t *__restrict__ blk, int nblk, int S, const Geo &g,
+                                int32_t *__restrict__ seg_start, DevState *__restrict__ st) {
     __shared__ uint32_t tot[2 + 3 * GVOM_MAX_BATCH];
     const int t = threadIdx.x;
     const int W = 2 + 3 * S;
-    for (int col = t; col < W; col += 64) {
+    for (int col = t; col < W; col += blockDim.x) {
         uint32_t a = 0;
 #pragma unroll 8
         for (int b = 0; b < nblk; ++b) a += blk[(int64_t)b * W + col];
@@ -661,6 +662,11 @@ __global__ __launch_bounds__(64) void k_seg_bounds(const uint32_t *__restrict__ 
         ch += (nv + MCH - 1) / MCH;
     }
     st->chunk_off[S] = ch;
+}
+
+__global__ __launch_bounds__(64) void k_seg_bounds(const uint32_t *__restrict__ blk, int nblk, int S, Geo g,
+                                                   int32_t *__restrict__ seg_start, DevState *__restrict__ st) {
+    seg_bounds_body(blk, nblk, S, g, seg_start, st);
 }
 
 // ------------------------------------------------------------------------------------
@@ -865,11 +871,21 @@ __device__ __forceinline__ double red32(double v) {
     return v;
 }
 
+// Ring bookkeeping after the scan stage of S scans (gvom.py:211-216), by one thread.
+__device__ __forceinline__ void commit_process(DevState *st, int S, int B) {
+    int next = st->next_slot;
+    int mask = st->filled_mask;
+    for (int s = 0; s < S; ++s) mask |= 1 << ((next + s) % B);
+    st->filled_mask = mask;
+    st->last_slot = (next + S - 1) % B;
+    st->next_slot = (next + S) % B;
+}
+
 __global__ __launch_bounds__(256) void k_metrics(const uint32_t *__restrict__ seg_key,
                                                  const Moment *__restrict__ mom, Geo g,
-                                                 const DevState *__restrict__ st,
+                                                 DevState *__restrict__ st,
                                                  const int32_t *__restrict__ tables,
-                                                 SlotRec *__restrict__ recs, int64_t ncap) {
+                                                 SlotRec *__restrict__ recs, int64_t ncap, int S) {
     const int l32 = threadIdx.x & 31;
     const int32_t nseg = st->nseg;
     const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> 5;
@@ -960,14 +976,9 @@ __global__ __launch_bounds__(256) void k_metrics(const uint32_t *__restrict__ se
 }
 
 __global__ void k_commit_process(DevState *st, int S, int B) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int next = st->next_slot;
-    int mask = st->filled_mask;
-    for (int s = 0; s < S; ++s) mask |= 1 << ((next + s) % B);
-    st->filled_mask = mask;
-    st->last_slot = (next + S - 1) % B;
-    st->next_slot = (next + S) % B;
+    if (threadIdx.x == 0 && blockIdx.x == 0) commit_process(st, S, B);
 }
+
 
 // Zero the slot tables of the S slots this batch writes (slot choice read on device).
 // Each thread clears 4 consecutive words of one table.
@@ -1143,27 +1154,37 @@ hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start,
     return hipGetLastError();
 }
 
-hipError_t launch_moments(int dtype, const void *world, const uint32_t *skeys, const uint32_t *svals,
-                          const int32_t *seg_start, const uint32_t *seg_key, int64_t seg_cap,
-                          const ScanDesc *sd, int S, const Geo &g, DevState *st, int32_t *slot_tables,
-                          SlotRec *slot_recs, int64_t ncap, Moment *part, Moment *mom, hipStream_t s) {
+// Chunk partials (k_mom_part) need only the sorted keys and segments; the segment sums
+// (k_mom_seg) also read the pass counts the ray cast left in the slot tables.
+hipError_t launch_mom_part(int dtype, const void *world, const uint32_t *skeys, const uint32_t *svals,
+                           int64_t seg_cap, const ScanDesc *sd, int S, const Geo &g, DevState *st, Moment *part,
+                           hipStream_t s) {
     unsigned nbp = nblocks(((seg_cap + MCH - 1) / MCH + S) * 64, 256);
     if (dtype == GVOM_F32)
         k_mom_part<float><<<nbp, 256, 0, s>>>((const float *)world, skeys, svals, sd, S, g, st, part);
     else
         k_mom_part<double><<<nbp, 256, 0, s>>>((const double *)world, skeys, svals, sd, S, g, st, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_mom_seg(const int32_t *seg_start, const uint32_t *seg_key, int64_t seg_cap, const Geo &g,
+                          DevState *st, int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap, const Moment *part,
+                          Moment *mom, hipStream_t s) {
     k_mom_seg<<<nblocks(seg_cap, 256), 256, 0, s>>>(seg_start, seg_key, part, g, st, slot_tables, slot_recs, ncap,
                                                     mom);
     return hipGetLastError();
 }
 
 hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t seg_cap, const Geo &g,
-                          const DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap,
+                          DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap, int S,
                           hipStream_t s) {
-    // grid-stride: enough 32-lane groups to fill the chip, never more than segments can need
+    // grid-stride: enough 32-lane groups to fill the chip, never more than segments can need;
+    // then the ring bookkeeping (one thread)
     int64_t nb = nblocks(seg_cap * 32, 256);
     if (nb > 2048) nb = 2048;
-    k_metrics<<<(unsigned)nb, 256, 0, s>>>(seg_key, mom, g, st, slot_tables, slot_recs, ncap);
+    if (nb < 1) nb = 1;
+    k_metrics<<<(unsigned)nb, 256, 0, s>>>(seg_key, mom, g, st, slot_tables, slot_recs, ncap, S);
+    k_commit_process<<<1, 64, 0, s>>>(st, S, g.B);
     return hipGetLastError();
 }
 
@@ -1173,7 +1194,3 @@ hipError_t launch_clear_slots(int32_t *slot_tables, const DevState *st, int S, c
     return hipGetLastError();
 }
 
-hipError_t launch_commit_process(DevState *st, int S, int B, hipStream_t s) {
-    k_commit_process<<<1, 64, 0, s>>>(st, S, B);
-    return hipGetLastError();
-}

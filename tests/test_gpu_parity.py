@@ -95,3 +95,39 @@ def test_empty_and_degenerate_scans():
              [synth.ScanInput(far, s.ego, s.transform, 0)],
              [synth.ScanInput(s.points, s.ego + np.array([40.0, 0.0, 0.0]), s.transform, 0)]]
     _run_and_compare(tuple(args), steps, "edge")
+
+
+@pytest.mark.parametrize("variant", [(256, 1, 0), (128, 1, 4)])
+def test_odd_map_sizes(variant):
+    """A map whose sides are not multiples of the kernels' tiles (37 x 37 columns, 13 z
+    levels, buffer 3): partial 16 x 16 column tiles in the 2D pass, partial 4 x 4 ray-cast
+    tiles, z columns shorter than a wavefront.  Three frames of the small scan, both ray
+    casts."""
+    import gvom
+    wl = synth.workload(1)
+    args = (0.8, 0.4, 37, 13, 3) + tuple(wl.map_args[5:])
+    steps = [[synth.ScanInput(wl.steps[0][0].points, wl.steps[0][0].ego + np.array([0.9 * k, -0.4 * k, 0.0]),
+                              wl.steps[0][0].transform, k)] for k in range(3)]
+    g = gvom.Gvom(*args)
+    g.set_raycast_config(*variant[:2], False, False, bool(variant[2] & 4))
+    o = OracleGvom(*args)
+    for k, step in enumerate(steps):
+        for s in step:
+            g.Process_pointcloud(s.points, s.ego, s.transform)
+            o.Process_pointcloud(s.points, s.ego, s.transform)
+            slot = o.last_buffer_index
+            compare_scan(g.slot_state(slot), o.slots[slot], f"odd {variant} step {k} slot {slot}")
+        out_g = g.combine_maps()
+        out_o = o.combine_maps()
+        compare_combined(g.combined_state(), o.combined, f"odd {variant} step {k} combined")
+        compare_maps(g.maps_state(), o.combined, f"odd {variant} step {k}")
+        for name, a, b in zip(("pos", "neg", "vis"), (out_g[1], out_g[2], out_g[4]), (out_o[1], out_o[2], out_o[4])):
+            assert_equal(f"odd {variant} step {k} {name}", a, b)
+        # OccupancyGrid payloads on a side that is not a multiple of 4 (byte stores)
+        from oracle import ros_ref
+        ms = g.maps_state()
+        grids = g.occupancy_grids()
+        exp = ros_ref.occupancy_payloads(ms["pos"], ms["neg"], ms["rough"], ms["vis"])
+        for name, v in exp.items():
+            assert_equal(f"odd {variant} step {k} {name}", grids[name], v)
+        print(f"odd map {variant} step {k}: cells {len(o.combined.hit)} ok", flush=True)
