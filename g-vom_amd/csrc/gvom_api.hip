@@ -83,7 +83,7 @@ struct gvom_handle {
     double slot_origin[GVOM_MAX_BUFFER][3] = {};
     double latest_ego[3] = {0, 0, 0};
 
-    // Stage timing: one set of 9 events per process call (0..4) and combine call (5..8);
+    // Stage timing: one set of 10 events per process call (0..4, 9) and combine call (5..8);
     // gvom_stage_times averages every stage over all sets recorded since timing was enabled.
     // differentiable form: buffers (allocated on first use) and the forward's context
     void *diff_rec = nullptr;
@@ -335,9 +335,9 @@ static void host_origin(const Geo &g, const double ego[3], double o[3]) {
     o[2] = std::floor((ego[2] / g.z_res) - (double)g.Z / 2.0);
 }
 
-// Timing events: process records 0..4 of a new set, combine records 5..8 of the current set
-// (or of a new one when that set already has a combine).
-static constexpr int EV_PER_SET = 9;
+// Timing events: process records 0..4 and 9 (end of the ray-cast kernel) of a new set,
+// combine records 5..8 of the current set (or of a new one when that set already has a combine).
+static constexpr int EV_PER_SET = 10;
 static constexpr int MAX_SETS = 8192;
 
 static void stage_event(gvom_handle *h, int k) {
@@ -502,6 +502,7 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     CKH(h, launch_raycast(dtype, h->world, h->valid, sd, S, NT, g, h->st, tiled ? h->tscratch : h->slot_tables,
                           h->ray_block, h->ray_rpt, tiled ? 0 : h->ray_box, tiled ? 1 : 0, -1,
                           xcd ? h->rorder : nullptr, st));
+    stage_event(h, 9);
     if (tiled) CKH(h, launch_tile_convert(g, h->st, S, -1, h->tscratch, h->slot_tables, st));
     stage_event(h, 2);
     // join: the segment sums read the pass counts and the partials
@@ -509,7 +510,7 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     stage_event(h, 3);
     CKH(h, launch_mom_seg(h->seg_start, h->seg_key, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, h->part,
                           h->mom, st));
-    // k_metrics' last block also advances the ring on the device
+    // launch_metrics also launches the one-wave commit that advances the ring on the device
     CKH(h, launch_metrics(h->seg_key, h->mom, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, S, st));
     stage_event(h, 4);
 
@@ -931,10 +932,10 @@ extern "C" int gvom_stage_times(gvom_handle *h, float *ms, int32_t n) {
     if (!h || !ms) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
     CKH(h, hipStreamSynchronize(h->stream));
-    static const int pairs[7][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {5, 6}, {6, 7}, {7, 8}};
+    static const int pairs[9][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {5, 6}, {6, 7}, {7, 8}, {1, 9}, {9, 2}};
     for (int k = 0; k < n; ++k) {
         ms[k] = 0.f;
-        if (k >= 7) continue;
+        if (k >= 9) continue;
         double sum = 0.0;
         int cnt = 0;
         for (size_t q = 0; q < h->set_used.size(); ++q) {

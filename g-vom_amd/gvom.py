@@ -370,8 +370,8 @@ class Gvom:
         nat.check(self._lib.gvom_set_timing(self._h, 1 if enabled else 0), self._h)
 
     def stage_times(self):
-        ms = (ctypes.c_float * 7)()
-        nat.check(self._lib.gvom_stage_times(self._h, ms, 7), self._h)
+        ms = (ctypes.c_float * 9)()
+        nat.check(self._lib.gvom_stage_times(self._h, ms, 9), self._h)
         return list(ms)
 
     def sync(self):

@@ -218,7 +218,9 @@ int gvom_ring_state(const gvom_handle *h, int32_t *buffer_index, int32_t *last_b
  * (gvom_set_timing resets the record).  Stage order: 0 clear+prepare, 1 ray cast,
  * 2 the part of the sort / segments / chunk-partial chain (run on a second stream,
  * concurrently with the ray cast) that is not hidden under it, 3 segment sums+metrics,
- * 4 combine tables, 5 column scan+records+columns, 6 2D maps.  n = entries available in ms.  (The reference has commented-out
+ * 4 combine tables, 5 column scan+records+columns, 6 2D maps; then the two parts of stage 1:
+ * 7 the ray-cast kernel alone, 8 the tiled scratch's conversion into the slot table (0 when the
+ * direct-table cast runs).  n = entries available in ms (at most 9 are filled).  (The reference has commented-out
  * wall-clock prints at these points, gvom.py:107-217, 224, 321-322, 390.) */
 int gvom_set_timing(gvom_handle *h, int32_t enabled);
 int gvom_stage_times(gvom_handle *h, float *ms, int32_t n);
