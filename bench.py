@@ -8,9 +8,11 @@ with a ring buffer of 8 scans.  One step = one frame = Process_pointcloud of one
 cycled) are resident in HBM before the timed region; the maps stay on the device.
 
 `value` is the frame rate of that sequence replayed from a captured hipGraph (8 frames
-per launch); `plain_eager` is the same frames issued call by call.  The eager pass
-records HIP events around every stage on the library's stream; the dominant kernel's
-achieved HBM bandwidth is its algorithmic bytes per launch over its average duration.
+per launch), with each frame's scan stage allowed to start while the previous frame's
+combine runs (scan/combine overlap; --no-overlap turns it off); `plain_eager` is the same
+frames issued call by call.  The eager pass without overlap records HIP events around
+every stage; the dominant kernel's achieved HBM bandwidth is its algorithmic bytes per
+launch over its average duration.
 
 Extra legs (on by default, --quick skips them):
   batched   3 sensors (131,072 / 65,536 / 16,384 points, 10-40 % dropout) x 8 steps,
@@ -107,7 +109,7 @@ def graph_rate(g, dev, scans, reps, warm=2):
     return reps * len(scans) / (time.perf_counter() - t0)
 
 
-def batched_leg(reps):
+def batched_leg(reps, overlap):
     import gvom
     import gvom_native as nat
     import gvom_synth
@@ -122,6 +124,7 @@ def batched_leg(reps):
     n_pts = sum(f.shape[0] for f in flat)
     g = gvom.Gvom(*wl.map_args)
     g.reserve(max(s.points.shape[0] for s in scans), max(f.shape[0] for f in flat))
+    g.set_overlap(overlap)
     g.graph_begin(len(scans))
     for k in range(len(wl.steps)):
         g.process_batch_device(dev[k], offs[k], np.stack([s.ego for s in wl.steps[k]]),
@@ -138,6 +141,7 @@ def batched_leg(reps):
     t_b = (time.perf_counter() - t0) / reps
     p = gvom.Gvom(*wl.map_args)
     p.reserve(max(s.points.shape[0] for s in scans))
+    p.set_overlap(overlap)
     for k, s in enumerate(scans[:8]):
         p.process_pointcloud_device(sdev[k], s.points.shape[0], s.ego, s.transform)
         p.combine_maps_device()
@@ -299,6 +303,8 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline only (no graph, batched or diff legs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="keep each frame's scan stage behind the previous combine (no second stream)")
     ap.add_argument("--ray-config", default="auto",
                     help="'auto' (autotune on the first scan) or 'block,rays_per_thread,flags' "
                          "(flags: bit 0 LDS near-sensor box, bit 1 azimuth-sector ray order, bit 2 "
@@ -351,7 +357,19 @@ def main():
     g.set_timing(False)
 
     eager_fps = args.steps / el
-    fps = eager_fps
+    # the stage times above come from serial frames; from here on the scan stage of frame
+    # k + 1 may start while the combine of frame k runs (tiled ray cast only)
+    overlap = (not args.no_overlap) and bool(ray_cfg["flags"] & 4)
+    g.set_overlap(overlap)
+    eager_ovl_fps = None
+    if overlap:
+        g.sync()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            frame(k)
+        g.sync()
+        eager_ovl_fps = args.steps / (time.perf_counter() - t0)
+    fps = eager_ovl_fps or eager_fps
     if not args.quick:
         fps = graph_rate(g, dev, scans, max(1, args.steps // len(scans)))
     ms = 1e3 / fps
@@ -390,9 +408,11 @@ def main():
         "config": {"workload": "plain frame: Process_pointcloud (131072-pt 128x1024 f32 scan, f32 4x4 transform) "
                                "+ combine_maps, 256x256x64 voxels @0.4/0.2 m, buffer 8"
                                + ("" if args.quick else "; 8 frames per hipGraph launch"),
-                   "points_per_scan": N, "grid": [256, 256, 64], "buffer": 8, "frames_cycled": len(scans)},
+                   "points_per_scan": N, "grid": [256, 256, 64], "buffer": 8, "frames_cycled": len(scans),
+                   "overlap": overlap},
         "points_per_s": round(fps * N, 1),
-        "plain_eager": {"frames_per_s": round(eager_fps, 2), "ms_per_frame": round(1e3 / eager_fps, 4)},
+        "plain_eager": {"frames_per_s": round(eager_fps, 2), "ms_per_frame": round(1e3 / eager_fps, 4),
+                        "overlapped_frames_per_s": round(eager_ovl_fps, 2) if eager_ovl_fps else None},
         "stages_ms": {k: round(v, 5) for k, v in zip(STAGES, stages)},
         "scan": {"cells": ss["cells"], "ray_steps": ss["steps"], "combined_cells": C},
         "raycast_config": ray_cfg,
@@ -405,7 +425,7 @@ def main():
     if not args.quick:
         out["host_api"] = host_api_leg(wl, scans, max(8, args.steps // 4))
         out["atomics"] = atomics_leg(g, ss["steps"], stages[7])
-        out["batched"] = batched_leg(max(1, args.steps // 24))
+        out["batched"] = batched_leg(max(1, args.steps // 24), not args.no_overlap)
         out["fwd_bwd"] = diff_leg(max(5, args.steps // 8))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)

@@ -64,6 +64,7 @@ struct gvom_handle {
     double *height = nullptr, *inferred = nullptr, *slope_x = nullptr, *slope_y = nullptr, *rough = nullptr,
            *delta = nullptr;
     int32_t *pos = nullptr, *neg = nullptr, *vis = nullptr, *col_count = nullptr;
+    uint64_t *occ_bits = nullptr;   // combined map: per column, bit z of word z / 64 = occupied
     uint2 *occ_list = nullptr;
     int32_t *export_buf = nullptr;
     int32_t *chunk_tot = nullptr;
@@ -107,16 +108,23 @@ struct gvom_handle {
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> set_used;   // bit k set when event k of that set was recorded
     int cur_set = -1;
-    int ray_block = 256, ray_rpt = 1, ray_box = 0, ray_xcd = 0;
+    int ray_block = 128, ray_rpt = 1, ray_box = 0, ray_xcd = 0;
     uint8_t *rsec = nullptr;
     uint32_t *rorder = nullptr;
     int8_t *grids = nullptr;   // OccupancyGrid payloads staged for host copies (5 * X bytes)
     uint8_t *raw = nullptr;    // PointCloud2 bytes staged from the host
     int64_t rawcap = 0;
     // tiled ray-cast scratch: min(B, 32) zeroed tables in the 4 x 4 x 1 segment layout
-    int ray_tiled = 0;
+    int ray_tiled = 1;
     int32_t *tscratch = nullptr;
     int tscratch_scans = 0;
+    // Overlap of a device-pointer scan stage with the preceding combine (gvom_set_overlap):
+    // the stage's first part (transform, keys, ray cast into the tiled scratch, sort chain)
+    // runs on `front`, which waits only for ev_back; ev_back is recorded on the main stream
+    // after every call that uses the scan stage's work buffers, but not after a combine.
+    int overlap = 0;
+    hipStream_t front = nullptr;
+    hipEvent_t ev_front = nullptr, ev_back = nullptr;
 };
 
 #define CKH(h, expr)                                                                 \
@@ -133,14 +141,18 @@ static void free_all(gvom_handle *h) {
                     h->slope_x, h->slope_y, h->rough, h->delta, h->pos, h->neg, h->vis, h->occ_list,
                     h->col_count, h->chunk_tot, h->export_buf, h->pts_in, h->world, h->valid, h->k0, h->v0, h->k1, h->v1,
                     h->hist, h->blk, h->seg_key, h->seg_start, h->mom, h->part, h->sd, h->diff_rec,
-                    h->diff_colh, h->diff_part, h->diff_sums, h->diff_colpt, h->diff_pmark, h->graph_sd, h->rsec, h->rorder, h->grids, h->raw, h->tscratch};
+                    h->diff_colh, h->diff_part, h->diff_sums, h->diff_colpt, h->diff_pmark, h->graph_sd, h->rsec, h->rorder, h->grids, h->raw, h->tscratch,
+                    h->occ_bits};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : h->ev_pool)
         if (e) (void)hipEventDestroy(e);
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->ev_front) (void)hipEventDestroy(h->ev_front);
+    if (h->ev_back) (void)hipEventDestroy(h->ev_back);
     if (h->side) (void)hipStreamDestroy(h->side);
+    if (h->front) (void)hipStreamDestroy(h->front);
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
@@ -213,6 +225,10 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     CKC(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
     CKC(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
     CKC(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    CKC(hipStreamCreateWithFlags(&h->front, hipStreamNonBlocking));
+    CKC(hipEventCreateWithFlags(&h->ev_front, hipEventDisableTiming));
+    CKC(hipEventCreateWithFlags(&h->ev_back, hipEventDisableTiming));
+    CKC(hipEventRecord(h->ev_back, h->stream));
     CKC(dalloc(&h->st, 1));
     CKC(hipMemset(h->st, 0, sizeof(DevState)));
     CKC(dalloc(&h->slot_tables, V * g.B));
@@ -232,6 +248,7 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     CKC(dalloc(&h->vis, X));
     CKC(dalloc(&h->occ_list, V));
     CKC(dalloc(&h->col_count, X + 16));
+    CKC(dalloc(&h->occ_bits, X * ((g.Z + 63) / 64)));
     CKC(dalloc(&h->chunk_tot, col_chunks(X) + 1));
     CKC(dalloc(&h->sd, GVOM_MAX_BATCH));
 #undef CKC
@@ -244,6 +261,8 @@ extern "C" void gvom_destroy(gvom_handle *h) {
     {
         std::lock_guard<std::mutex> lk(h->mu);
         if (h->stream) (void)hipStreamSynchronize(h->stream);
+        if (h->front) (void)hipStreamSynchronize(h->front);
+        if (h->side) (void)hipStreamSynchronize(h->side);
         free_all(h);
     }
     delete h;
@@ -340,7 +359,7 @@ static void host_origin(const Geo &g, const double ego[3], double o[3]) {
 static constexpr int EV_PER_SET = 10;
 static constexpr int MAX_SETS = 8192;
 
-static void stage_event(gvom_handle *h, int k) {
+static void stage_event(gvom_handle *h, int k, hipStream_t on = nullptr) {
     if (!h->timing) return;
     if (k == 0 || (k == 5 && (h->cur_set < 0 || (h->set_used[h->cur_set] >> 5 & 1)))) {
         if ((int)h->set_used.size() >= MAX_SETS) return;
@@ -353,12 +372,16 @@ static void stage_event(gvom_handle *h, int k) {
         }
     }
     if (h->cur_set < 0) return;
-    if (hipEventRecord(h->ev_pool[EV_PER_SET * h->cur_set + k], h->stream) == hipSuccess)
+    if (hipEventRecord(h->ev_pool[EV_PER_SET * h->cur_set + k], on ? on : h->stream) == hipSuccess)
         h->set_used[h->cur_set] |= 1 << k;
 }
 
 // Tiled ray-cast scratch: one zeroed table per scan a batch can hold (min(B, 32)); the
 // conversion kernel leaves it zeroed after every use.
+// Everything enqueued on the main stream so far may use the scan stage's work buffers: the
+// next overlapped scan stage waits for it.
+static hipError_t mark_back(gvom_handle *h) { return hipEventRecord(h->ev_back, h->stream); }
+
 static int ensure_tile_scratch(gvom_handle *h) {
     if (h->tscratch) return GVOM_OK;
     const Geo &g = h->g;
@@ -374,13 +397,14 @@ static int ensure_tile_scratch(gvom_handle *h) {
     }
     CKH(h, dalloc(&h->tscratch, words * n));
     CKH(h, hipMemsetAsync(h->tscratch, 0, (size_t)words * n * sizeof(int32_t), h->stream));
+    CKH(h, mark_back(h));
     h->tscratch_scans = n;
     return GVOM_OK;
 }
 
 // The scan stage for S scans (gvom.py:105-220 for each).
 static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets, int S, int dtype,
-                        const double *ego, const void *T, int t_dtype, bool on_device) {
+                        const double *ego, const void *T, int t_dtype, bool on_device, bool may_overlap) {
     const Geo &g = h->g;
     h->diff_n = -1;   // the work buffers a pending gvom_diff_backward reads are reused below
     if (S < 1 || S > GVOM_MAX_BATCH || S > g.B) {
@@ -458,6 +482,12 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
         if (rc_t) return rc_t;
     }
     hipStream_t st = h->stream;
+    // The first part of the stage (up to the ray cast into the tiled scratch) touches only
+    // work buffers, the scratch and the caller's points; with overlap on it runs on `front`
+    // and may proceed while a combine enqueued before this call still runs on st.
+    const bool ovl = h->overlap && tiled && on_device && may_overlap;
+    hipStream_t fs = ovl ? h->front : st;
+    if (ovl) CKH(h, hipStreamWaitEvent(fs, h->ev_back, 0));
     const size_t esz = dtype == GVOM_F32 ? 4 : 8;
     const void *pts = xyz;
     ScanDesc *sd = h->sd;
@@ -477,20 +507,22 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
         h->cap_ops.push_back(S);
     } else {
         if (!on_device && NT > 0) {
-            CKH(h, hipMemcpyAsync(h->pts_in, xyz, (size_t)NT * 3 * esz, hipMemcpyHostToDevice, st));
+            CKH(h, hipMemcpyAsync(h->pts_in, xyz, (size_t)NT * 3 * esz, hipMemcpyHostToDevice, fs));
             pts = h->pts_in;
         }
-        CKH(h, hipMemcpyAsync(h->sd, sdh, sizeof(ScanDesc) * S, hipMemcpyHostToDevice, st));
+        CKH(h, hipMemcpyAsync(h->sd, sdh, sizeof(ScanDesc) * S, hipMemcpyHostToDevice, fs));
     }
-    stage_event(h, 0);
+    stage_event(h, 0, fs);
     const bool xcd = h->ray_xcd != 0;
-    // the tiled path's conversion writes every word of the slot tables
-    if (!tiled || xcd) CKH(h, launch_clear_slots(h->slot_tables, h->st, S, g, st));
+    // the tiled path's conversion writes every word of the slot tables; the direct cast
+    // counts into cleared tables
+    if (!tiled) CKH(h, launch_clear_slots(h->slot_tables, h->st, S, g, st));
+    if (xcd) CKH(h, launch_fill_i32(&h->st->sec_count[0], 0, 16, fs));   // sec_count and sec_cursor
     CKH(h, launch_prepare(dtype, pts, h->world, h->k0, h->v0, h->valid, sd, S, NT, g, h->st,
-                          xcd ? h->rsec : nullptr, st));
-    if (xcd) CKH(h, launch_ray_order(h->rsec, NT, h->st, h->rorder, st));
+                          xcd ? h->rsec : nullptr, fs));
+    if (xcd) CKH(h, launch_ray_order(h->rsec, NT, h->st, h->rorder, fs));
     // fork: keys, segments and chunk partials on the side stream, concurrent with the cast
-    CKH(h, hipEventRecord(h->ev_fork, st));
+    CKH(h, hipEventRecord(h->ev_fork, fs));
     CKH(h, hipStreamWaitEvent(h->side, h->ev_fork, 0));
     uint32_t *sk = h->k0, *sv = h->v0;
     CKH(h, radix_sort(h->k0, h->v0, h->k1, h->v1, NT, g.VB + sbits, h->hist, &sk, &sv, h->side));
@@ -498,11 +530,16 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     CKH(h, launch_segments(sk, NT, h->seg_start, h->seg_key, h->blk, S, g, h->st, h->side));
     CKH(h, launch_mom_part(dtype, h->world, sk, sv, NT, sd, S, g, h->st, h->part, h->side));
     CKH(h, hipEventRecord(h->ev_join, h->side));
-    stage_event(h, 1);
+    stage_event(h, 1, fs);
     CKH(h, launch_raycast(dtype, h->world, h->valid, sd, S, NT, g, h->st, tiled ? h->tscratch : h->slot_tables,
                           h->ray_block, h->ray_rpt, tiled ? 0 : h->ray_box, tiled ? 1 : 0, -1,
-                          xcd ? h->rorder : nullptr, st));
-    stage_event(h, 9);
+                          xcd ? h->rorder : nullptr, fs));
+    stage_event(h, 9, fs);
+    if (ovl) {
+        // back on the main stream, after the combine: the conversion writes the slot table
+        CKH(h, hipEventRecord(h->ev_front, fs));
+        CKH(h, hipStreamWaitEvent(st, h->ev_front, 0));
+    }
     if (tiled) CKH(h, launch_tile_convert(g, h->st, S, -1, h->tscratch, h->slot_tables, st));
     stage_event(h, 2);
     // join: the segment sums read the pass counts and the partials
@@ -511,8 +548,9 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     CKH(h, launch_mom_seg(h->seg_start, h->seg_key, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, h->part,
                           h->mom, st));
     // launch_metrics also launches the one-wave commit that advances the ring on the device
-    CKH(h, launch_metrics(h->seg_key, h->mom, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, S, st));
+    CKH(h, launch_metrics(h->seg_key, h->mom, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, sd, S, st));
     stage_event(h, 4);
+    CKH(h, mark_back(h));
 
     for (int s = 0; s < S; ++s) {
         int slot = (h->next_slot + s) % g.B;
@@ -532,7 +570,7 @@ extern "C" int gvom_process_pointcloud(gvom_handle *h, const void *xyz, int64_t 
     if (!h || !ego || n < 0) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
     int64_t off[2] = {0, n};
-    return process_impl(h, xyz, off, 1, dtype, ego, T, t_dtype, false);
+    return process_impl(h, xyz, off, 1, dtype, ego, T, t_dtype, false, false);
 }
 
 extern "C" int gvom_process_pointcloud_device(gvom_handle *h, const void *xyz, int64_t n, int32_t dtype,
@@ -540,7 +578,7 @@ extern "C" int gvom_process_pointcloud_device(gvom_handle *h, const void *xyz, i
     if (!h || !ego || n < 0) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
     int64_t off[2] = {0, n};
-    return process_impl(h, xyz, off, 1, dtype, ego, T, t_dtype, true);
+    return process_impl(h, xyz, off, 1, dtype, ego, T, t_dtype, true, true);
 }
 
 extern "C" int gvom_process_batch(gvom_handle *h, const void *xyz, const int64_t *offsets, int32_t n_scans,
@@ -548,7 +586,7 @@ extern "C" int gvom_process_batch(gvom_handle *h, const void *xyz, const int64_t
                                   int32_t on_device) {
     if (!h || !offsets || !ego) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
-    return process_impl(h, xyz, offsets, n_scans, dtype, ego, T, t_dtype, on_device != 0);
+    return process_impl(h, xyz, offsets, n_scans, dtype, ego, T, t_dtype, on_device != 0, on_device != 0);
 }
 
 // PointCloud2 input (gvom_ros.py:108): gather x, y, z into float64 points, then the
@@ -606,7 +644,7 @@ extern "C" int gvom_process_pointcloud2(gvom_handle *h, const void *data, const 
     CKH(h, launch_cloud_gather(src, n, L->width, L->point_step, L->row_step, L->offset, L->datatype,
                                L->is_bigendian ? 1 : 0, xyz, st));
     int64_t off[2] = {0, n};
-    return process_impl(h, xyz, off, 1, GVOM_F64, ego, T, t_dtype, true);
+    return process_impl(h, xyz, off, 1, GVOM_F64, ego, T, t_dtype, true, false);
 }
 
 // The combine stage (gvom.py:222-393).
@@ -617,11 +655,12 @@ static int combine_impl(gvom_handle *h) {
     if (h->capturing) h->cap_ops.push_back(-1);
     const int cur = 1 - h->parity;
     stage_event(h, 5);
-    CKH(h, launch_comb_tables(g, h->st, h->slot_tables, h->comb_tables, h->col_count, st));
+    CKH(h, launch_comb_tables(g, h->st, h->slot_tables, h->comb_tables, h->col_count, h->occ_bits, h->height,
+                              h->inferred, st));
     stage_event(h, 6);
     CKH(h, launch_scan_columns(h->col_count, g.X, h->chunk_tot, h->st, st));
     CKH(h, launch_comb_records(g, h->st, h->slot_tables, h->slot_recs, h->ncap, h->comb_tables, h->comb_recs,
-                               h->ccap, h->col_count, h->chunk_tot, h->occ_list, h->height, h->inferred, st));
+                               h->ccap, h->col_count, h->chunk_tot, h->occ_bits, h->occ_list, h->height, st));
     stage_event(h, 7);
     CKH(h, launch_maps2d(g, h->st, h->height, h->inferred, h->comb_tables, h->comb_recs, h->ccap, h->slope_x,
                          h->slope_y, h->rough, h->delta, h->pos, h->neg, h->vis, st));
@@ -959,7 +998,7 @@ extern "C" int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, 
     if (!h || !T || !ego || n < 0) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
     int64_t off[2] = {0, n};
-    int rc = process_impl(h, points, off, 1, GVOM_F64, ego, T, GVOM_F64, true);
+    int rc = process_impl(h, points, off, 1, GVOM_F64, ego, T, GVOM_F64, true, false);
     if (rc) return rc;
     const Geo &g = h->g;
     const int64_t X = g.X;
@@ -1042,6 +1081,10 @@ extern "C" int gvom_graph_begin(gvom_handle *h, int32_t max_scans) {
         h->err = "disable timing before graph capture";
         return GVOM_ESTATE;
     }
+    if (h->ray_tiled) {
+        const int rc_t = ensure_tile_scratch(h);
+        if (rc_t) return rc_t;
+    }
     if (h->graph_sd_cap < max_scans) {
         if (h->graph_sd) CKH(h, hipFree(h->graph_sd));
         CKH(h, dalloc(&h->graph_sd, max_scans));
@@ -1057,6 +1100,8 @@ extern "C" int gvom_graph_begin(gvom_handle *h, int32_t max_scans) {
     h->mirror_have = h->have_last;
     CKH(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     h->capturing = true;
+    // the first overlapped scan stage of the sequence forks from here
+    CKH(h, mark_back(h));
     return GVOM_OK;
 }
 
@@ -1077,6 +1122,9 @@ extern "C" int gvom_graph_end(gvom_handle *h, gvom_graph **out) {
     h->have_last = h->mirror_have;
     gvom_graph *gr = new gvom_graph();
     hipError_t e = hipStreamEndCapture(h->stream, &gr->graph);
+    // ev_back was last recorded inside the capture: record it again outside
+    const hipError_t em = mark_back(h);
+    if (e == hipSuccess) e = em;
     if (e == hipSuccess) e = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
     if (e != hipSuccess) {
         h->err = std::string("graph capture failed: ") + hipGetErrorString(e);
@@ -1128,6 +1176,7 @@ extern "C" int gvom_graph_launch(gvom_handle *h, gvom_graph *gr, const double *e
     }
     if (n > 0) CKH(h, hipMemcpyAsync(h->graph_sd, sd.data(), sizeof(ScanDesc) * n, hipMemcpyHostToDevice, h->stream));
     CKH(h, hipGraphLaunch(gr->exec, h->stream));
+    CKH(h, mark_back(h));
     // host mirror of the ring, as the captured calls would have left it
     int k = 0;
     for (int op : gr->ops) {
@@ -1223,7 +1272,8 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
                         ok = ok && launch_raycast(dtype, h->world, h->valid, h->sd, 1, n, g, h->st,
                                                   itile ? h->tscratch : scratch, blocks[ib], rpts[ir], ibox, itile, 0,
                                                   iord ? h->rorder : nullptr, st) == hipSuccess;
-                        if (ok && itile) ok = launch_tile_convert(g, h->st, 1, 0, h->tscratch, scratch, st) == hipSuccess;
+                        if (ok && itile)
+                            ok = launch_tile_convert(g, h->st, 1, 0, h->tscratch, scratch, st) == hipSuccess;
                         ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
                              hipEventElapsedTime(&t, e0, e1) == hipSuccess;
                         if (!ok) {
@@ -1266,6 +1316,17 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
         if (best_ms) *best_ms = best;
     }
     return out;
+}
+
+extern "C" int gvom_set_overlap(gvom_handle *h, int32_t enabled) {
+    if (!h) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (h->capturing) {
+        h->err = "cannot change overlap while capturing";
+        return GVOM_ESTATE;
+    }
+    h->overlap = enabled ? 1 : 0;
+    return GVOM_OK;
 }
 
 extern "C" int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t rays_per_thread, int32_t flags) {

@@ -68,7 +68,10 @@ __device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_read
 __global__ __launch_bounds__(256) void k_comb_tables(Geo g, const DevState *__restrict__ st,
                                                      const int32_t *__restrict__ slot_tables,
                                                      int32_t *__restrict__ comb_tables,
-                                                     int32_t *__restrict__ col_count) {
+                                                     int32_t *__restrict__ col_count,
+                                                     uint64_t *__restrict__ occ_bits,
+                                                     double *__restrict__ height,
+                                                     double *__restrict__ inferred) {
     // Everything but the z index is uniform across the wavefront: column coordinates,
     // each source's shift and column base live in scalar registers, so a source word costs
     // one vector compare and one load (SGPR base + lane offset).  All offsets fit in 32 bits
@@ -94,6 +97,8 @@ __global__ __launch_bounds__(256) void k_comb_tables(Geo g, const DevState *__re
     }
     const int32_t *ltab = comb_tables + (int64_t)parity * V;
     uint32_t count[CPW] = {0, 0, 0, 0};
+    int first_free[CPW] = {-1, -1, -1, -1};
+    const int NW = (Z + 63) >> 6;   // occupancy words per column
     for (int z0 = 0; z0 < Z; z0 += 64) {
         const int z = z0 + lane;
         const bool zin = z < Z;
@@ -158,16 +163,40 @@ __global__ __launch_bounds__(256) void k_comb_tables(Geo g, const DevState *__re
                     p += (uint32_t)vl[j];
                 }
             }
-            if (zin && c0 + j < X) out[j * Z + z] = m ? (int32_t)(GVOM_OCC | m) : (int32_t)p;
-            count[j] += __popcll(__ballot(m != 0));
+            const int32_t o = m ? (int32_t)(GVOM_OCC | m) : (int32_t)p;
+            if (zin && c0 + j < X) out[j * Z + z] = o;
+            const uint64_t ob = __ballot(m != 0);
+            count[j] += __popcll(ob);
+            if (lane == 0 && c0 + j < X) occ_bits[(int64_t)(c0 + j) * NW + (z0 >> 6)] = ob;
+            // lowest free voxel with at least one pass (gvom.py:580-590)
+            const uint64_t fb = __ballot(zin && m == 0 && p >= 1u);
+            if (first_free[j] < 0 && fb) first_free[j] = z0 + __builtin_ctzll(fb);
         }
     }
     if (lane < CPW && c0 + lane < X) {
         uint32_t cnt = count[0];
+        int ff = first_free[0];
 #pragma unroll
         for (int j = 1; j < CPW; ++j)
-            if (lane == j) cnt = count[j];
-        col_count[c0 + lane] = (int32_t)cnt;
+            if (lane == j) {
+                cnt = count[j];
+                ff = first_free[j];
+            }
+        const int32_t c = c0 + lane;
+        col_count[c] = (int32_t)cnt;
+        inferred[c] = ff >= 0 ? ((double)ff + co[2]) * g.z_res : -1000.0;
+        if (cnt == 0) {
+            // no occupied voxel: the robot-radius default around the latest ego, measured to
+            // the cell corner (gvom.py:561-570), else unknown (-1000); occupied columns get
+            // their height from the record merge
+            const int32_t x = c / XY, y = c - (c / XY) * XY;
+            const double *ego = st->latest_ego;
+            double h = -1000.0;
+            const double xp = ((co[0] + (double)x) * g.xy_res) - ego[0];
+            const double yp = ((co[1] + (double)y) * g.xy_res) - ego[1];
+            if (xp * xp + yp * yp <= g.robot_r * g.robot_r) h = ego[2] - g.g2l;
+            height[c] = h;
+        }
     }
 }
 
@@ -293,73 +322,43 @@ __global__ __launch_bounds__(256) void k_comb_ids(Geo g, const DevState *__restr
                                                   int32_t *__restrict__ comb_tables,
                                                   const int32_t *__restrict__ col_base,
                                                   const int32_t *__restrict__ chunk_tot,
-                                                  uint2 *__restrict__ occ_list, double *__restrict__ height,
-                                                  double *__restrict__ inferred) {
+                                                  const uint64_t *__restrict__ occ_bits,
+                                                  uint2 *__restrict__ occ_list) {
+    // Only the occupied voxels are touched: their z positions come from the occupancy words
+    // k_comb_tables wrote, so a column without occupancy costs one scalar load.
     const int lane = threadIdx.x & 63;
-    const int64_t c0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * CPW;
-    if (c0 >= g.X) return;
+    const int32_t c0 = uni((int32_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * CPW);
+    if (c0 >= (int32_t)g.X) return;
     const int cur = 1 - st->parity;
-    const double *co = st->comb_origin[cur];
+    const int32_t Z = g.Z, X = (int32_t)g.X;
+    const int NW = (Z + 63) >> 6;
     int32_t *ctab = comb_tables + (int64_t)cur * g.V;
     // column bases: earlier chunks' totals + in-chunk exclusive prefix (CPW divides COLSCAN)
     uint32_t chunk_pre = 0;
     {
-        const int64_t q = c0 / COLSCAN;
-        for (int64_t k = lane; k < q; k += 64) chunk_pre += (uint32_t)chunk_tot[k];
+        const int32_t q = c0 / COLSCAN;
+        for (int32_t k = lane; k < q; k += 64) chunk_pre += (uint32_t)chunk_tot[k];
         for (int off = 32; off > 0; off >>= 1) chunk_pre += __shfl_xor(chunk_pre, off);
     }
-    uint32_t run[CPW];
-    bool any_occ[CPW];
-    int first_free[CPW];
+    const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
     for (int j = 0; j < CPW; ++j) {
-        run[j] = c0 + j < g.X ? chunk_pre + (uint32_t)col_base[c0 + j] : 0u;
-        any_occ[j] = false;
-        first_free[j] = -1;
-    }
-    const uint64_t lt = (1ull << lane) - 1ull;
-    for (int z0 = 0; z0 < g.Z; z0 += 64) {
-        const int z = z0 + lane;
-        int32_t v[CPW];
-#pragma unroll
-        for (int j = 0; j < CPW; ++j) v[j] = (z < g.Z && c0 + j < g.X) ? ctab[(c0 + j) * g.Z + z] : 0;
-#pragma unroll
-        for (int j = 0; j < CPW; ++j) {
-            const bool occ = v[j] < 0;
-            const uint64_t ob = __ballot(occ);
-            const uint64_t fb = __ballot(!occ && v[j] >= 1);
-            if (occ) {
-                const uint32_t id = run[j] + __popcll(ob & lt);
-                const bool lowest = !any_occ[j] && lane == __builtin_ctzll(ob);
-                ctab[(c0 + j) * g.Z + z] = (int32_t)(GVOM_OCC | id);
-                occ_list[id] = make_uint2((uint32_t)((c0 + j) * g.Z + z) | (lowest ? 0x80000000u : 0u),
-                                          (uint32_t)v[j] & MASK31);
+        if (c0 + j >= X) break;
+        uint32_t run = chunk_pre + (uint32_t)col_base[c0 + j];
+        bool any_occ = false;
+        for (int w = 0; w < NW; ++w) {
+            const uint64_t ob = occ_bits[(int64_t)(c0 + j) * NW + w];
+            if (!ob) continue;
+            if ((ob >> lane) & 1ull) {
+                const int32_t lin = (c0 + j) * Z + w * 64 + lane;
+                const uint32_t id = run + __popcll(ob & lt);
+                const bool lowest = !any_occ && lane == __builtin_ctzll(ob);
+                const int32_t v = ctab[lin];
+                ctab[lin] = (int32_t)(GVOM_OCC | id);
+                occ_list[id] = make_uint2((uint32_t)lin | (lowest ? 0x80000000u : 0u), (uint32_t)v & MASK31);
             }
-            run[j] += __popcll(ob);
-            any_occ[j] = any_occ[j] || ob != 0;
-            if (first_free[j] < 0 && fb) first_free[j] = z0 + __builtin_ctzll(fb);
-        }
-    }
-    if (lane < CPW && c0 + lane < g.X) {
-        bool ao = any_occ[0];
-        int ff = first_free[0];
-#pragma unroll
-        for (int j = 1; j < CPW; ++j)
-            if (lane == j) {
-                ao = any_occ[j];
-                ff = first_free[j];
-            }
-        const int64_t c = c0 + lane;
-        inferred[c] = ff >= 0 ? ((double)ff + co[2]) * g.z_res : -1000.0;
-        if (!ao) {
-            int64_t x, y;
-            col_xy(c, g, x, y);
-            const double *ego = st->latest_ego;
-            double h = -1000.0;
-            double xp = ((co[0] + (double)x) * g.xy_res) - ego[0];
-            double yp = ((co[1] + (double)y) * g.xy_res) - ego[1];
-            if (xp * xp + yp * yp <= g.robot_r * g.robot_r) h = ego[2] - g.g2l;
-            height[c] = h;
+            run += __popcll(ob);
+            any_occ = true;
         }
     }
 }
@@ -770,8 +769,10 @@ inline unsigned nblocks(int64_t n, int per) { return (unsigned)((n + per - 1) / 
 }  // namespace
 
 hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_tables, int32_t *comb_tables,
-                              int32_t *col_count, hipStream_t s) {
-    k_comb_tables<<<nblocks((g.X + CPW - 1) / CPW, 4), 256, 0, s>>>(g, st, slot_tables, comb_tables, col_count);
+                              int32_t *col_count, uint64_t *occ_bits, double *height, double *inferred,
+                              hipStream_t s) {
+    k_comb_tables<<<nblocks((g.X + CPW - 1) / CPW, 4), 256, 0, s>>>(g, st, slot_tables, comb_tables, col_count,
+                                                                    occ_bits, height, inferred);
     return hipGetLastError();
 }
 
@@ -785,10 +786,10 @@ hipError_t launch_scan_columns(int32_t *col_count, int64_t X, int32_t *chunk_tot
 hipError_t launch_comb_records(const Geo &g, DevState *st, const int32_t *slot_tables,
                                const SlotRec *slot_recs, int64_t ncap, int32_t *comb_tables,
                                CombRec *comb_recs, int64_t ccap, const int32_t *col_base,
-                               const int32_t *chunk_tot, uint2 *occ_list, double *height, double *inferred,
+                               const int32_t *chunk_tot, const uint64_t *occ_bits, uint2 *occ_list, double *height,
                                hipStream_t s) {
-    k_comb_ids<<<nblocks((g.X + CPW - 1) / CPW, 4), 256, 0, s>>>(g, st, comb_tables, col_base, chunk_tot, occ_list,
-                                                                 height, inferred);
+    k_comb_ids<<<nblocks((g.X + CPW - 1) / CPW, 4), 256, 0, s>>>(g, st, comb_tables, col_base, chunk_tot, occ_bits,
+                                                                 occ_list);
     k_comb_merge<<<2048, 256, 0, s>>>(g, st, slot_tables, slot_recs, ncap, comb_tables, comb_recs, ccap, occ_list,
                                       chunk_tot, col_chunks(g.X), height);
     k_commit_combine<<<1, 64, 0, s>>>(st, chunk_tot, col_chunks(g.X));

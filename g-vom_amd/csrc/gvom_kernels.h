@@ -40,21 +40,25 @@ hipError_t launch_mom_seg(const int32_t *seg_start, const uint32_t *seg_key, int
                           DevState *st, int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap, const Moment *part,
                           Moment *mom, hipStream_t s);
 // Neighbourhood mean/covariance per occupied voxel (gvom.py:1250-1387).
+// Then the one-wave commit: slot origins, latest ego and ring indices of the S scans.
 hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t seg_cap, const Geo &g,
-                          DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap, int S,
-                          hipStream_t s);
+                          DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap,
+                          const ScanDesc *sd, int S, hipStream_t s);
 
 
 // ---- combine stage (gvom_combine.hip) ----------------------------------------------------
-hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_tables,
-                              int32_t *comb_tables, int32_t *col_count, hipStream_t s);
+// Per column it also writes the occupied-voxel count, the occupancy words (bit z of word
+// z / 64), the inferred height and, for columns without occupancy, the height.
+hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_tables, int32_t *comb_tables,
+                              int32_t *col_count, uint64_t *occ_bits, double *height, double *inferred,
+                              hipStream_t s);
 int64_t col_chunks(int64_t X);
 hipError_t launch_scan_columns(int32_t *col_count, int64_t X, int32_t *chunk_tot, DevState *st, hipStream_t s);
-// Cell ids + compact list + inferred height, then the record merge + eigenvalues + heights.
+// Cell ids + compact list, then the record merge + eigenvalues + heights of occupied columns.
 hipError_t launch_comb_records(const Geo &g, DevState *st, const int32_t *slot_tables,
                                const SlotRec *slot_recs, int64_t ncap, int32_t *comb_tables,
                                CombRec *comb_recs, int64_t ccap, const int32_t *col_base,
-                               const int32_t *chunk_tot, uint2 *occ_list, double *height, double *inferred,
+                               const int32_t *chunk_tot, const uint64_t *occ_bits, uint2 *occ_list, double *height,
                                hipStream_t s);
 hipError_t launch_maps2d(const Geo &g, const DevState *st, const double *height, const double *inferred,
                          const int32_t *comb_tables, const CombRec *comb_recs, int64_t ccap, double *slope_x,

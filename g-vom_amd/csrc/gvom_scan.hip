@@ -101,21 +101,6 @@ __global__ __launch_bounds__(256) void k_prepare(const P *__restrict__ in, P *__
     if (rsec && threadIdx.x < 8) scount[threadIdx.x] = 0;
     if (rsec) __syncthreads();
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        // Slot origins for combine (gvom.py:208) and the ego the height map uses (gvom.py:110-112).
-        int next = st->next_slot;
-        for (int s = 0; s < S; ++s) {
-            double o[3];
-            scan_origin(sd[s], g, o);
-            int slot = (next + s) % g.B;
-            st->slot_origin[slot][0] = o[0];
-            st->slot_origin[slot][1] = o[1];
-            st->slot_origin[slot][2] = o[2];
-        }
-        st->latest_ego[0] = sd[S - 1].ego[0];
-        st->latest_ego[1] = sd[S - 1].ego[1];
-        st->latest_ego[2] = sd[S - 1].ego[2];
-    }
     if (i < NT) {
     int s = scan_of(sd, S, i);
     const ScanDesc &d = sd[s];
@@ -871,9 +856,25 @@ __device__ __forceinline__ double red32(double v) {
     return v;
 }
 
-// Ring bookkeeping after the scan stage of S scans (gvom.py:211-216), by one thread.
-__device__ __forceinline__ void commit_process(DevState *st, int S, int B) {
+// Ring bookkeeping after the scan stage of S scans (gvom.py:211-216), by one thread: slot
+// origins for combine (gvom.py:208), the ego the height map uses (gvom.py:110-112), then the
+// ring indices.  Everything the combine reads about the ring is written here, at the end of
+// the scan stage, so that the stage's earlier kernels may run while a combine still reads
+// the previous state.
+__device__ __forceinline__ void commit_process(DevState *st, const ScanDesc *sd, const Geo &g, int S) {
+    const int B = g.B;
     int next = st->next_slot;
+    for (int s = 0; s < S; ++s) {
+        double o[3];
+        scan_origin(sd[s], g, o);
+        const int slot = (next + s) % B;
+        st->slot_origin[slot][0] = o[0];
+        st->slot_origin[slot][1] = o[1];
+        st->slot_origin[slot][2] = o[2];
+    }
+    st->latest_ego[0] = sd[S - 1].ego[0];
+    st->latest_ego[1] = sd[S - 1].ego[1];
+    st->latest_ego[2] = sd[S - 1].ego[2];
     int mask = st->filled_mask;
     for (int s = 0; s < S; ++s) mask |= 1 << ((next + s) % B);
     st->filled_mask = mask;
@@ -975,8 +976,8 @@ __global__ __launch_bounds__(256) void k_metrics(const uint32_t *__restrict__ se
     }
 }
 
-__global__ void k_commit_process(DevState *st, int S, int B) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) commit_process(st, S, B);
+__global__ void k_commit_process(DevState *st, const ScanDesc *sd, Geo g, int S) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) commit_process(st, sd, g, S);
 }
 
 
@@ -1084,31 +1085,49 @@ hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, co
 
 // Tiled scratch -> slot tables, and the scratch back to zero: one block per 4 x 4 tile
 // column of one scan of the batch.  The tile column's z_size * 16 words are contiguous in
-// the scratch; each of its 4 x-rows is 4 * z_size contiguous words of the slot table.
+// the scratch (16-byte loads and zero stores); each of its 4 x-rows is 4 * z_size
+// contiguous words of the slot table and is written by one wavefront.
 __global__ __launch_bounds__(256) void k_tile_convert(Geo g, const DevState *__restrict__ st, int force_slot,
                                                       int32_t *__restrict__ scratch, int32_t *__restrict__ tables) {
-    __shared__ int32_t sh[256 * 16];   // [z][16] for z_size <= 256
+    // [z][17]: the tile's 16 columns at one z, padded by one word so that a wavefront reading
+    // 4 z-consecutive words of 16 columns spreads over the banks
+    __shared__ int32_t sh[256 * 17];   // z_size <= 256
     const TileGeo tg = tile_geo(g);
     const int tcol = blockIdx.x, s = blockIdx.y;
     const int txi = tcol / tg.ty, tyi = tcol - (tcol / tg.ty) * tg.ty;
-    const int Z = g.Z;
-    int32_t *src = scratch + (int64_t)s * tg.vt + (int64_t)tcol * Z * 16;
-    for (int k = threadIdx.x; k < Z * 16; k += 256) {
-        sh[k] = src[k];
-        src[k] = 0;
+    const int Z = g.Z, XY = g.XY;
+    // the tile column is 64-byte aligned (vt and z_size * 16 are multiples of 16 words)
+    int4 *src = reinterpret_cast<int4 *>(scratch + (int64_t)s * tg.vt + (int64_t)tcol * Z * 16);
+    for (int k = threadIdx.x; k < Z * 4; k += 256) {
+        const int4 v = src[k];
+        src[k] = make_int4(0, 0, 0, 0);
+        int32_t *d = sh + (k >> 2) * 17 + (k & 3) * 4;
+        d[0] = v.x;
+        d[1] = v.y;
+        d[2] = v.z;
+        d[3] = v.w;
     }
     __syncthreads();
     const int slot = force_slot >= 0 ? force_slot : (st->next_slot + s) % g.B;
     int32_t *tab = tables + (int64_t)slot * g.V;
-    // x-row a (0..3): columns (x, y0 .. y0 + ny - 1), ny * Z contiguous words
-    const int y0 = tyi * 4, ny = (g.XY - y0) < 4 ? (g.XY - y0) : 4;
-    for (int a = 0; a < 4; ++a) {
-        const int x = txi * 4 + a;
-        if (x >= g.XY) break;
-        int32_t *row = tab + ((int64_t)x * g.XY + y0) * Z;
-        for (int k = threadIdx.x; k < ny * Z; k += 256) {
+    // wavefront a writes x-row a: columns (x, y0 .. y0 + ny - 1), ny * Z contiguous words
+    const int a = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int x = txi * 4 + a;
+    const int y0 = tyi * 4, ny = (XY - y0) < 4 ? (XY - y0) : 4;
+    if (x >= XY) return;
+    int32_t *row = tab + ((int64_t)x * XY + y0) * Z;
+    const int n = ny * Z;
+    if ((Z & 3) == 0) {
+        // 16-byte stores: words k .. k + 3 are z .. z + 3 of column b
+        for (int k = lane * 4; k < n; k += 256) {
             const int b = k / Z, z = k - (k / Z) * Z;
-            row[k] = sh[z * 16 + a * 4 + b];
+            const int32_t *q = sh + z * 17 + a * 4 + b;
+            *reinterpret_cast<int4 *>(row + k) = make_int4(q[0], q[17], q[34], q[51]);
+        }
+    } else {
+        for (int k = lane; k < n; k += 64) {
+            const int b = k / Z, z = k - (k / Z) * Z;
+            row[k] = sh[z * 17 + a * 4 + b];
         }
     }
 }
@@ -1176,15 +1195,15 @@ hipError_t launch_mom_seg(const int32_t *seg_start, const uint32_t *seg_key, int
 }
 
 hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t seg_cap, const Geo &g,
-                          DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap, int S,
-                          hipStream_t s) {
+                          DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap,
+                          const ScanDesc *sd, int S, hipStream_t s) {
     // grid-stride: enough 32-lane groups to fill the chip, never more than segments can need;
     // then the ring bookkeeping (one thread)
     int64_t nb = nblocks(seg_cap * 32, 256);
     if (nb > 2048) nb = 2048;
     if (nb < 1) nb = 1;
     k_metrics<<<(unsigned)nb, 256, 0, s>>>(seg_key, mom, g, st, slot_tables, slot_recs, ncap, S);
-    k_commit_process<<<1, 64, 0, s>>>(st, S, g.B);
+    k_commit_process<<<1, 64, 0, s>>>(st, sd, g, S);
     return hipGetLastError();
 }
 

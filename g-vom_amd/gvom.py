@@ -217,10 +217,17 @@ class Gvom:
         nat.check(self._lib.gvom_probe_atomics(self._h, int(pattern), int(n_atomics), out), self._h)
         return out[0], int(out[1])
 
-    def set_raycast_config(self, block=256, rays_per_thread=1, use_box=False, sector_order=False, tiled=False):
-        """Ray-cast launch shape and variant (tiled scratch excludes the LDS box)."""
+    def set_raycast_config(self, block=128, rays_per_thread=1, use_box=False, sector_order=False, tiled=True):
+        """Ray-cast launch shape and variant (tiled scratch excludes the LDS box); a new
+        handle starts at the defaults."""
         flags = (1 if use_box else 0) | (2 if sector_order else 0) | (4 if tiled else 0)
         nat.check(self._lib.gvom_set_raycast_config(self._h, int(block), int(rays_per_thread), flags), self._h)
+
+    def set_overlap(self, enabled=True):
+        """Let a device-pointer process call with the tiled ray cast start its transform,
+        sort and ray cast while the preceding combine_maps_device still runs (off by
+        default).  Its points must be complete when the call is made."""
+        nat.check(self._lib.gvom_set_overlap(self._h, 1 if enabled else 0), self._h)
 
     # -- hipGraph capture ------------------------------------------------------------
     def graph_begin(self, max_scans=64):

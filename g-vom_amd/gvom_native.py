@@ -48,6 +48,7 @@ SIGNATURES = {
     "gvom_set_timing": (ctypes.c_int, [_P, _I32]),
     "gvom_stage_times": (ctypes.c_int, [_P, _P, _I32]),
     "gvom_stream": (ctypes.c_void_p, [_P]),
+    "gvom_set_overlap": (ctypes.c_int, [_P, _I32]),
     "gvom_diff_forward": (ctypes.c_int, [_P, _P, _I64, _P, _P, ctypes.c_double, _P]),
     "gvom_graph_begin": (ctypes.c_int, [_P, _I32]),
     "gvom_graph_end": (ctypes.c_int, [_P, _P]),

@@ -229,6 +229,16 @@ int gvom_stage_times(gvom_handle *h, float *ms, int32_t n);
  * against it. */
 void *gvom_stream(gvom_handle *h);
 
+/* Scan/combine overlap (no reference counterpart; off by default).  When enabled, a
+ * device-pointer scan stage (gvom_process_pointcloud_device, gvom_process_batch with
+ * on_device) with the tiled ray cast starts its transform, key sort and ray cast on a
+ * second stream as soon as the previous scan stage has finished, concurrently with a
+ * gvom_combine_maps_device enqueued in between; the rest of the stage, and everything else,
+ * stays ordered on the handle's stream.  The caller's points must therefore be complete
+ * when the process call is made (not merely enqueued on the handle's stream after the
+ * preceding combine).  Also applies inside a hipGraph capture.  Fails while capturing. */
+int gvom_set_overlap(gvom_handle *h, int32_t enabled);
+
 /* hipGraph capture of a launch sequence (no reference counterpart).
  * Between gvom_graph_begin and gvom_graph_end, gvom_process_pointcloud_device,
  * gvom_process_batch(on_device=1) and gvom_combine_maps_device calls are recorded
@@ -258,7 +268,9 @@ void gvom_graph_destroy(gvom_graph *g);
 int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int32_t dtype, const double ego[3],
                   const void *T, int32_t t_dtype, int32_t chosen[3], float *best_ms);
 /* Sets the ray-cast configuration directly (flags as for gvom_autotune; the box and the
- * tiled scratch exclude each other). */
+ * tiled scratch exclude each other).  A new handle starts at {128, 1, tiled}, the fastest
+ * configuration on the headline scan; the tiled scratch (min(buffer_size, 32) tables) is
+ * allocated by the first process call or gvom_graph_begin. */
 int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t rays_per_thread, int32_t flags);
 
 /* Differentiable form (no reference counterpart).  Processes one scan like

@@ -141,10 +141,11 @@ def test_graph_replay_matches_eager(cfg4):
 def test_ray_cast_variants_equal(cfg4, flags):
     """Every ray-cast variant the autotuner chooses from (tiled scratch, sector order, LDS
     box) leaves slot tables and records bit-identical to the plain cast, for single scans
-    and ragged 3-scan batches."""
+    and ragged 3-scan batches, and so do the combined map and the 2D maps."""
     import gvom
     wl = cfg4
     a = gvom.Gvom(*wl.map_args)
+    a.set_raycast_config(256, 1, False, False, False)
     b = gvom.Gvom(*wl.map_args)
     b.set_raycast_config(128, 2, bool(flags & 1), bool(flags & 2), bool(flags & 4))
     for k, step in enumerate(wl.steps):
@@ -164,4 +165,57 @@ def test_ray_cast_variants_equal(cfg4, flags):
             assert (sa is None) == (sb is None)
             if sa is not None:
                 _state_equal(sa, sb, f"flags {flags} step {k} slot {slot}")
-        print(f"ray variant {flags}: step {k} slots identical", flush=True)
+        a.combine_maps()
+        b.combine_maps()
+        _state_equal(a.combined_state(), b.combined_state(), f"flags {flags} step {k} combined")
+        _state_equal(a.maps_state(), b.maps_state(), f"flags {flags} step {k} maps")
+        print(f"ray variant {flags}: step {k} slots and combined map identical", flush=True)
+
+
+def test_overlap_matches_serial():
+    """With scan/combine overlap on, device-pointer frames issued back to back (eager, then
+    a captured 4-frame sequence replayed twice) leave the ring, every slot, the combined map
+    and the 2D maps bit-identical to the same frames without overlap."""
+    import gvom
+    import gvom_native as nat
+    wl = synth.workload(2)
+    scans = [st[0] for st in wl.steps]
+    dev = [nat.DeviceArray(s.points) for s in scans]
+    serial = gvom.Gvom(*wl.map_args)
+    ovl = gvom.Gvom(*wl.map_args)
+    ovl.set_overlap(True)
+
+    def frame(g, k, shift):
+        s = scans[k % len(scans)]
+        g.process_pointcloud_device(dev[k % len(scans)], s.points.shape[0], s.ego + shift, s.transform)
+        g.combine_maps_device()
+
+    def compare(label):
+        assert serial.ring_state() == ovl.ring_state()
+        _state_equal(serial.combined_state(), ovl.combined_state(), f"{label} combined")
+        _state_equal(serial.maps_state(), ovl.maps_state(), f"{label} maps")
+        for slot in range(wl.map_args[4]):
+            a_s, b_s = serial.slot_state(slot), ovl.slot_state(slot)
+            assert (a_s is None) == (b_s is None)
+            if a_s is not None:
+                _state_equal(a_s, b_s, f"{label} slot {slot}")
+
+    zero = np.zeros(3)
+    for k in range(6):
+        frame(serial, k, zero)
+        frame(ovl, k, zero)
+    compare("eager")
+    print("overlap eager: 6 frames identical", flush=True)
+    ovl.sync()
+    ovl.graph_begin(16)
+    for k in range(4):
+        frame(ovl, k, zero)
+    gr = ovl.graph_end()
+    shift = np.array([0.4, -0.2, 0.1])
+    for r in range(2):
+        egos = np.stack([scans[k].ego + r * shift for k in range(4)])
+        gr.launch(egos, np.stack([scans[k].transform for k in range(4)]))
+        for k in range(4):
+            frame(serial, k, r * shift)
+        compare(f"replay {r}")
+        print(f"overlap graph replay {r}: identical", flush=True)

@@ -131,3 +131,31 @@ def test_odd_map_sizes(variant):
         for name, v in exp.items():
             assert_equal(f"odd {variant} step {k} {name}", grids[name], v)
         print(f"odd map {variant} step {k}: cells {len(o.combined.hit)} ok", flush=True)
+
+
+@pytest.mark.parametrize("variant", [(128, 1, 4), (256, 1, 0)])
+def test_vertical_and_lateral_motion(variant):
+    """Ego heights and positions that move the map origin by a few voxels up, down and
+    sideways between frames, so that every source of the combine is read shifted in x, y
+    and z; 256 x 256 x 64 map, buffer 4, five frames, each checked against the oracle."""
+    import gvom
+    wl = synth.workload(2)
+    s0 = wl.steps[0][0]
+    dz = [0.0, 0.5, -0.7, 1.3, -2.1]
+    steps = [[synth.ScanInput(s0.points, s0.ego + np.array([0.45 * k, -0.3 * k, dz[k]]), s0.transform, k)]
+             for k in range(5)]
+    g = gvom.Gvom(*wl.map_args)
+    g.set_raycast_config(*variant[:2], False, False, bool(variant[2] & 4))
+    o = OracleGvom(*wl.map_args)
+    for k, step in enumerate(steps):
+        for s in step:
+            g.Process_pointcloud(s.points, s.ego, s.transform)
+            o.Process_pointcloud(s.points, s.ego, s.transform)
+            slot = o.last_buffer_index
+            compare_scan(g.slot_state(slot), o.slots[slot], f"motion {variant} step {k} slot {slot}")
+        out_g = g.combine_maps()
+        out_o = o.combine_maps()
+        compare_combined(g.combined_state(), o.combined, f"motion {variant} step {k} combined")
+        compare_maps(g.maps_state(), o.combined, f"motion {variant} step {k}")
+        assert_equal(f"motion {variant} step {k} origin_m", out_g[0], out_o[0])
+        print(f"motion {variant} step {k}: cells {len(o.combined.hit)} ok", flush=True)
