@@ -65,6 +65,7 @@ struct gvom_handle {
            *delta = nullptr;
     int32_t *pos = nullptr, *neg = nullptr, *vis = nullptr, *col_count = nullptr;
     uint64_t *occ_bits = nullptr;   // combined map: per column, bit z of word z / 64 = occupied
+    int32_t *zero_page = nullptr;   // zeros the combine reads in place of off-map words
     uint2 *occ_list = nullptr;
     int32_t *export_buf = nullptr;
     int32_t *chunk_tot = nullptr;
@@ -142,7 +143,7 @@ static void free_all(gvom_handle *h) {
                     h->col_count, h->chunk_tot, h->export_buf, h->pts_in, h->world, h->valid, h->k0, h->v0, h->k1, h->v1,
                     h->hist, h->blk, h->seg_key, h->seg_start, h->mom, h->part, h->sd, h->diff_rec,
                     h->diff_colh, h->diff_part, h->diff_sums, h->diff_colpt, h->diff_pmark, h->graph_sd, h->rsec, h->rorder, h->grids, h->raw, h->tscratch,
-                    h->occ_bits};
+                    h->occ_bits, h->zero_page};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : h->ev_pool)
@@ -249,6 +250,8 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     CKC(dalloc(&h->occ_list, V));
     CKC(dalloc(&h->col_count, X + 16));
     CKC(dalloc(&h->occ_bits, X * ((g.Z + 63) / 64)));
+    CKC(dalloc(&h->zero_page, zero_page_words()));
+    CKC(hipMemset(h->zero_page, 0, sizeof(int32_t) * zero_page_words()));
     CKC(dalloc(&h->chunk_tot, col_chunks(X) + 1));
     CKC(dalloc(&h->sd, GVOM_MAX_BATCH));
 #undef CKC
@@ -655,8 +658,8 @@ static int combine_impl(gvom_handle *h) {
     if (h->capturing) h->cap_ops.push_back(-1);
     const int cur = 1 - h->parity;
     stage_event(h, 5);
-    CKH(h, launch_comb_tables(g, h->st, h->slot_tables, h->comb_tables, h->col_count, h->occ_bits, h->height,
-                              h->inferred, st));
+    CKH(h, launch_comb_tables(g, h->st, h->slot_tables, h->zero_page, h->comb_tables, h->col_count, h->occ_bits,
+                              h->height, h->inferred, st));
     stage_event(h, 6);
     CKH(h, launch_scan_columns(h->col_count, g.X, h->chunk_tot, h->st, st));
     CKH(h, launch_comb_records(g, h->st, h->slot_tables, h->slot_recs, h->ncap, h->comb_tables, h->comb_recs,

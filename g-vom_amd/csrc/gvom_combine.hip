@@ -200,6 +200,168 @@ __global__ __launch_bounds__(256) void k_comb_tables(Geo g, const DevState *__re
     }
 }
 
+// Same combine for xy_size % 4 == 0, with less scalar work per column.  The 4 columns of a
+// group lie in one x row; each source's shift (precomputed by the scan stage's commit) gives
+// one uniform column pointer, and a word that is off the map (column or z) is read from a
+// zero page instead of being masked after the load, so no per-word predicate outlives its
+// address.  Each wavefront handles GPW groups.  (The general kernel above issues ~1200
+// scalar instructions per wavefront, which made it scalar-issue bound.)
+constexpr int GPW = 2;
+constexpr int ZERO_PAGE_WORDS = 4 * 256 + 64;   // covers lane + j * z_size for z_size <= 256
+
+// combine rule of one voxel: union of the sources' occupancy and sum of their passes, then
+// the previous map (gvom.py:1010-1063)
+__device__ __forceinline__ int32_t comb_word(uint32_t m, uint32_t p, bool okl, int32_t vl, bool &occ, bool &freev) {
+    if (okl) {
+        if (m) {
+            if (vl < 0) m |= 1u << LAST_BIT;
+        } else if (vl < 0) {
+            if (p <= 10u) m = 1u << LAST_BIT;   // carry-over (gvom.py:1058)
+        } else {
+            p += (uint32_t)vl;
+        }
+    }
+    occ = m != 0;
+    freev = m == 0 && p >= 1u;
+    return m ? (int32_t)(GVOM_OCC | m) : (int32_t)p;
+}
+
+__global__ __launch_bounds__(256) void k_comb_rows(Geo g, const DevState *__restrict__ st,
+                                                   const int32_t *__restrict__ slot_tables,
+                                                   const int32_t *__restrict__ zero_page,
+                                                   int32_t *__restrict__ comb_tables,
+                                                   int32_t *__restrict__ col_count,
+                                                   uint64_t *__restrict__ occ_bits,
+                                                   double *__restrict__ height,
+                                                   double *__restrict__ inferred) {
+    const int lane = threadIdx.x & 63;
+    const int32_t XY = g.XY, Z = g.Z, X = (int32_t)g.X, V = (int32_t)g.V;
+    const int32_t cw = uni((int32_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (CPW * GPW));
+    if (cw >= X) return;
+    const int parity = uni(st->parity), cur = 1 - parity;
+    const int have_last = uni(st->have_last);
+    const int last_slot = uni(st->last_slot);
+    const double *co = st->slot_origin[last_slot];   // combined origin = newest slot's (gvom.py:229)
+    const int32_t ldx = have_last ? uni((int32_t)(co[0] - st->comb_origin[parity][0])) : 0;
+    const int32_t ldy = have_last ? uni((int32_t)(co[1] - st->comb_origin[parity][1])) : 0;
+    const int32_t ldz = have_last ? uni((int32_t)(co[2] - st->comb_origin[parity][2])) : 0;
+    const int32_t *ltab = comb_tables + (int64_t)parity * V;
+    const int32_t *zp = zero_page + lane;
+    const int NW = (Z + 63) >> 6;
+    uint32_t count[GPW][CPW], ffree[GPW][CPW];
+#pragma unroll
+    for (int q = 0; q < GPW; ++q)
+#pragma unroll
+        for (int j = 0; j < CPW; ++j) {
+            count[q][j] = 0;
+            ffree[q][j] = 0xFFFFFFFFu;
+        }
+    for (int z0 = 0; z0 < Z; z0 += 64) {
+        const int z = z0 + lane;
+        const bool zin = z < Z;
+        uint32_t mask[GPW][CPW], P[GPW][CPW];
+#pragma unroll
+        for (int q = 0; q < GPW; ++q)
+#pragma unroll
+            for (int j = 0; j < CPW; ++j) mask[q][j] = P[q][j] = 0;
+        for (int s0 = 0; s0 < g.B; s0 += SRC_GROUP) {
+#pragma unroll
+            for (int q = 0; q < GPW; ++q) {
+                const int32_t c0 = cw + q * CPW;
+                const int32_t x0 = c0 / XY, y0 = c0 - (c0 / XY) * XY;
+                int32_t v[SRC_GROUP][CPW];
+#pragma unroll
+                for (int k = 0; k < SRC_GROUP; ++k) {
+                    // shifts precomputed by the scan stage's commit (one 16-byte scalar load)
+                    const int s = s0 + k < g.B ? s0 + k : 0;
+                    const int4 sh = *reinterpret_cast<const int4 *>(st->comb_shift[s]);
+                    const bool use = (s0 + k < g.B) & (sh.w != 0) & (c0 < X);
+                    const int32_t ux = x0 + sh.x, uy0 = y0 + sh.y;
+                    const bool rowok = use & ((uint32_t)ux < (uint32_t)XY);   // uniform
+                    const int zs = z + sh.z;
+                    const bool zok = zin & ((uint32_t)zs < (uint32_t)Z);
+                    // column j of the source: cp + j * Z (wrapping arithmetic; dereferenced
+                    // only where the column is on the map)
+                    const int32_t *cp = slot_tables + (int64_t)s * V +
+                                        (int32_t)(((uint32_t)ux * (uint32_t)XY + (uint32_t)uy0) * (uint32_t)Z);
+#pragma unroll
+                    for (int j = 0; j < CPW; ++j) {
+                        const bool cok = rowok & ((uint32_t)(uy0 + j) < (uint32_t)XY);   // uniform
+                        const int32_t *a = (cok & zok) ? cp + j * Z + zs : zp + j * Z;
+                        v[k][j] = *a;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < SRC_GROUP; ++k)
+#pragma unroll
+                    for (int j = 0; j < CPW; ++j) {
+                        if (v[k][j] < 0) mask[q][j] |= 1u << (s0 + k);
+                        else P[q][j] += (uint32_t)v[k][j];
+                    }
+            }
+        }
+        const int zl = z + ldz;
+        const bool zokl = (have_last != 0) & zin & ((uint32_t)zl < (uint32_t)Z);
+#pragma unroll
+        for (int q = 0; q < GPW; ++q) {
+            const int32_t c0 = cw + q * CPW;
+            if (c0 >= X) break;
+            const int32_t x0 = c0 / XY, y0 = c0 - (c0 / XY) * XY;
+            const int32_t ux = x0 + ldx, uy0 = y0 + ldy;
+            const bool rowok = (uint32_t)ux < (uint32_t)XY;
+            const int32_t *cp = ltab + (int32_t)(((uint32_t)ux * (uint32_t)XY + (uint32_t)uy0) * (uint32_t)Z);
+            int32_t vl[CPW];
+            bool okl[CPW];
+#pragma unroll
+            for (int j = 0; j < CPW; ++j) {
+                okl[j] = rowok & zokl & ((uint32_t)(uy0 + j) < (uint32_t)XY);
+                vl[j] = *(okl[j] ? cp + j * Z + zl : zp + j * Z);
+            }
+            int32_t *out = comb_tables + (int64_t)cur * V + (int64_t)c0 * Z;
+#pragma unroll
+            for (int j = 0; j < CPW; ++j) {
+                bool occ, freev;
+                const int32_t o = comb_word(mask[q][j], P[q][j], okl[j], vl[j], occ, freev);
+                if (zin) out[j * Z + z] = o;
+                const uint64_t ob = __ballot(occ);
+                count[q][j] += __popcll(ob);
+                if (lane == 0) occ_bits[(int64_t)(c0 + j) * NW + (z0 >> 6)] = ob;
+                // lowest free voxel with at least one pass (gvom.py:580-590)
+                const uint64_t fb = __ballot(zin && freev);
+                if (ffree[q][j] == 0xFFFFFFFFu && fb) ffree[q][j] = (uint32_t)(z0 + __builtin_ctzll(fb));
+            }
+        }
+    }
+    if (lane < GPW * CPW) {
+        const int32_t c = cw + lane;
+        if (c < X) {
+            uint32_t cnt = 0, ff = 0xFFFFFFFFu;
+#pragma unroll
+            for (int q = 0; q < GPW; ++q)
+#pragma unroll
+                for (int j = 0; j < CPW; ++j)
+                    if (lane == q * CPW + j) {
+                        cnt = count[q][j];
+                        ff = ffree[q][j];
+                    }
+            col_count[c] = (int32_t)cnt;
+            inferred[c] = ff != 0xFFFFFFFFu ? ((double)ff + co[2]) * g.z_res : -1000.0;
+            if (cnt == 0) {
+                // no occupied voxel: the robot-radius default around the latest ego, measured to
+                // the cell corner (gvom.py:561-570), else unknown (-1000); occupied columns get
+                // their height from the record merge
+                const int32_t x = c / XY, y = c - (c / XY) * XY;
+                const double *ego = st->latest_ego;
+                double h = -1000.0;
+                const double xp = ((co[0] + (double)x) * g.xy_res) - ego[0];
+                const double yp = ((co[1] + (double)y) * g.xy_res) - ego[1];
+                if (xp * xp + yp * yp <= g.robot_r * g.robot_r) h = ego[2] - g.g2l;
+                height[c] = h;
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------
 // Column id scan: blocks of 1024 threads scan COLSCAN consecutive column counts in place
 // (exclusive, int4 loads) and write their total; k_comb_records adds the totals of the
@@ -324,41 +486,37 @@ __global__ __launch_bounds__(256) void k_comb_ids(Geo g, const DevState *__restr
                                                   const int32_t *__restrict__ chunk_tot,
                                                   const uint64_t *__restrict__ occ_bits,
                                                   uint2 *__restrict__ occ_list) {
-    // Only the occupied voxels are touched: their z positions come from the occupancy words
-    // k_comb_tables wrote, so a column without occupancy costs one scalar load.
+    // One lane per column, 64 columns per wavefront.  Only the occupied voxels are touched:
+    // their z positions come from the occupancy words k_comb_tables wrote.
     const int lane = threadIdx.x & 63;
-    const int32_t c0 = uni((int32_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * CPW);
-    if (c0 >= (int32_t)g.X) return;
+    const int32_t X = (int32_t)g.X, Z = g.Z;
+    const int32_t cb = uni((int32_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64);
+    if (cb >= X) return;
     const int cur = 1 - st->parity;
-    const int32_t Z = g.Z, X = (int32_t)g.X;
     const int NW = (Z + 63) >> 6;
     int32_t *ctab = comb_tables + (int64_t)cur * g.V;
-    // column bases: earlier chunks' totals + in-chunk exclusive prefix (CPW divides COLSCAN)
+    // earlier chunks' totals (a wavefront's 64 columns lie in one chunk: 64 divides COLSCAN)
     uint32_t chunk_pre = 0;
     {
-        const int32_t q = c0 / COLSCAN;
+        const int32_t q = cb / COLSCAN;
         for (int32_t k = lane; k < q; k += 64) chunk_pre += (uint32_t)chunk_tot[k];
         for (int off = 32; off > 0; off >>= 1) chunk_pre += __shfl_xor(chunk_pre, off);
     }
-    const uint64_t lt = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int j = 0; j < CPW; ++j) {
-        if (c0 + j >= X) break;
-        uint32_t run = chunk_pre + (uint32_t)col_base[c0 + j];
-        bool any_occ = false;
-        for (int w = 0; w < NW; ++w) {
-            const uint64_t ob = occ_bits[(int64_t)(c0 + j) * NW + w];
-            if (!ob) continue;
-            if ((ob >> lane) & 1ull) {
-                const int32_t lin = (c0 + j) * Z + w * 64 + lane;
-                const uint32_t id = run + __popcll(ob & lt);
-                const bool lowest = !any_occ && lane == __builtin_ctzll(ob);
-                const int32_t v = ctab[lin];
-                ctab[lin] = (int32_t)(GVOM_OCC | id);
-                occ_list[id] = make_uint2((uint32_t)lin | (lowest ? 0x80000000u : 0u), (uint32_t)v & MASK31);
-            }
-            run += __popcll(ob);
-            any_occ = true;
+    const int32_t c = cb + lane;
+    if (c >= X) return;
+    uint32_t id = chunk_pre + (uint32_t)col_base[c];
+    bool lowest = true;
+    for (int w = 0; w < NW; ++w) {
+        uint64_t ob = occ_bits[(int64_t)c * NW + w];
+        while (ob) {
+            const int z = w * 64 + __builtin_ctzll(ob);
+            ob &= ob - 1;
+            const int32_t lin = c * Z + z;
+            const int32_t v = ctab[lin];
+            ctab[lin] = (int32_t)(GVOM_OCC | id);
+            occ_list[id] = make_uint2((uint32_t)lin | (lowest ? 0x80000000u : 0u), (uint32_t)v & MASK31);
+            lowest = false;
+            ++id;
         }
     }
 }
@@ -768,11 +926,18 @@ inline unsigned nblocks(int64_t n, int per) { return (unsigned)((n + per - 1) / 
 
 }  // namespace
 
-hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_tables, int32_t *comb_tables,
-                              int32_t *col_count, uint64_t *occ_bits, double *height, double *inferred,
-                              hipStream_t s) {
-    k_comb_tables<<<nblocks((g.X + CPW - 1) / CPW, 4), 256, 0, s>>>(g, st, slot_tables, comb_tables, col_count,
-                                                                    occ_bits, height, inferred);
+int64_t zero_page_words() { return ZERO_PAGE_WORDS; }
+
+hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_tables, const int32_t *zero_page,
+                              int32_t *comb_tables, int32_t *col_count, uint64_t *occ_bits, double *height,
+                              double *inferred, hipStream_t s) {
+    if (g.XY % CPW == 0 && zero_page)
+        k_comb_rows<<<nblocks((g.X + CPW * GPW - 1) / (CPW * GPW), 4), 256, 0, s>>>(g, st, slot_tables, zero_page,
+                                                                                 comb_tables, col_count, occ_bits,
+                                                                                 height, inferred);
+    else
+        k_comb_tables<<<nblocks((g.X + CPW - 1) / CPW, 4), 256, 0, s>>>(g, st, slot_tables, comb_tables, col_count,
+                                                                        occ_bits, height, inferred);
     return hipGetLastError();
 }
 
@@ -788,7 +953,7 @@ hipError_t launch_comb_records(const Geo &g, DevState *st, const int32_t *slot_t
                                CombRec *comb_recs, int64_t ccap, const int32_t *col_base,
                                const int32_t *chunk_tot, const uint64_t *occ_bits, uint2 *occ_list, double *height,
                                hipStream_t s) {
-    k_comb_ids<<<nblocks((g.X + CPW - 1) / CPW, 4), 256, 0, s>>>(g, st, comb_tables, col_base, chunk_tot, occ_bits,
+    k_comb_ids<<<nblocks((g.X + 63) / 64, 4), 256, 0, s>>>(g, st, comb_tables, col_base, chunk_tot, occ_bits,
                                                                  occ_list);
     k_comb_merge<<<2048, 256, 0, s>>>(g, st, slot_tables, slot_recs, ncap, comb_tables, comb_recs, ccap, occ_list,
                                       chunk_tot, col_chunks(g.X), height);

@@ -58,6 +58,10 @@ struct DevState {
     double latest_ego[3];
     int32_t sec_count[8];    // valid rays per azimuth sector (XCD-aware ray order)
     int32_t sec_cursor[8];
+    // per slot: shift of the newest slot's origin relative to the slot's (x, y, z voxels) and
+    // whether the slot holds a scan; written with the ring indices at the end of every scan
+    // stage for the combine that follows
+    alignas(16) int32_t comb_shift[GVOM_MAX_BUFFER][4];
 };
 
 // One occupied voxel of a scan slot (reference: hit_count, total_count, metrics f64[10],

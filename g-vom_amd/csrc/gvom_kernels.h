@@ -49,9 +49,12 @@ hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t se
 // ---- combine stage (gvom_combine.hip) ----------------------------------------------------
 // Per column it also writes the occupied-voxel count, the occupancy words (bit z of word
 // z / 64), the inferred height and, for columns without occupancy, the height.
-hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_tables, int32_t *comb_tables,
-                              int32_t *col_count, uint64_t *occ_bits, double *height, double *inferred,
-                              hipStream_t s);
+// zero_page: zero_page_words() zeroed int32 the row kernel reads instead of off-map words
+// (xy_size % 4 == 0; null selects the general kernel).
+int64_t zero_page_words();
+hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_tables, const int32_t *zero_page,
+                              int32_t *comb_tables, int32_t *col_count, uint64_t *occ_bits, double *height,
+                              double *inferred, hipStream_t s);
 int64_t col_chunks(int64_t X);
 hipError_t launch_scan_columns(int32_t *col_count, int64_t X, int32_t *chunk_tot, DevState *st, hipStream_t s);
 // Cell ids + compact list, then the record merge + eigenvalues + heights of occupied columns.

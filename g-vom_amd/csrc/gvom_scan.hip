@@ -878,8 +878,19 @@ __device__ __forceinline__ void commit_process(DevState *st, const ScanDesc *sd,
     int mask = st->filled_mask;
     for (int s = 0; s < S; ++s) mask |= 1 << ((next + s) % B);
     st->filled_mask = mask;
-    st->last_slot = (next + S - 1) % B;
+    const int last = (next + S - 1) % B;
+    st->last_slot = last;
     st->next_slot = (next + S) % B;
+    // the combine's source shifts: combined origin = newest slot's (gvom.py:229), each slot read
+    // at (combined voxel + origin difference) (gvom.py:1016-1022)
+    const double *co = st->slot_origin[last];
+    for (int s = 0; s < B; ++s) {
+        const double *so = st->slot_origin[s];
+        st->comb_shift[s][0] = (int32_t)(co[0] - so[0]);
+        st->comb_shift[s][1] = (int32_t)(co[1] - so[1]);
+        st->comb_shift[s][2] = (int32_t)(co[2] - so[2]);
+        st->comb_shift[s][3] = (mask >> s) & 1;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_metrics(const uint32_t *__restrict__ seg_key,
