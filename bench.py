@@ -358,8 +358,8 @@ def main():
 
     eager_fps = args.steps / el
     # the stage times above come from serial frames; from here on the scan stage of frame
-    # k + 1 may start while the combine of frame k runs (tiled ray cast only)
-    overlap = (not args.no_overlap) and bool(ray_cfg["flags"] & 4)
+    # k + 1 may run while the combine of frame k runs
+    overlap = not args.no_overlap
     g.set_overlap(overlap)
     eager_ovl_fps = None
     if overlap:

@@ -77,6 +77,7 @@ struct gvom_handle {
     uint32_t *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr, *hist = nullptr, *blk = nullptr,
              *seg_key = nullptr;
     int32_t *seg_start = nullptr;
+    int64_t hwords = 0;   // words of each of the sort's two histograms
     Moment *mom = nullptr, *part = nullptr;
     ScanDesc *sd = nullptr;
 
@@ -232,7 +233,7 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     CKC(hipEventRecord(h->ev_back, h->stream));
     CKC(dalloc(&h->st, 1));
     CKC(hipMemset(h->st, 0, sizeof(DevState)));
-    CKC(dalloc(&h->slot_tables, V * g.B));
+    CKC(dalloc(&h->slot_tables, V * 2 * g.B));   // two buffers per ring slot
     h->ccap = V;
     CKC(dalloc(&h->comb_tables, 2 * V));
     CKC(dalloc(&h->comb_recs, 2 * h->ccap));
@@ -254,6 +255,7 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     CKC(hipMemset(h->zero_page, 0, sizeof(int32_t) * zero_page_words()));
     CKC(dalloc(&h->chunk_tot, col_chunks(X) + 1));
     CKC(dalloc(&h->sd, GVOM_MAX_BATCH));
+    CKC(hipDeviceSynchronize());   // the fills above ran on the null stream
 #undef CKC
     *out = h;
     return GVOM_OK;
@@ -294,9 +296,9 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
         int64_t nc = h->ncap * 2 > need_ncap ? h->ncap * 2 : need_ncap;
         if (nc > g.V) nc = g.V;
         SlotRec *nr = nullptr;
-        CKH(h, dalloc(&nr, nc * g.B));
+        CKH(h, dalloc(&nr, nc * 2 * g.B));   // two buffers per ring slot
         if (h->slot_recs) {
-            for (int s = 0; s < g.B; ++s)
+            for (int s = 0; s < 2 * g.B; ++s)
                 CKH(h, hipMemcpyAsync(nr + (int64_t)s * nc, h->slot_recs + (int64_t)s * h->ncap,
                                       (size_t)h->ncap * sizeof(SlotRec), hipMemcpyDeviceToDevice, h->stream));
             CKH(h, hipStreamSynchronize(h->stream));
@@ -327,7 +329,11 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
         CKH(h, dalloc(&h->v0, pc));
         CKH(h, dalloc(&h->k1, pc));
         CKH(h, dalloc(&h->v1, pc));
-        CKH(h, dalloc(&h->hist, radix_hist_entries(pc)));
+        // the sort's two histograms; the first must start at zero (see radix_sort)
+        h->hwords = radix_hist_entries(pc);
+        CKH(h, dalloc(&h->hist, 2 * h->hwords));
+        CKH(h, hipMemsetAsync(h->hist, 0, sizeof(uint32_t) * 2 * h->hwords, h->stream));
+        CKH(h, hipStreamSynchronize(h->stream));
         CKH(h, dalloc(&h->blk, (pc / 2048 + 2) * (2 + 3 * GVOM_MAX_BATCH)));
         CKH(h, dalloc(&h->seg_key, pc));
         CKH(h, dalloc(&h->seg_start, pc + 1));
@@ -485,10 +491,11 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
         if (rc_t) return rc_t;
     }
     hipStream_t st = h->stream;
-    // The first part of the stage (up to the ray cast into the tiled scratch) touches only
-    // work buffers, the scratch and the caller's points; with overlap on it runs on `front`
-    // and may proceed while a combine enqueued before this call still runs on st.
-    const bool ovl = h->overlap && tiled && on_device && may_overlap;
+    // The stage writes only work buffers, the tiled scratch and the slots' second buffers
+    // (new_buf), never what a combine reads; with overlap on it runs on `front` and may proceed
+    // while a combine enqueued before this call still runs on st.  Only its commit (ring
+    // state, buffer bits) goes on st, after that combine.
+    const bool ovl = h->overlap && on_device && may_overlap;
     hipStream_t fs = ovl ? h->front : st;
     if (ovl) CKH(h, hipStreamWaitEvent(fs, h->ev_back, 0));
     const size_t esz = dtype == GVOM_F32 ? 4 : 8;
@@ -519,16 +526,16 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     const bool xcd = h->ray_xcd != 0;
     // the tiled path's conversion writes every word of the slot tables; the direct cast
     // counts into cleared tables
-    if (!tiled) CKH(h, launch_clear_slots(h->slot_tables, h->st, S, g, st));
+    if (!tiled) CKH(h, launch_clear_slots(h->slot_tables, h->st, S, g, fs));
     if (xcd) CKH(h, launch_fill_i32(&h->st->sec_count[0], 0, 16, fs));   // sec_count and sec_cursor
     CKH(h, launch_prepare(dtype, pts, h->world, h->k0, h->v0, h->valid, sd, S, NT, g, h->st,
-                          xcd ? h->rsec : nullptr, fs));
+                          xcd ? h->rsec : nullptr, h->hist, h->hwords, fs));
     if (xcd) CKH(h, launch_ray_order(h->rsec, NT, h->st, h->rorder, fs));
     // fork: keys, segments and chunk partials on the side stream, concurrent with the cast
     CKH(h, hipEventRecord(h->ev_fork, fs));
     CKH(h, hipStreamWaitEvent(h->side, h->ev_fork, 0));
     uint32_t *sk = h->k0, *sv = h->v0;
-    CKH(h, radix_sort(h->k0, h->v0, h->k1, h->v1, NT, g.VB + sbits, h->hist, &sk, &sv, h->side));
+    CKH(h, radix_sort(h->k0, h->v0, h->k1, h->v1, NT, g.VB + sbits, h->hist, h->hwords, &sk, &sv, h->side));
     h->sorted_vals = sv;
     CKH(h, launch_segments(sk, NT, h->seg_start, h->seg_key, h->blk, S, g, h->st, h->side));
     CKH(h, launch_mom_part(dtype, h->world, sk, sv, NT, sd, S, g, h->st, h->part, h->side));
@@ -538,20 +545,20 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
                           h->ray_block, h->ray_rpt, tiled ? 0 : h->ray_box, tiled ? 1 : 0, -1,
                           xcd ? h->rorder : nullptr, fs));
     stage_event(h, 9, fs);
+    if (tiled) CKH(h, launch_tile_convert(g, h->st, S, -1, h->tscratch, h->slot_tables, fs));
+    stage_event(h, 2, fs);
+    // join: the segment sums read the pass counts and the partials
+    CKH(h, hipStreamWaitEvent(fs, h->ev_join, 0));
+    stage_event(h, 3, fs);
+    CKH(h, launch_mom_seg(h->seg_start, h->seg_key, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, h->part,
+                          h->mom, fs));
+    CKH(h, launch_metrics(h->seg_key, h->mom, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, S, fs));
     if (ovl) {
-        // back on the main stream, after the combine: the conversion writes the slot table
+        // back on the main stream, after any combine enqueued before this call
         CKH(h, hipEventRecord(h->ev_front, fs));
         CKH(h, hipStreamWaitEvent(st, h->ev_front, 0));
     }
-    if (tiled) CKH(h, launch_tile_convert(g, h->st, S, -1, h->tscratch, h->slot_tables, st));
-    stage_event(h, 2);
-    // join: the segment sums read the pass counts and the partials
-    CKH(h, hipStreamWaitEvent(st, h->ev_join, 0));
-    stage_event(h, 3);
-    CKH(h, launch_mom_seg(h->seg_start, h->seg_key, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, h->part,
-                          h->mom, st));
-    // launch_metrics also launches the one-wave commit that advances the ring on the device
-    CKH(h, launch_metrics(h->seg_key, h->mom, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, sd, S, st));
+    CKH(h, launch_commit_process(h->st, sd, g, S, st));
     stage_event(h, 4);
     CKH(h, mark_back(h));
 
@@ -816,7 +823,7 @@ extern "C" int gvom_get_slot(gvom_handle *h, int32_t slot, int32_t *index_map, i
     if (origin)
         for (int k = 0; k < 3; ++k) origin[k] = ds.slot_origin[slot][k];
     if (index_map) {
-        rc = export_table(h, h->slot_tables + (int64_t)slot * h->g.V, index_map);
+        rc = export_table(h, h->slot_tables + (int64_t)cur_buf(ds.flip, slot) * h->g.V, index_map);
         if (rc) return rc;
     }
     if (hit || total || min_h || metrics) {
@@ -826,8 +833,8 @@ extern "C" int gvom_get_slot(gvom_handle *h, int32_t slot, int32_t *index_map, i
         }
         std::vector<SlotRec> recs((size_t)(C > 0 ? C : 1));
         if (C > 0) {
-            CKH(h, hipMemcpy(recs.data(), h->slot_recs + (int64_t)slot * h->ncap, (size_t)C * sizeof(SlotRec),
-                             hipMemcpyDeviceToHost));
+            CKH(h, hipMemcpy(recs.data(), h->slot_recs + (int64_t)cur_buf(ds.flip, slot) * h->ncap,
+                             (size_t)C * sizeof(SlotRec), hipMemcpyDeviceToHost));
         }
         for (int64_t i = 0; i < C; ++i) {
             if (hit) hit[i] = recs[i].hit;
@@ -1028,7 +1035,7 @@ extern "C" int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, 
         h->diff_part_cap = np;
     }
     const int slot = h->last_slot;
-    CKH(h, launch_diff_forward(g, h->st, slot, h->slot_tables, h->slot_recs + (int64_t)slot * h->ncap, h->ncap,
+    CKH(h, launch_diff_forward(g, h->st, slot, h->slot_tables, h->slot_recs, h->ncap,
                                h->seg_start, h->sorted_vals, (const double *)h->world, n, h->diff_rec, h->diff_colh,
                                h->diff_colpt, h->diff_pmark, h->diff_part, h->diff_sums, w_height, h->stream));
     double sums[8];
@@ -1252,7 +1259,8 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
         if (hipMemcpyAsync(saved, h->st, sizeof(DevState), hipMemcpyDeviceToDevice, st) != hipSuccess ||
             hipMemcpyAsync(h->sd, &d, sizeof(ScanDesc), hipMemcpyHostToDevice, st) != hipSuccess ||
             launch_fill_i32(&h->st->sec_count[0], 0, 16, st) != hipSuccess ||
-            launch_prepare(dtype, points, h->world, h->k0, h->v0, h->valid, h->sd, 1, n, g, h->st, h->rsec, st) !=
+            launch_prepare(dtype, points, h->world, h->k0, h->v0, h->valid, h->sd, 1, n, g, h->st, h->rsec, nullptr, 0,
+                           st) !=
                 hipSuccess ||
             launch_ray_order(h->rsec, n, h->st, h->rorder, st) != hipSuccess) {
             h->err = "autotune: setup failed";

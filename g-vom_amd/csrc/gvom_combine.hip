@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void k_comb_tables(Geo g, const DevState *__re
     if (c0 >= X) return;
     const int parity = uni(st->parity), cur = 1 - parity;
     const int filled = uni(st->filled_mask), have_last = uni(st->have_last);
-    const int last_slot = uni(st->last_slot);
+    const int last_slot = uni(st->last_slot), flip = uni(st->flip);
     const double *co = st->slot_origin[last_slot];   // combined origin = newest slot's (gvom.py:229)
     int32_t cx[CPW], cy[CPW];
     {
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_comb_tables(Geo g, const DevState *__re
                 const int32_t dz = uni((int32_t)(co[2] - so[2]));
                 const int zs = z + dz;
                 const bool zok = use && zin && zs >= 0 && zs < Z;
-                const int32_t *tab = slot_tables + (int64_t)(use ? s : 0) * V;
+                const int32_t *tab = slot_tables + (int64_t)cur_buf(flip, use ? s : 0) * V;
 #pragma unroll
                 for (int j = 0; j < CPW; ++j) {
                     const int32_t ux = cx[j] + dx, uy = cy[j] + dy;
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_comb_rows(Geo g, const DevState *__rest
     if (cw >= X) return;
     const int parity = uni(st->parity), cur = 1 - parity;
     const int have_last = uni(st->have_last);
-    const int last_slot = uni(st->last_slot);
+    const int last_slot = uni(st->last_slot), flip = uni(st->flip);
     const double *co = st->slot_origin[last_slot];   // combined origin = newest slot's (gvom.py:229)
     const int32_t ldx = have_last ? uni((int32_t)(co[0] - st->comb_origin[parity][0])) : 0;
     const int32_t ldy = have_last ? uni((int32_t)(co[1] - st->comb_origin[parity][1])) : 0;
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(256) void k_comb_rows(Geo g, const DevState *__rest
                     const bool zok = zin & ((uint32_t)zs < (uint32_t)Z);
                     // column j of the source: cp + j * Z (wrapping arithmetic; dereferenced
                     // only where the column is on the map)
-                    const int32_t *cp = slot_tables + (int64_t)s * V +
+                    const int32_t *cp = slot_tables + (int64_t)cur_buf(flip, s) * V +
                                         (int32_t)(((uint32_t)ux * (uint32_t)XY + (uint32_t)uy0) * (uint32_t)Z);
 #pragma unroll
                     for (int j = 0; j < CPW; ++j) {
@@ -537,7 +537,7 @@ __global__ __launch_bounds__(256) void k_comb_merge(Geo g, const DevState *__res
                                                     const uint2 *__restrict__ occ_list,
                                                     const int32_t *__restrict__ chunk_tot, int64_t nchunk,
                                                     double *__restrict__ height) {
-    const int parity = st->parity, cur = 1 - parity;
+    const int parity = st->parity, cur = 1 - parity, flip = st->flip;
     const double *co = st->comb_origin[cur];
     const int lane = threadIdx.x & 63;
     uint32_t C = 0;
@@ -565,7 +565,8 @@ __global__ __launch_bounds__(256) void k_comb_merge(Geo g, const DevState *__res
                 use[k] = s < g.B && ((mask >> s) & 1u);
                 const int32_t *addr = slot_tables;   // any valid word when the source is unused
                 if (use[k]) {
-                    SrcCol sc = source_column(slot_tables + (int64_t)s * g.V, co, st->slot_origin[s], x, y, g);
+                    SrcCol sc = source_column(slot_tables + (int64_t)cur_buf(flip, s) * g.V, co, st->slot_origin[s],
+                                              x, y, g);
                     addr = sc.col + z + sc.dz;
                 }
                 sid[k] = (uint32_t)*addr & MASK31;
@@ -573,7 +574,7 @@ __global__ __launch_bounds__(256) void k_comb_merge(Geo g, const DevState *__res
             SlotRec r[MERGE_GROUP];
 #pragma unroll
             for (int k = 0; k < MERGE_GROUP; ++k)
-                r[k] = slot_recs[(int64_t)(use[k] ? s0 + k : 0) * ncap + (use[k] ? sid[k] : 0)];
+                r[k] = slot_recs[(int64_t)cur_buf(flip, use[k] ? s0 + k : 0) * ncap + (use[k] ? sid[k] : 0)];
 #pragma unroll
             for (int k = 0; k < MERGE_GROUP; ++k)
                 if (use[k]) merge_record<double>(cm, hit, tot, mh, r[k].m, r[k].hit, r[k].total, r[k].min_h);

@@ -115,10 +115,11 @@ __device__ void min_eig(double xx, double xy, double xz, double yy, double yz, d
 // Per occupied voxel of the slot: lambda_min, eigenvector, mean, 2/n; block partial of
 // sum lambda_min.
 __global__ __launch_bounds__(RB) void k_diff_eigen(const DevState *__restrict__ st, int slot, const SlotRec *__restrict__ recs,
-                                                   DiffRec *__restrict__ dr, double *__restrict__ part) {
+                                                   int64_t ncap, DiffRec *__restrict__ dr, double *__restrict__ part) {
     __shared__ double sh[RB / 64];
     const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t C = st->slot_cells[slot];
+    recs += (int64_t)cur_buf(st->flip, slot) * ncap;
     double lam = 0.0;
     if (id < C) {
         const SlotRec &r = recs[id];
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(RB) void k_diff_columns(Geo g, const DevState *__re
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double k = 0.0, hsum = 0.0;
     if (c < g.X) {
-        const int32_t *col = table + (int64_t)slot * g.V + c * g.Z;
+        const int32_t *col = table + (int64_t)cur_buf(st->flip, slot) * g.V + c * g.Z;
         int32_t pt = -1;
         double h = 0.0;
         for (int z = 0; z < g.Z; ++z) {
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(RB) void k_diff_backward(Geo g, const DevState *__r
         if (f0 >= -g.ex - 1 && f0 <= g.XY + g.ex && f1 >= -g.ex - 1 && f1 <= g.XY + g.ex && f2 >= -g.ez - 1 &&
             f2 <= g.Z + g.ez) {
             const int64_t b0 = (int64_t)f0, b1 = (int64_t)f1, b2 = (int64_t)f2;
-            const int32_t *tab = table + (int64_t)slot * g.V;
+            const int32_t *tab = table + (int64_t)cur_buf(st->flip, slot) * g.V;
             for (int64_t ux = b0 - g.ex; ux <= b0 + g.ex; ++ux) {
                 if (ux < 0 || ux >= g.XY) continue;
                 for (int64_t uy = b1 - g.ex; uy <= b1 + g.ex; ++uy) {
@@ -321,7 +322,7 @@ hipError_t launch_diff_forward(const Geo &g, const DevState *st, int slot, const
     const unsigned ne = nb(ncap, RB), nc = nb(g.X, RB);
     hipError_t e = launch_fill_i32(pmark, -1, n > 0 ? n : 1, s);
     if (e != hipSuccess) return e;
-    k_diff_eigen<<<ne, RB, 0, s>>>(st, slot, recs, (DiffRec *)dr, part);
+    k_diff_eigen<<<ne, RB, 0, s>>>(st, slot, recs, ncap, (DiffRec *)dr, part);
     k_diff_sum<<<1, RB, 0, s>>>(part, ne, sums + 0);
     k_diff_columns<<<nc, RB, 0, s>>>(g, st, slot, table, seg_start, svals, world, colh, colpt, pmark, part,
                                      part + nc);

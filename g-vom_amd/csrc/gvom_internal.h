@@ -48,6 +48,12 @@ struct DevState {
     int32_t comb_total;    // cell count of the combine in flight
     int32_t comb_cells[2];
     int32_t slot_cells[GVOM_MAX_BUFFER + 2];
+    // Every ring slot has two physical table/record buffers, 2 slot and 2 slot + 1; bit slot of
+    // `flip` selects the one holding the slot's scan.  A scan stage writes the other one and
+    // its commit flips the bit, so a combine may still read the ring while the next scan
+    // stage runs.
+    int32_t flip;
+    int32_t scan_cells[GVOM_MAX_BATCH];      // in-map cells of each scan of the batch in flight
     int32_t first_seg[GVOM_MAX_BATCH];
     int32_t inb_end[GVOM_MAX_BATCH];
     int32_t scan_pos[GVOM_MAX_BATCH];        // first sorted position of each scan's valid keys
@@ -63,6 +69,13 @@ struct DevState {
     // stage for the combine that follows
     alignas(16) int32_t comb_shift[GVOM_MAX_BUFFER][4];
 };
+
+// Physical buffer of ring slot `slot`: the one holding its scan, and the one the scan stage in
+// flight writes.
+__host__ __device__ __forceinline__ int cur_buf(int32_t flip, int slot) { return 2 * slot + ((flip >> slot) & 1); }
+__host__ __device__ __forceinline__ int new_buf(int32_t flip, int slot) {
+    return 2 * slot + 1 - ((flip >> slot) & 1);
+}
 
 // One occupied voxel of a scan slot (reference: hit_count, total_count, metrics f64[10],
 // min_height, gvom.py:198-208, 1079-1085).

@@ -8,9 +8,11 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s);
 
 // ---- scan stage (gvom_scan.hip) ------------------------------------------------------
 // Transform, validity, own-voxel sort key per point (reference gvom.py:1122-1163).
+// With hist (hwords words, radix_hist_entries): the sort's first-pass counts are accumulated
+// into hist, which must be zero.
 hipError_t launch_prepare(int dtype, const void *in, void *world, uint32_t *keys, uint32_t *vals,
                           uint8_t *valid, const ScanDesc *sd, int S, int64_t NT, const Geo &g,
-                          DevState *st, uint8_t *rsec, hipStream_t s);
+                          DevState *st, uint8_t *rsec, uint32_t *hist, int64_t hwords, hipStream_t s);
 // Tiled ray-cast scratch (4 x 4 x 1 voxels per 64-byte segment): words per scan, and the
 // conversion into the slot tables (which also zeroes the scratch).
 int64_t tile_scratch_words(const Geo &g);
@@ -23,16 +25,19 @@ hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, co
                           int S, int64_t NT, const Geo &g, const DevState *st, int32_t *slot_tables,
                           int block, int rays_per_thread, int use_box, int tiled, int force_slot,
                           const uint32_t *order, hipStream_t s);
-// Stable LSD radix sort of (key, val) on the low `bits` key bits.  On return *kout/*vout
-// point at whichever buffer pair holds the result.
+// Stable LSD radix sort of (key, val) on the low `bits` (<= 32) key bits, 8-bit passes.
+// hist holds 2 x hwords per-tile digit counts: the first pass's filled by launch_prepare
+// (the last pass clears them for the next sort), the later passes' by a histogram kernel.
+// On return *kout/*vout point at whichever buffer pair holds the result.
 hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits,
-                      uint32_t *hist, uint32_t **kout, uint32_t **vout, hipStream_t s);
+                      uint32_t *hist, int64_t hwords, uint32_t **kout, uint32_t **vout, hipStream_t s);
 int64_t radix_hist_entries(int64_t n);
 // Segments of equal keys in the sorted array, per-scan bounds and slot cell counts.
 hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start, uint32_t *seg_key,
                            uint32_t *blk, int S, const Geo &g, DevState *st, hipStream_t s);
 // Per-segment moments (chunk partials, then per-segment sums) + slot table occupancy
-// marks + hit/total/min-height records.
+// marks + hit/total/min-height records.  All scan-stage writes to slot tables and records go
+// to the buffers new_buf(flip, slot) of the slots the batch fills.
 hipError_t launch_mom_part(int dtype, const void *world, const uint32_t *skeys, const uint32_t *svals,
                            int64_t seg_cap, const ScanDesc *sd, int S, const Geo &g, DevState *st, Moment *part,
                            hipStream_t s);
@@ -40,10 +45,12 @@ hipError_t launch_mom_seg(const int32_t *seg_start, const uint32_t *seg_key, int
                           DevState *st, int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap, const Moment *part,
                           Moment *mom, hipStream_t s);
 // Neighbourhood mean/covariance per occupied voxel (gvom.py:1250-1387).
-// Then the one-wave commit: slot origins, latest ego and ring indices of the S scans.
 hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t seg_cap, const Geo &g,
-                          DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap,
-                          const ScanDesc *sd, int S, hipStream_t s);
+                          DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap, int S,
+                          hipStream_t s);
+// One-wave commit of the S scans: slot origins, latest ego, buffer bits, cell counts, ring
+// indices and the next combine's source shifts.
+hipError_t launch_commit_process(DevState *st, const ScanDesc *sd, const Geo &g, int S, hipStream_t s);
 
 
 // ---- combine stage (gvom_combine.hip) ----------------------------------------------------

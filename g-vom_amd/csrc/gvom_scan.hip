@@ -91,15 +91,24 @@ __device__ __forceinline__ int ray_sector(double dx, double dy) {
     return s < 0 ? 0 : (s > 7 ? 7 : s);
 }
 
+// Radix sort tiles: RS_TILE keys per block of every pass.
+constexpr int RS_TILE = 2048;
+
+// With `hist` (the radix sort's first-pass histogram, hwords words, zero on entry: the
+// sort's last pass clears it for the next scan stage), the per-tile digit counts of the
+// first pass are accumulated here, so the sort starts with its first scatter.
 template <typename P>
 __global__ __launch_bounds__(256) void k_prepare(const P *__restrict__ in, P *__restrict__ world,
                                                  uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                  uint8_t *__restrict__ valid,
                                                  const ScanDesc *__restrict__ sd, int S, int64_t NT,
-                                                 Geo g, DevState *__restrict__ st, uint8_t *__restrict__ rsec) {
+                                                 Geo g, DevState *__restrict__ st, uint8_t *__restrict__ rsec,
+                                                 uint32_t *__restrict__ hist, int64_t hwords) {
     __shared__ int32_t scount[8];
+    __shared__ uint32_t lh[256];
     if (rsec && threadIdx.x < 8) scount[threadIdx.x] = 0;
-    if (rsec) __syncthreads();
+    if (hist) lh[threadIdx.x] = 0;
+    if (rsec || hist) __syncthreads();
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < NT) {
     int s = scan_of(sd, S, i);
@@ -130,6 +139,7 @@ __global__ __launch_bounds__(256) void k_prepare(const P *__restrict__ in, P *__
     }
     keys[i] = key;
     vals[i] = (uint32_t)i;
+    if (hist) atomicAdd(&lh[key & 0xFFu], 1u);
     if (rsec) {
         int sec = 255;
         if (ok) {
@@ -139,10 +149,11 @@ __global__ __launch_bounds__(256) void k_prepare(const P *__restrict__ in, P *__
         rsec[i] = (uint8_t)sec;
     }
     }
-    if (rsec) {
-        __syncthreads();
-        if (threadIdx.x < 8 && scount[threadIdx.x]) atomicAdd(&st->sec_count[threadIdx.x], scount[threadIdx.x]);
-    }
+    if (rsec || hist) __syncthreads();
+    if (rsec && threadIdx.x < 8 && scount[threadIdx.x]) atomicAdd(&st->sec_count[threadIdx.x], scount[threadIdx.x]);
+    // the block's 256 points lie in one sort tile (256 divides RS_TILE)
+    if (hist && lh[threadIdx.x])
+        atomicAdd(&hist[((int64_t)blockIdx.x * 256 / RS_TILE) * 256 + threadIdx.x], lh[threadIdx.x]);
 }
 
 // Groups valid rays by azimuth sector: each block appends its rays of sector s, in point
@@ -331,10 +342,11 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
     }
 #define RAY_INDEX(j) (order ? ((j) < o_len ? (int64_t)order[o_beg + (j)] : NT) : (j))
     const int lane = threadIdx.x & 63;
-    const int next = st->next_slot;
+    const int next = st->next_slot, flip = st->flip;
     const TileGeo tg = tile_geo(g);
-    // force_slot >= 0 (autotuning) writes every scan into table force_slot instead of its ring slot
-#define RAY_SLOT(s) (force_slot >= 0 ? force_slot : ((next + (s)) % g.B))
+    // the table the scan stage writes for the batch's s-th scan (force_slot >= 0, autotuning:
+    // table force_slot for every scan)
+#define RAY_SLOT(s) (force_slot >= 0 ? force_slot : new_buf(flip, (next + (s)) % g.B))
     // the block's box: scan of its first ray, centred on that scan's ego voxel
     int bs = 0;
     int64_t bx0 = 0, by0 = 0, bz0 = 0;
@@ -406,8 +418,7 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
 }
 
 // ------------------------------------------------------------------------------------
-// Radix sort: 8-bit digits, tiles of 2048 keys (256 threads x 8).
-constexpr int RS_TILE = 2048;
+// Radix sort: 8-bit digits, tiles of RS_TILE keys (defined with k_prepare).
 // Points per moment chunk (one wavefront, see k_mom_part).
 constexpr int MCH = 64;
 
@@ -432,11 +443,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t *__restri
     if (threadIdx.x < 256) hist[(int64_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];   // block-major
 }
 
+// hist: this pass's per-tile digit counts (block-major).  hclear (last pass only) is the
+// first pass's histogram, cleared for the next sort.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *__restrict__ kin,
                                                            const uint32_t *__restrict__ vin,
                                                            uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                            int64_t n, int shift, const uint32_t *__restrict__ hist,
-                                                           int nblk) {
+                                                           int nblk, uint32_t *__restrict__ hclear, int64_t hwords) {
     __shared__ uint32_t wcnt[RS_WAVES][256];
     __shared__ uint32_t ptot[4][256], pbef[4][256];
     __shared__ uint32_t gbase[256];
@@ -526,6 +539,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *__res
             vout[pos] = vv[r];
         }
     }
+    if (hclear)
+        for (int64_t k = (int64_t)blockIdx.x * RS_THREADS + threadIdx.x; k < hwords;
+             k += (int64_t)gridDim.x * RS_THREADS)
+            hclear[k] = 0;
 }
 
 // ------------------------------------------------------------------------------------
@@ -571,10 +588,50 @@ __global__ __launch_bounds__(256) void k_seg_count(const uint32_t *__restrict__ 
     if (threadIdx.x < W) blk[(int64_t)blockIdx.x * W + threadIdx.x] = c[threadIdx.x];
 }
 
+// Totals of the per-block counts -> nseg, nvalid, end sentinel of the last segment,
+// per-scan [first segment, end of in-map segments) and slot cell counts, by one block
+// (blockDim >= 64).  Segments are ordered by scan, and within a scan in-map keys precede
+// halo keys.
+__device__ void seg_bounds_body(const uint32_t *__restrict__ blk, int nblk, int S, const Geo &g,
+                                int32_t *__restrict__ seg_start, DevState *__restrict__ st) {
+    __shared__ uint32_t tot[2 + 3 * GVOM_MAX_BATCH];
+    const int t = threadIdx.x;
+    const int W = 2 + 3 * S;
+    for (int col = t; col < W; col += blockDim.x) {
+        uint32_t a = 0;
+#pragma unroll 8
+        for (int b = 0; b < nblk; ++b) a += blk[(int64_t)b * W + col];
+        tot[col] = a;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    st->nseg = (int32_t)tot[0];
+    st->nvalid = (int32_t)tot[1];
+    seg_start[tot[0]] = (int32_t)tot[1];
+    int32_t f = 0, pos = 0, ch = 0;
+    for (int s = 0; s < S; ++s) {
+        const int32_t in = (int32_t)tot[2 + 3 * s], halo = (int32_t)tot[3 + 3 * s];
+        const int32_t nv = S > 1 ? (int32_t)tot[4 + 3 * s] : (int32_t)tot[1];
+        st->first_seg[s] = f;
+        st->inb_end[s] = f + in;
+        st->scan_cells[s] = in;
+        st->scan_pos[s] = pos;
+        st->scan_valid[s] = nv;
+        st->chunk_off[s] = ch;
+        f += in + halo;
+        pos += nv;
+        ch += (nv + MCH - 1) / MCH;
+    }
+    st->chunk_off[S] = ch;
+}
+
+// Segment heads in sorted order; the last block then also derives the totals (the per-block
+// counts were complete before this kernel started).
 __global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *__restrict__ k, int64_t n,
-                                                     const uint32_t *__restrict__ blk, int W,
+                                                     const uint32_t *__restrict__ blk, int S, Geo g,
                                                      int32_t *__restrict__ seg_start,
-                                                     uint32_t *__restrict__ seg_key) {
+                                                     uint32_t *__restrict__ seg_key, DevState *__restrict__ st) {
+    const int W = 2 + 3 * S;
     __shared__ uint32_t wsum[4];
     __shared__ uint32_t wpre[4];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -610,43 +667,10 @@ __global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *__restrict_
             ++r;
         }
     }
-}
-
-// Totals of the per-block counts -> nseg, nvalid, end sentinel of the last segment,
-// per-scan [first segment, end of in-map segments) and slot cell counts, by one block
-// (blockDim >= 64).  Segments are ordered by scan, and within a scan in-map keys precede
-// halo keys.
-__device__ void seg_bounds_body(const uint32_t *__restrict__ blk, int nblk, int S, const Geo &g,
-                                int32_t *__restrict__ seg_start, DevState *__restrict__ st) {
-    __shared__ uint32_t tot[2 + 3 * GVOM_MAX_BATCH];
-    const int t = threadIdx.x;
-    const int W = 2 + 3 * S;
-    for (int col = t; col < W; col += blockDim.x) {
-        uint32_t a = 0;
-#pragma unroll 8
-        for (int b = 0; b < nblk; ++b) a += blk[(int64_t)b * W + col];
-        tot[col] = a;
+    if (blockIdx.x == gridDim.x - 1) {
+        __syncthreads();
+        seg_bounds_body(blk, (int)gridDim.x, S, g, seg_start, st);
     }
-    __syncthreads();
-    if (t != 0) return;
-    st->nseg = (int32_t)tot[0];
-    st->nvalid = (int32_t)tot[1];
-    seg_start[tot[0]] = (int32_t)tot[1];
-    int32_t f = 0, pos = 0, ch = 0;
-    for (int s = 0; s < S; ++s) {
-        const int32_t in = (int32_t)tot[2 + 3 * s], halo = (int32_t)tot[3 + 3 * s];
-        const int32_t nv = S > 1 ? (int32_t)tot[4 + 3 * s] : (int32_t)tot[1];
-        st->first_seg[s] = f;
-        st->inb_end[s] = f + in;
-        st->slot_cells[(st->next_slot + s) % g.B] = in;
-        st->scan_pos[s] = pos;
-        st->scan_valid[s] = nv;
-        st->chunk_off[s] = ch;
-        f += in + halo;
-        pos += nv;
-        ch += (nv + MCH - 1) / MCH;
-    }
-    st->chunk_off[S] = ch;
 }
 
 __global__ __launch_bounds__(64) void k_seg_bounds(const uint32_t *__restrict__ blk, int nblk, int S, Geo g,
@@ -809,12 +833,12 @@ __global__ __launch_bounds__(256) void k_mom_seg(const int32_t *__restrict__ seg
     const uint32_t vk = key & ((1u << g.VB) - 1u);
     if (!((vk >> (g.VB - 1)) & 1u)) {
         const int s = (int)(key >> g.VB);
-        const int slot = (st->next_slot + s) % g.B;
+        const int buf = new_buf(st->flip, (st->next_slot + s) % g.B);
         const int32_t id = (int32_t)(gi - st->first_seg[s]);
-        int32_t *tv = tables + (int64_t)slot * g.V + vk;
+        int32_t *tv = tables + (int64_t)buf * g.V + vk;
         const int32_t passes = *tv;
         *tv = (int32_t)(GVOM_OCC | (uint32_t)id);
-        SlotRec *r = recs + (int64_t)slot * ncap + id;
+        SlotRec *r = recs + (int64_t)buf * ncap + id;
         r->hit = n;
         r->total = (int32_t)((uint32_t)passes + (uint32_t)n);
         r->min_h = mh;
@@ -857,10 +881,10 @@ __device__ __forceinline__ double red32(double v) {
 }
 
 // Ring bookkeeping after the scan stage of S scans (gvom.py:211-216), by one thread: slot
-// origins for combine (gvom.py:208), the ego the height map uses (gvom.py:110-112), then the
-// ring indices.  Everything the combine reads about the ring is written here, at the end of
-// the scan stage, so that the stage's earlier kernels may run while a combine still reads
-// the previous state.
+// origins for combine (gvom.py:208), the ego the height map uses (gvom.py:110-112), the
+// written slots' buffer bits and cell counts, then the ring indices.  Everything the combine
+// reads about the ring is written here, after the scan stage, so that the stage's kernels
+// may run while a combine still reads the previous state.
 __device__ __forceinline__ void commit_process(DevState *st, const ScanDesc *sd, const Geo &g, int S) {
     const int B = g.B;
     int next = st->next_slot;
@@ -875,9 +899,15 @@ __device__ __forceinline__ void commit_process(DevState *st, const ScanDesc *sd,
     st->latest_ego[0] = sd[S - 1].ego[0];
     st->latest_ego[1] = sd[S - 1].ego[1];
     st->latest_ego[2] = sd[S - 1].ego[2];
-    int mask = st->filled_mask;
-    for (int s = 0; s < S; ++s) mask |= 1 << ((next + s) % B);
+    int mask = st->filled_mask, flip = st->flip;
+    for (int s = 0; s < S; ++s) {
+        const int slot = (next + s) % B;
+        mask |= 1 << slot;
+        flip ^= 1 << slot;   // the buffer the stage wrote now holds the slot
+        st->slot_cells[slot] = st->scan_cells[s];
+    }
     st->filled_mask = mask;
+    st->flip = flip;
     const int last = (next + S - 1) % B;
     st->last_slot = last;
     st->next_slot = (next + S) % B;
@@ -908,9 +938,9 @@ __global__ __launch_bounds__(256) void k_metrics(const uint32_t *__restrict__ se
         const uint32_t vk = key & ((1u << g.VB) - 1u);
         if ((vk >> (g.VB - 1)) & 1u) continue;   // halo segment: no record (uniform in the group)
         const int s = (int)(key >> g.VB);
-        const int slot = (st->next_slot + s) % g.B;
+        const int buf = new_buf(st->flip, (st->next_slot + s) % g.B);
         const int32_t fs = st->first_seg[s], ie = st->inb_end[s];
-        const int32_t *table = tables + (int64_t)slot * g.V;
+        const int32_t *table = tables + (int64_t)buf * g.V;
         const uint32_t Z = (uint32_t)g.Z, XY = (uint32_t)g.XY;
         const uint32_t uc = vk / Z;
         const int64_t uz = vk - uc * Z, ux = uc / XY, uy = uc - (uc / XY) * XY;
@@ -971,7 +1001,7 @@ __global__ __launch_bounds__(256) void k_metrics(const uint32_t *__restrict__ se
         cxx = red32(cxx); cxy = red32(cxy); cxz = red32(cxz);
         cyy = red32(cyy); cyz = red32(cyz); czz = red32(czz);
         if (l32 == 0) {
-            SlotRec *r = recs + (int64_t)slot * ncap + (gi - fs);
+            SlotRec *r = recs + (int64_t)buf * ncap + (gi - fs);
             r->m[0] = m0;
             r->m[1] = m1;
             r->m[2] = m2;
@@ -1001,8 +1031,8 @@ __global__ __launch_bounds__(256) void k_clear_slots(int32_t *__restrict__ table
     const int s = blockIdx.y;
     if (blockIdx.x == 0 && s == 0 && threadIdx.x < 16)
         (&const_cast<DevState *>(st)->sec_count[0])[threadIdx.x] = 0;   // sec_count and sec_cursor
-    const int slot = __builtin_amdgcn_readfirstlane((st->next_slot + s) % B);
-    int32_t *t = tables + (int64_t)slot * V;
+    const int buf = __builtin_amdgcn_readfirstlane(new_buf(st->flip, (st->next_slot + s) % B));
+    int32_t *t = tables + (int64_t)buf * V;
     const int32_t base = (int32_t)blockIdx.x * 4096;
     if ((V & 3) == 0) {
 #pragma unroll
@@ -1049,14 +1079,14 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s) {
 
 hipError_t launch_prepare(int dtype, const void *in, void *world, uint32_t *keys, uint32_t *vals,
                           uint8_t *valid, const ScanDesc *sd, int S, int64_t NT, const Geo &g,
-                          DevState *st, uint8_t *rsec, hipStream_t s) {
+                          DevState *st, uint8_t *rsec, uint32_t *hist, int64_t hwords, hipStream_t s) {
     unsigned nb = nblocks(NT, 256);
     if (dtype == GVOM_F32)
         k_prepare<float><<<nb, 256, 0, s>>>((const float *)in, (float *)world, keys, vals, valid, sd, S, NT, g, st,
-                                             rsec);
+                                             rsec, hist, hwords);
     else
         k_prepare<double><<<nb, 256, 0, s>>>((const double *)in, (double *)world, keys, vals, valid, sd, S, NT, g, st,
-                                              rsec);
+                                              rsec, hist, hwords);
     return hipGetLastError();
 }
 
@@ -1119,8 +1149,8 @@ __global__ __launch_bounds__(256) void k_tile_convert(Geo g, const DevState *__r
         d[3] = v.w;
     }
     __syncthreads();
-    const int slot = force_slot >= 0 ? force_slot : (st->next_slot + s) % g.B;
-    int32_t *tab = tables + (int64_t)slot * g.V;
+    const int buf = force_slot >= 0 ? force_slot : new_buf(st->flip, (st->next_slot + s) % g.B);
+    int32_t *tab = tables + (int64_t)buf * g.V;
     // wavefront a writes x-row a: columns (x, y0 .. y0 + ny - 1), ny * Z contiguous words
     const int a = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int x = txi * 4 + a;
@@ -1155,15 +1185,24 @@ int64_t tile_scratch_words(const Geo &g) { return tile_geo(g).vt; }
 int64_t radix_hist_entries(int64_t n) { return 256 * (int64_t)nblocks(n, RS_TILE) + 256; }
 
 hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits,
-                      uint32_t *hist, uint32_t **kout, uint32_t **vout, hipStream_t s) {
+                      uint32_t *hist, int64_t hwords, uint32_t **kout, uint32_t **vout, hipStream_t s) {
     *kout = k0;
     *vout = v0;
     if (n <= 0) return hipSuccess;
+    if (bits > 32) return hipErrorInvalidValue;
     int nblk = (int)nblocks(n, RS_TILE);
     uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
-    for (int shift = 0; shift < bits; shift += 8) {
-        k_rs_hist<<<nblk, RS_THREADS, 0, s>>>(ka, n, shift, hist);
-        k_rs_scatter<<<nblk, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, hist, nblk);
+    for (int shift = 0, p = 0; shift < bits; shift += 8, ++p) {
+        // pass 0 reads the first histogram (from launch_prepare), later passes the second
+        uint32_t *hp = p == 0 ? hist : hist + hwords;
+        const bool last = shift + 8 >= bits;
+        if (p > 0) k_rs_hist<<<nblk, RS_THREADS, 0, s>>>(ka, n, shift, hp);
+        k_rs_scatter<<<nblk, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, hp, nblk, last && p > 0 ? hist : nullptr,
+                                                  hwords);
+        if (last && p == 0) {
+            hipError_t e = launch_fill_i32((int32_t *)hist, 0, hwords, s);
+            if (e != hipSuccess) return e;
+        }
         uint32_t *tk = ka; ka = kb; kb = tk;
         uint32_t *tv = va; va = vb; vb = tv;
     }
@@ -1175,12 +1214,12 @@ hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, in
 hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start, uint32_t *seg_key,
                            uint32_t *blk, int S, const Geo &g, DevState *st, hipStream_t s) {
     int nblk = n > 0 ? (int)nblocks(n, RS_TILE) : 0;
-    const int W = 2 + 3 * S;
     if (n > 0) {
         k_seg_count<<<nblk, 256, 0, s>>>(skeys, n, S, g, blk);
-        k_seg_compact<<<nblk, 256, 0, s>>>(skeys, n, blk, W, seg_start, seg_key);
+        k_seg_compact<<<nblk, 256, 0, s>>>(skeys, n, blk, S, g, seg_start, seg_key, st);
+    } else {
+        k_seg_bounds<<<1, 64, 0, s>>>(blk, nblk, S, g, seg_start, st);
     }
-    k_seg_bounds<<<1, 64, 0, s>>>(blk, nblk, S, g, seg_start, st);
     return hipGetLastError();
 }
 
@@ -1206,14 +1245,17 @@ hipError_t launch_mom_seg(const int32_t *seg_start, const uint32_t *seg_key, int
 }
 
 hipError_t launch_metrics(const uint32_t *seg_key, const Moment *mom, int64_t seg_cap, const Geo &g,
-                          DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap,
-                          const ScanDesc *sd, int S, hipStream_t s) {
-    // grid-stride: enough 32-lane groups to fill the chip, never more than segments can need;
-    // then the ring bookkeeping (one thread)
+                          DevState *st, const int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap, int S,
+                          hipStream_t s) {
+    // grid-stride: enough 32-lane groups to fill the chip, never more than segments can need
     int64_t nb = nblocks(seg_cap * 32, 256);
     if (nb > 2048) nb = 2048;
     if (nb < 1) nb = 1;
     k_metrics<<<(unsigned)nb, 256, 0, s>>>(seg_key, mom, g, st, slot_tables, slot_recs, ncap, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_commit_process(DevState *st, const ScanDesc *sd, const Geo &g, int S, hipStream_t s) {
     k_commit_process<<<1, 64, 0, s>>>(st, sd, g, S);
     return hipGetLastError();
 }

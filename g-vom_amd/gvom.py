@@ -224,9 +224,8 @@ class Gvom:
         nat.check(self._lib.gvom_set_raycast_config(self._h, int(block), int(rays_per_thread), flags), self._h)
 
     def set_overlap(self, enabled=True):
-        """Let a device-pointer process call with the tiled ray cast start its transform,
-        sort and ray cast while the preceding combine_maps_device still runs (off by
-        default).  Its points must be complete when the call is made."""
+        """Let a device-pointer process call run while the preceding combine_maps_device
+        still runs (off by default).  Its points must be complete when the call is made."""
         nat.check(self._lib.gvom_set_overlap(self._h, 1 if enabled else 0), self._h)
 
     # -- hipGraph capture ------------------------------------------------------------

@@ -229,14 +229,15 @@ int gvom_stage_times(gvom_handle *h, float *ms, int32_t n);
  * against it. */
 void *gvom_stream(gvom_handle *h);
 
-/* Scan/combine overlap (no reference counterpart; off by default).  When enabled, a
+/* Scan/combine overlap (no reference counterpart; off by default).  Every ring slot has two
+ * physical buffers and a scan stage writes the one the ring does not hold, so when enabled a
  * device-pointer scan stage (gvom_process_pointcloud_device, gvom_process_batch with
- * on_device) with the tiled ray cast starts its transform, key sort and ray cast on a
- * second stream as soon as the previous scan stage has finished, concurrently with a
- * gvom_combine_maps_device enqueued in between; the rest of the stage, and everything else,
- * stays ordered on the handle's stream.  The caller's points must therefore be complete
- * when the process call is made (not merely enqueued on the handle's stream after the
- * preceding combine).  Also applies inside a hipGraph capture.  Fails while capturing. */
+ * on_device) runs on a second stream as soon as the previous scan stage has finished,
+ * concurrently with a gvom_combine_maps_device enqueued in between; only its commit (ring
+ * state, buffer selection) and everything else stay ordered on the handle's stream.  The
+ * caller's points must therefore be complete when the process call is made (not merely
+ * enqueued on the handle's stream after the preceding combine).  Also applies inside a
+ * hipGraph capture.  Fails while capturing. */
 int gvom_set_overlap(gvom_handle *h, int32_t enabled);
 
 /* hipGraph capture of a launch sequence (no reference counterpart).
