@@ -137,7 +137,7 @@ def test_graph_replay_matches_eager(cfg4):
         print(f"graph replay {r} ok", flush=True)
 
 
-@pytest.mark.parametrize("flags", [4, 6, 1, 2])
+@pytest.mark.parametrize("flags", [4, 6, 1, 2, 3])
 def test_ray_cast_variants_equal(cfg4, flags):
     """Every ray-cast variant the autotuner chooses from (tiled scratch, sector order, LDS
     box) leaves slot tables and records bit-identical to the plain cast, for single scans
@@ -219,3 +219,14 @@ def test_overlap_matches_serial():
             frame(serial, k, r * shift)
         compare(f"replay {r}")
         print(f"overlap graph replay {r}: identical", flush=True)
+    # two-scan batches back to back
+    flat = nat.DeviceArray(np.concatenate([scans[0].points, scans[1].points]))
+    off = [0, scans[0].points.shape[0], scans[0].points.shape[0] + scans[1].points.shape[0]]
+    egos = np.stack([scans[0].ego, scans[1].ego])
+    Ts = np.stack([scans[0].transform, scans[1].transform])
+    for k in range(3):
+        for g in (serial, ovl):
+            g.process_batch_device(flat, off, egos + k * shift, Ts)
+            g.combine_maps_device()
+    compare("batches")
+    print("overlap batches: identical", flush=True)
