@@ -50,6 +50,13 @@ def test_cfg2_four_frames():
     _run_and_compare(wl.map_args, wl.steps, "cfg2")
 
 
+def test_cfg3_large_map():
+    """512 x 512 x 64 voxels (16.8M per table), 236k-point 128 x 2048 scans, buffer 8, the
+    sensor moving and turning: three frames, every slot, combined map and 2D map checked."""
+    wl = synth.workload(3, frames=3)
+    _run_and_compare(wl.map_args, wl.steps, "cfg3")
+
+
 def test_f64_points_and_transform():
     wl = synth.workload(1)
     s = wl.steps[0][0]
