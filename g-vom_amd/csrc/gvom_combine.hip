@@ -617,6 +617,9 @@ __global__ void k_commit_combine(DevState *st, const int32_t *__restrict__ chunk
 constexpr int TILE = 16;
 constexpr int HALO = 15;
 constexpr int LT = TILE + 2 * HALO;   // 46 <= 64: one 64-bit mask per tile row / column
+constexpr int TT = TILE * TILE;       // cells per tile
+constexpr int MT = 2 * TT;            // threads per block: two per cell
+static_assert(MT % 64 == 0 && MT <= 1024 && LT <= 64, "whole wavefronts, one 64-bit mask per tile row");
 
 // First set bit of m within [lo, hi) (bit positions), or -1.
 __device__ __forceinline__ int first_in(uint64_t m, int lo, int hi) {
@@ -626,11 +629,11 @@ __device__ __forceinline__ int first_in(uint64_t m, int lo, int hi) {
     return r ? lo + __builtin_ctzll(r) : -1;
 }
 
-// One block of 512 threads per 16 x 16 tile: threads 0-255 run the slope fit and the guessed
-// height of their column, threads 256-511 walk the same columns' positive-obstacle windows
-// (dependent table and record loads), so the two latency chains overlap; the first half
-// then writes the maps.
-__global__ __launch_bounds__(512) void k_maps2d(Geo g, const DevState *__restrict__ st,
+// One block of MT = 2 TILE^2 threads per TILE x TILE tile: the first TILE^2 threads run the
+// slope fit and the guessed height of their column, the others walk the same columns'
+// positive-obstacle windows (dependent table and record loads), so the two latency chains
+// overlap; the first half then writes the maps.
+__global__ __launch_bounds__(MT) void k_maps2d(Geo g, const DevState *__restrict__ st,
                                                 const double *__restrict__ height,
                                                 const double *__restrict__ inferred,
                                                 const int32_t *__restrict__ comb_tables,
@@ -642,10 +645,10 @@ __global__ __launch_bounds__(512) void k_maps2d(Geo g, const DevState *__restric
     __shared__ double H[LT][LT];
     __shared__ uint64_t rowm[LT];   // bit ly set when H[lx][ly] is observed (rows: fixed x)
     __shared__ uint64_t colm[LT];   // bit lx set when H[lx][ly] is observed (columns: fixed y)
-    __shared__ int32_t winpv[TILE * TILE];   // window density result per column of the tile
+    __shared__ int32_t winpv[TT];   // window density result per column of the tile
     const int XY = g.XY;
     const int bx = blockIdx.y * TILE, by = blockIdx.x * TILE;   // tile origin (x rows, y fastest)
-    for (int k = threadIdx.x; k < LT * LT; k += 512) {
+    for (int k = threadIdx.x; k < LT * LT; k += MT) {
         int lx = k / LT, ly = k % LT;
         int gx = bx - HALO + lx, gy = by - HALO + ly;
         H[lx][ly] = (gx >= 0 && gx < XY && gy >= 0 && gy < XY) ? height[(int64_t)gx * XY + gy] : -1000.0;
@@ -654,7 +657,7 @@ __global__ __launch_bounds__(512) void k_maps2d(Geo g, const DevState *__restric
     {
         // observed masks by ballot: one row and one column per wavefront and round
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        for (int r = wv; r < LT; r += 8) {
+        for (int r = wv; r < LT; r += MT / 64) {
             bool ro = false, co = false;
             if (lane < LT) {
                 ro = H[r][lane] > -1000;
@@ -668,7 +671,7 @@ __global__ __launch_bounds__(512) void k_maps2d(Geo g, const DevState *__restric
         }
     }
     __syncthreads();
-    const int cell = threadIdx.x & 255, half = threadIdx.x >> 8;
+    const int cell = threadIdx.x % TT, half = threadIdx.x / TT;
     const int tx = cell / TILE, ty = cell % TILE;
     const int x0 = bx + tx, y0 = by + ty;
     const bool act = x0 < XY && y0 < XY;
@@ -967,7 +970,7 @@ hipError_t launch_maps2d(const Geo &g, const DevState *st, const double *height,
                          double *slope_y, double *rough, double *delta, int32_t *pos, int32_t *neg, int32_t *vis,
                          hipStream_t s) {
     dim3 grid((unsigned)((g.XY + TILE - 1) / TILE), (unsigned)((g.XY + TILE - 1) / TILE));
-    k_maps2d<<<grid, 512, 0, s>>>(g, st, height, inferred, comb_tables, comb_recs, ccap, slope_x, slope_y, rough,
+    k_maps2d<<<grid, MT, 0, s>>>(g, st, height, inferred, comb_tables, comb_recs, ccap, slope_x, slope_y, rough,
                                   delta, pos, neg, vis);
     return hipGetLastError();
 }
