@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) void k_comb_tables(Geo g, const DevState *__re
 // zero page instead of being masked after the load, so no per-word predicate outlives its
 // address.  Each wavefront handles GPW groups.  (The general kernel above issues ~1200
 // scalar instructions per wavefront, which made it scalar-issue bound.)
-constexpr int GPW = 2;
+constexpr int GPW = 1;   // measured: 1 group per wavefront 34.4 us, 2 groups 35.3 us, 4 groups 36.8 us
 constexpr int ZERO_PAGE_WORDS = 4 * 256 + 64;   // covers lane + j * z_size for z_size <= 256
 
 // combine rule of one voxel: union of the sources' occupancy and sum of their passes, then
