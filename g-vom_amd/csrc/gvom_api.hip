@@ -742,6 +742,33 @@ extern "C" int gvom_occupancy_grids(gvom_handle *h, double density_threshold, do
     return GVOM_OK;
 }
 
+// ---- image kernels the reference defines but never calls (gvom.py:807-885) ------------------
+extern "C" int gvom_expand_binary(gvom_handle *h, const int32_t *in, int32_t *out, int32_t xy_size, double r) {
+    if (!h || !in || !out || xy_size < 1) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, launch_expand_binary(in, out, xy_size, r, h->stream));
+    return GVOM_OK;
+}
+
+extern "C" int gvom_lowpass_binary(gvom_handle *h, const int32_t *in, int32_t *out, int32_t xy_size,
+                                   int32_t filter_size, double filter_fraction) {
+    if (!h || !in || !out || xy_size < 1 || filter_size < 0) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, launch_lowpass_binary(in, out, xy_size, filter_size, filter_fraction, h->stream));
+    return GVOM_OK;
+}
+
+extern "C" int gvom_convolve(gvom_handle *h, const double *in, const double *kernel, double *out, int32_t xy_size,
+                             int32_t kernel_size) {
+    if (!h || !in || !kernel || !out || xy_size < 1 || kernel_size < 1 || kernel_size % 2 == 0) {
+        if (h) h->err = "convolve needs device buffers and an odd kernel_size";
+        return GVOM_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, launch_convolve(in, kernel, out, xy_size, kernel_size, h->stream));
+    return GVOM_OK;
+}
+
 extern "C" int gvom_probe_atomics(gvom_handle *h, int32_t pattern, int64_t n_atomics, float *ms) {
     if (!h || !ms || pattern < 0 || pattern > 5 || n_atomics <= 0) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);

@@ -86,3 +86,10 @@ hipError_t launch_debug_height(const Geo &g, const DevState *st, int parity, con
                                const double *rough, const double *sx, const double *sy,
                                const double *delta, float *out7, float *out3, hipStream_t s);
 
+
+// ---- image kernels the reference defines but never calls (gvom_image.hip) -------------------
+hipError_t launch_expand_binary(const int32_t *in, int32_t *out, int XY, double r, hipStream_t s);
+hipError_t launch_lowpass_binary(const int32_t *in, int32_t *out, int XY, int filter_size, double fraction,
+                                 hipStream_t s);
+hipError_t launch_convolve(const double *in, const double *kern, double *out, int XY, int kernel_size,
+                           hipStream_t s);

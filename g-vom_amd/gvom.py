@@ -380,6 +380,43 @@ class Gvom:
         nat.check(self._lib.gvom_stage_times(self._h, ms, 9), self._h)
         return list(ms)
 
+    # -- gvom.py:807-885: image kernels the reference defines but never calls -----------
+    def expand_binary(self, img, r):
+        """__expand_binary (gvom.py:807-832): disk dilation of a 0/1 int32 map, radius r."""
+        img = np.ascontiguousarray(np.asarray(img, dtype=np.int32))
+        X = self.xy_size
+        if img.shape != (X, X):
+            raise ValueError(f"image must be {X} x {X}")
+        din, dout = nat.DeviceArray(img), nat.DeviceArray(np.zeros_like(img))
+        nat.check(self._lib.gvom_expand_binary(self._h, din.ptr, dout.ptr, X, float(r)), self._h)
+        self.sync()
+        return dout.to_host()
+
+    def lowpass_binary(self, img, filter_size, filter_fraction):
+        """__lowpass_binary (gvom.py:834-862) on a 0/1 int32 map."""
+        img = np.ascontiguousarray(np.asarray(img, dtype=np.int32))
+        X = self.xy_size
+        if img.shape != (X, X):
+            raise ValueError(f"image must be {X} x {X}")
+        din, dout = nat.DeviceArray(img), nat.DeviceArray(np.zeros_like(img))
+        nat.check(self._lib.gvom_lowpass_binary(self._h, din.ptr, dout.ptr, X, int(filter_size),
+                                                float(filter_fraction)), self._h)
+        self.sync()
+        return dout.to_host()
+
+    def convolve(self, img, kernel):
+        """__convolve (gvom.py:864-885) with its arithmetic as written (every term reads
+        img[x + r, y + r]); kernel: odd k x k float64."""
+        img = np.ascontiguousarray(np.asarray(img, dtype=np.float64))
+        kernel = np.ascontiguousarray(np.asarray(kernel, dtype=np.float64))
+        X = self.xy_size
+        if img.shape != (X, X) or kernel.ndim != 2 or kernel.shape[0] != kernel.shape[1]:
+            raise ValueError(f"image must be {X} x {X} and the kernel square")
+        din, dk, dout = nat.DeviceArray(img), nat.DeviceArray(kernel), nat.DeviceArray(np.zeros_like(img))
+        nat.check(self._lib.gvom_convolve(self._h, din.ptr, dk.ptr, dout.ptr, X, kernel.shape[0]), self._h)
+        self.sync()
+        return dout.to_host()
+
     def sync(self):
         nat.check(self._lib.gvom_sync(self._h), self._h)
 

@@ -49,6 +49,9 @@ SIGNATURES = {
     "gvom_stage_times": (ctypes.c_int, [_P, _P, _I32]),
     "gvom_stream": (ctypes.c_void_p, [_P]),
     "gvom_set_overlap": (ctypes.c_int, [_P, _I32]),
+    "gvom_expand_binary": (ctypes.c_int, [_P, _P, _P, _I32, ctypes.c_double]),
+    "gvom_lowpass_binary": (ctypes.c_int, [_P, _P, _P, _I32, _I32, ctypes.c_double]),
+    "gvom_convolve": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32]),
     "gvom_diff_forward": (ctypes.c_int, [_P, _P, _I64, _P, _P, ctypes.c_double, _P]),
     "gvom_graph_begin": (ctypes.c_int, [_P, _I32]),
     "gvom_graph_end": (ctypes.c_int, [_P, _P]),
@@ -153,6 +156,17 @@ class DeviceArray:
             rc = hip.hipMemcpy(self.ptr, self.host.ctypes.data_as(ctypes.c_void_p), self.nbytes, 1)
             if rc != 0:
                 raise GvomError(f"hipMemcpy failed ({rc})")
+
+    def to_host(self):
+        """A host copy of the buffer's current contents (synchronising hipMemcpy), shaped and
+        typed like the array it was made from."""
+        import numpy as np
+        out = np.empty_like(self.host)
+        if self.nbytes:
+            rc = _hip_runtime().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes, 2)
+            if rc != 0:
+                raise GvomError(f"hipMemcpy failed ({rc})")
+        return out
 
     def free(self):
         if self.ptr and self.ptr.value:

@@ -164,6 +164,22 @@ int gvom_process_pointcloud2(gvom_handle *h, const void *data, const gvom_cloud_
 int gvom_occupancy_grids(gvom_handle *h, double density_threshold, double min_roughness, double max_roughness,
                          int8_t *out, int32_t on_device, double origin_m[3]);
 
+/* The reference's 2D image kernels that its pipeline never calls, on xy_size x xy_size
+ * [x, y] row-major device buffers, enqueued on the handle's stream (no synchronisation):
+ *   gvom_expand_binary  replaces Gvom.__expand_binary  (gvom.py:807-832): disk dilation of a
+ *                       0/1 map, radius r (row offsets |i| <= floor(r), |j| <= floor(sqrt(r^2 - i^2)));
+ *   gvom_lowpass_binary replaces Gvom.__lowpass_binary (gvom.py:834-862): 1 where the input is
+ *                       non-zero and its (2 filter_size + 1)^2 in-map window mean >= filter_fraction;
+ *   gvom_convolve       replaces Gvom.__convolve       (gvom.py:864-885) with its arithmetic as
+ *                       written (every term reads input[x + r, y + r]), kernel kernel_size^2 f64
+ *                       row-major, r = (kernel_size - 1) / 2; kernel_size must be odd (the
+ *                       reference's own kernel never compiles: it passes a float r to range()). */
+int gvom_expand_binary(gvom_handle *h, const int32_t *in, int32_t *out, int32_t xy_size, double r);
+int gvom_lowpass_binary(gvom_handle *h, const int32_t *in, int32_t *out, int32_t xy_size, int32_t filter_size,
+                        double filter_fraction);
+int gvom_convolve(gvom_handle *h, const double *in, const double *kernel, double *out, int32_t xy_size,
+                  int32_t kernel_size);
+
 /* Atomic-throughput probe (no reference counterpart; a measurement aid): times one launch
  * of non-returning int32 atomic adds into a scratch table of V words on the handle's
  * stream.  pattern 0 scattered (device scope), 1 64 hot words, 2 scattered within the

@@ -172,10 +172,12 @@ def test_ray_cast_variants_equal(cfg4, flags):
         print(f"ray variant {flags}: step {k} slots and combined map identical", flush=True)
 
 
-def test_overlap_matches_serial():
+@pytest.mark.parametrize("variant", [(128, 1, 4), (256, 1, 0)])
+def test_overlap_matches_serial(variant):
     """With scan/combine overlap on, device-pointer frames issued back to back (eager, then
-    a captured 4-frame sequence replayed twice) leave the ring, every slot, the combined map
-    and the 2D maps bit-identical to the same frames without overlap."""
+    a captured 4-frame sequence replayed twice, then two-scan batches) leave the ring, every
+    slot, the combined map and the 2D maps bit-identical to the same frames without overlap;
+    tiled and direct ray casts."""
     import gvom
     import gvom_native as nat
     wl = synth.workload(2)
@@ -183,6 +185,8 @@ def test_overlap_matches_serial():
     dev = [nat.DeviceArray(s.points) for s in scans]
     serial = gvom.Gvom(*wl.map_args)
     ovl = gvom.Gvom(*wl.map_args)
+    for g in (serial, ovl):
+        g.set_raycast_config(*variant[:2], False, False, bool(variant[2] & 4))
     ovl.set_overlap(True)
 
     def frame(g, k, shift):

@@ -194,3 +194,4 @@ def test_pointcloud2_layout_errors(cfg2):
     assert g.ring_state() == before
     g.process_pointcloud2(msg, s.ego, s.transform.astype(np.float64))
     assert g.ring_state() != before
+
