@@ -19,8 +19,9 @@ Extra legs (on by default, --quick skips them):
             one combine per step, captured as one hipGraph: scans/s, against the same
             24 scans sent as 24 plain frames;
   diff      the differentiable map loss on one 65,536-point scan: forward and
-            forward+backward ms, and 20 steps of extrinsic refinement from a transform
-            perturbed by N(0, 0.5 deg) / N(0, 5 cm).
+            forward+backward ms on a transform perturbed by N(0, 0.5 deg) / N(0, 5 cm),
+            and 20 steps refining the rotation of an extrinsic perturbed the same way and
+            shared by 4 frames.
 
 Prints one JSON line.
 """
@@ -221,11 +222,10 @@ def copy_calibration(mib, reps=5):
 def diff_leg(reps):
     import gvom_native as nat
     import gvom_synth
-    from gvom_torch import MapLoss, se3_exp
+    from gvom_torch import MapLoss
     wl = gvom_synth.workload(5)
     s = wl.steps[0][0]
     rng = np.random.default_rng(105)
-    T_gt = s.transform.astype(np.float64)
     T = gvom_synth.perturb_transform(s.transform, rng).astype(np.float64)
     pts = s.points.astype(np.float64)
     dev = nat.DeviceArray(pts)
@@ -243,29 +243,12 @@ def diff_leg(reps):
         loss.backward(1.0, gdev.ptr.value)
     t_fb = (time.perf_counter() - t0) / reps
 
-    def err(Ta):
-        dR = Ta[:3, :3] @ T_gt[:3, :3].T
-        ang = np.degrees(np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1)))
-        return float(ang), float(np.linalg.norm(Ta[:3, 3] - T_gt[:3, 3]))
-
-    e0 = err(T)
-    L0 = loss.forward(dev.ptr.value, pts.shape[0], T, s.ego)[0]
-    Tk = T.copy()
-    m = np.zeros(6)
-    v = np.zeros(6)
-    lr = np.array([0.01, 0.01, 0.01, 0.002, 0.002, 0.002])
-    for k in range(1, 21):                   # Adam on the left se(3) perturbation, re-anchored each step
-        loss.forward(dev.ptr.value, pts.shape[0], Tk, s.ego)
-        gx = loss.backward(1.0)
-        m = 0.9 * m + 0.1 * gx
-        v = 0.999 * v + 0.001 * gx * gx
-        step = lr * (m / (1 - 0.9 ** k)) / (np.sqrt(v / (1 - 0.999 ** k)) + 1e-12)
-        Tk = se3_exp(-step) @ Tk
-    L1 = loss.forward(dev.ptr.value, pts.shape[0], Tk, s.ego)[0]
-    e1 = err(Tk)
-
-    # the same perturbation as a sensor extrinsic shared by 4 frames at known vehicle poses
-    # (the frames' transforms): misalignment is observable across frames
+    # 20 refinement steps of a perturbed sensor extrinsic shared by 4 frames at known vehicle
+    # poses (1 m apart, ~1 deg yaw drift).  The loss is invariant to a common translation and
+    # to a rotation about z of all points, so one scan's own transform gets no gradient along
+    # those directions; across the 4 frames the extrinsic's rotation shows as misalignment
+    # between frames, its translation only through their ~1 deg relative yaw.  The rotation is
+    # refined and the translation held.
     from gvom_torch import refine_extrinsic
     wl4 = gvom_synth.workload(2, frames=4)
     frames = [st[0] for st in wl4.steps]
@@ -273,7 +256,8 @@ def diff_leg(reps):
     E0 = gvom_synth.perturb_transform(np.eye(4, dtype=np.float32), np.random.default_rng(205)).astype(np.float64)
     loss4 = MapLoss(0.4, 0.2, 256, 64)
     E1, Ls = refine_extrinsic([f.points.astype(np.float64) for f in frames],
-                              [f.transform.astype(np.float64) for f in frames], E0, loss4, steps=20)
+                              [f.transform.astype(np.float64) for f in frames], E0, loss4, steps=20,
+                              lr=(0.0, 0.0, 0.0, 0.002, 0.002, 0.002))
 
     def err_e(Ea):
         dR = Ea[:3, :3] @ E_true[:3, :3].T
@@ -283,11 +267,9 @@ def diff_leg(reps):
                         "0.1 * column-height variance",
             "points": int(pts.shape[0]), "fwd_ms": round(1e3 * t_f, 3), "fwd_bwd_ms": round(1e3 * t_fb, 3),
             "bwd_over_fwd": round(t_fb / t_f, 3),
-            "refine_20_steps": {"loss": [round(float(L0), 4), round(float(L1), 4)],
-                                "rot_err_deg": [round(e0[0], 4), round(e1[0], 4)],
-                                "trans_err_m": [round(e0[1], 4), round(e1[1], 4)]},
             "refine_extrinsic_4_frames_20_steps": {
-                "workload": "4 frames of 64x1024 scans at known vehicle poses, shared perturbed extrinsic",
+                "workload": "4 frames of 64x1024 scans at known vehicle poses, shared perturbed extrinsic; "
+                            "rotation refined (Adam, 0.002 rad), translation held",
                 "loss": [round(Ls[0], 4), round(Ls[-1], 4)],
                 "rot_err_deg": [round(err_e(E0)[0], 4), round(err_e(E1)[0], 4)],
                 "trans_err_m": [round(err_e(E0)[1], 4), round(err_e(E1)[1], 4)]}}

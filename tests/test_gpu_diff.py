@@ -127,3 +127,20 @@ def test_gradients_match_central_differences(scene):
           f"points max {max(rel_pt):.2e} median {np.median(rel_pt):.2e}", flush=True)
     assert max(rel_xi) < 1e-3
     assert max(rel_pt) < 1e-3
+
+
+def test_rigid_motion_invariances(scene):
+    """The loss is unchanged by a world translation and by a rotation about z (isotropic xy
+    resolution; counts and assignment frozen), so at any transform the se(3) gradient's
+    translation part and its z-rotation part vanish, and the per-point gradient has zero net
+    force.  Roll and pitch change it (z has its own resolution): those parts are O(1)."""
+    pts, T, ego, _ = scene
+    _, gxi, gw, _, _ = _gpu(pts, T, ego)
+    world = _world(pts, T)
+    tilt = np.abs(gxi[3:5]).max()
+    print(f"gxi {gxi}", flush=True)
+    assert tilt > 0
+    assert np.abs(gxi[:3]).max() <= 1e-6 * tilt
+    assert abs(gxi[5]) <= 1e-5 * tilt
+    assert np.abs(gw.sum(0)).max() <= 1e-9 * np.abs(gw).sum()
+    assert abs(np.cross(world, gw).sum(0)[2]) <= 1e-5 * tilt
