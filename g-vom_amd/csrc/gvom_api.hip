@@ -2,6 +2,7 @@
 // memory, the ring-buffer bookkeeping of the reference class Gvom
 // (scripts/gvom.py:21-101, 198-216, 225-308) and the launch sequences of the scan and
 // combine stages.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -127,6 +128,14 @@ struct gvom_handle {
     int overlap = 0;
     hipStream_t front = nullptr;
     hipEvent_t ev_front = nullptr, ev_back = nullptr;
+    // An overlapped stage inside a capture forks `front` from the main stream and `side` from
+    // `front`.  hipStreamEndCapture has crashed on that nested fork under an older HIP runtime
+    // than the one this library is built against (torch bundles one, and when torch is imported
+    // first this library binds to it: both have the SONAME libamdhip64.so.7).  Under an older
+    // runtime a captured sequence therefore runs its scan stages on the main stream, as with
+    // overlap off; eager calls still overlap.
+    int runtime_version = 0;
+    bool nested_capture = true;
 };
 
 #define CKH(h, expr)                                                                 \
@@ -223,6 +232,8 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
         if (e != hipSuccess) return fail_create(h, std::string(#expr) + ": " + hipGetErrorString(e), GVOM_EHIP); \
     } while (0)
     CKC(hipGetDevice(&h->device));
+    CKC(hipRuntimeGetVersion(&h->runtime_version));
+    h->nested_capture = h->runtime_version / 100000 >= HIP_VERSION / 100000;   // major * 100 + minor
     CKC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     CKC(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
     CKC(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
@@ -495,7 +506,7 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     // (new_buf), never what a combine reads; with overlap on it runs on `front` and may proceed
     // while a combine enqueued before this call still runs on st.  Only its commit (ring
     // state, buffer bits) goes on st, after that combine.
-    const bool ovl = h->overlap && on_device && may_overlap;
+    const bool ovl = h->overlap && on_device && may_overlap && (h->nested_capture || !h->capturing);
     hipStream_t fs = ovl ? h->front : st;
     if (ovl) CKH(h, hipStreamWaitEvent(fs, h->ev_back, 0));
     const size_t esz = dtype == GVOM_F32 ? 4 : 8;
@@ -1354,6 +1365,21 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
         if (best_ms) *best_ms = best;
     }
     return out;
+}
+
+extern "C" const char *gvom_hip_runtime_path(void) {
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void *>(&hipRuntimeGetVersion), &info) && info.dli_fname) return info.dli_fname;
+    return "";
+}
+
+extern "C" int gvom_hip_versions(int32_t *built, int32_t *running) {
+    if (!built || !running) return GVOM_EINVAL;
+    *built = HIP_VERSION;
+    int v = 0;
+    const hipError_t e = hipRuntimeGetVersion(&v);
+    *running = v;
+    return e == hipSuccess ? GVOM_OK : GVOM_EHIP;
 }
 
 extern "C" int gvom_set_overlap(gvom_handle *h, int32_t enabled) {

@@ -59,6 +59,8 @@ SIGNATURES = {
     "gvom_graph_launch": (ctypes.c_int, [_P, _P, _P, _P, _I32]),
     "gvom_graph_destroy": (None, [_P]),
     "gvom_autotune": (ctypes.c_int, [_P, _P, _I64, _I32, _P, _P, _I32, _P, _P]),
+    "gvom_hip_versions": (ctypes.c_int, [_P, _P]),
+    "gvom_hip_runtime_path": (ctypes.c_char_p, []),
     "gvom_set_raycast_config": (ctypes.c_int, [_P, _I32, _I32, _I32]),
     "gvom_diff_backward": (ctypes.c_int, [_P, ctypes.c_double, _P, _P, _P]),
 }
@@ -113,6 +115,29 @@ def load(path: str = LIB_PATH):
     return lib
 
 
+def hip_versions():
+    """(built, running): the HIP version libgvom_hip.so was built against and that of the
+    HIP runtime it is bound to in this process (major * 10**7 + minor * 10**5 + patch)."""
+    built, running = ctypes.c_int32(), ctypes.c_int32()
+    load().gvom_hip_versions(ctypes.byref(built), ctypes.byref(running))
+    return built.value, running.value
+
+
+def hip_runtime_files():
+    """Distinct HIP runtime files (libamdhip64) mapped into this process.  torch loads its
+    bundled copy by the name libamdhip64.so from its own directory, this library by the
+    SONAME libamdhip64.so.7: whichever is loaded first serves both names' users that bind
+    later by SONAME, so importing torch before this library leaves one runtime, and the
+    other order leaves two."""
+    files = set()
+    with open("/proc/self/maps") as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 6 and "libamdhip64" in os.path.basename(parts[-1]):
+                files.add(os.path.realpath(parts[-1]))
+    return sorted(files)
+
+
 def check(rc: int, handle=None) -> int:
     if rc < 0:
         msg = load().gvom_last_error(handle)
@@ -131,8 +156,12 @@ _hip = None
 def _hip_runtime():
     global _hip
     if _hip is None:
-        load()   # libgvom_hip.so pulls in the HIP runtime
-        _hip = ctypes.CDLL("libamdhip64.so")
+        # the runtime file libgvom_hip.so is bound to: a bare "libamdhip64.so" can name
+        # another copy once torch has loaded its own under that name
+        path = load().gvom_hip_runtime_path().decode()
+        if not path:
+            raise GvomError("cannot locate the HIP runtime libgvom_hip.so is bound to")
+        _hip = ctypes.CDLL(path)
         _hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
         _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         _hip.hipFree.argtypes = [ctypes.c_void_p]

@@ -77,6 +77,11 @@ class MapLoss:
 
 def _function():
     import torch
+    runtimes = nat.hip_runtime_files()
+    if len(runtimes) > 1:
+        # torch's CUDA tensors would come from one HIP runtime and the kernels run on another
+        raise nat.GvomError("torch and libgvom_hip.so use different HIP runtimes in this process "
+                            f"({', '.join(runtimes)}); import torch before gvom so that both bind to one")
 
     class _MapLossFn(torch.autograd.Function):
         @staticmethod

@@ -253,8 +253,22 @@ void *gvom_stream(gvom_handle *h);
  * state, buffer selection) and everything else stay ordered on the handle's stream.  The
  * caller's points must therefore be complete when the process call is made (not merely
  * enqueued on the handle's stream after the preceding combine).  Also applies inside a
- * hipGraph capture.  Fails while capturing. */
+ * hipGraph capture, unless the HIP runtime in use is older than the one the library was built
+ * against (see gvom_hip_versions).  Fails while capturing. */
 int gvom_set_overlap(gvom_handle *h, int32_t enabled);
+
+/* The HIP version this library was built against and the version of the HIP runtime it is
+ * bound to in this process (HIP_VERSION encoding: major * 10^7 + minor * 10^5 + patch).
+ * Both runtimes carry the SONAME libamdhip64.so.7, so a process that loaded another copy
+ * first (torch bundles one) binds the library to that copy.  When the running runtime is
+ * older, a captured sequence does not overlap its scan stages with the combines (eager calls
+ * still do).  No counterpart in the reference (diagnostics). */
+int gvom_hip_versions(int32_t *built, int32_t *running);
+
+/* File of the HIP runtime this library is bound to in this process, so that a caller's own
+ * HIP calls (device buffers, copies) go to the same runtime.  No counterpart in the
+ * reference. */
+const char *gvom_hip_runtime_path(void);
 
 /* hipGraph capture of a launch sequence (no reference counterpart).
  * Between gvom_graph_begin and gvom_graph_end, gvom_process_pointcloud_device,

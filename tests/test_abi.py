@@ -106,3 +106,15 @@ def test_synthetic_workloads_are_deterministic():
     s = a.steps[0][0]
     assert s.points.dtype.name == "float32" and s.transform.dtype.name == "float32"
     assert 0.9 * 8192 < s.points.shape[0] < 8192
+
+
+def test_library_bound_to_the_runtime_it_was_built_for():
+    """The test session loads libgvom_hip.so before torch, so the library runs on the ROCm
+    HIP runtime it was built against (same major and minor version), not on torch's bundled
+    copy, which carries the same SONAME."""
+    import gvom_native as nat
+    built, running = nat.hip_versions()
+    files = nat.hip_runtime_files()
+    print(f"built for HIP {built}, running {running}; runtimes mapped: {files}")
+    assert built // 100000 == running // 100000
+    assert files

@@ -234,3 +234,46 @@ def test_overlap_matches_serial(variant):
             g.combine_maps_device()
     compare("batches")
     print("overlap batches: identical", flush=True)
+
+
+@pytest.mark.parametrize("cfg", [1, 2])
+def test_uneven_cadence_overlap_serial_oracle(cfg):
+    """Scans and combines in an uneven order (two scans per combine, a combine with no new
+    scan, three scans in a row), the sensor moving, with overlap on and off, each combine
+    checked against the oracle.  Buffer 1 (cfg1: every scan stage writes the one slot's
+    other buffer while the previous combine may still read it) and buffer 4 (cfg2, the ring
+    wrapping)."""
+    import gvom
+    import gvom_native as nat
+    wl = synth.workload(cfg)
+    scans = [st[0] for st in wl.steps]
+    dev = [nat.DeviceArray(s.points) for s in scans]
+    serial = gvom.Gvom(*wl.map_args)
+    ovl = gvom.Gvom(*wl.map_args)
+    ovl.set_overlap(True)
+    o = OracleGvom(*wl.map_args)
+    shift = np.array([0.45, -0.3, 0.2])
+    plan = "PPCCPCCPPPCPC"
+    k = 0
+    for step, op in enumerate(plan):
+        label = f"cfg{cfg} op {step} ({op})"
+        if op == "P":
+            s = scans[k % len(scans)]
+            ego = s.ego + k * shift
+            for g in (serial, ovl):
+                g.process_pointcloud_device(dev[k % len(scans)], s.points.shape[0], ego, s.transform)
+            o.Process_pointcloud(s.points, ego, s.transform)
+            k += 1
+            continue
+        origins = [g.combine_maps_device() for g in (serial, ovl)]
+        out_o = o.combine_maps()
+        assert out_o is not None
+        for g, mode, org in ((serial, "serial", origins[0]), (ovl, "overlap", origins[1])):
+            assert_equal(f"{label} {mode} origin_m", org, out_o[0])
+            assert g.ring_state()["buffer_index"] == o.buffer_index
+            assert g.ring_state()["last_buffer_index"] == o.last_buffer_index
+            slot = o.last_buffer_index
+            compare_scan(g.slot_state(slot), o.slots[slot], f"{label} {mode} slot {slot}")
+            compare_combined(g.combined_state(), o.combined, f"{label} {mode} combined")
+            compare_maps(g.maps_state(), o.combined, f"{label} {mode}")
+        print(f"{label}: cells {len(o.combined.hit)} ok", flush=True)

@@ -1,0 +1,91 @@
+"""The library bound to torch's bundled HIP runtime (torch imported first, in a fresh process).
+
+Both runtimes carry the SONAME libamdhip64.so.7, so a process that imports torch before gvom
+runs libgvom_hip.so on torch's copy (older than the ROCm runtime the library is built
+against on this image).  Under an older runtime a captured sequence does not fork the
+overlapped scan stage onto further streams; this test runs eager overlapped frames and a
+captured, overlapped 4-frame sequence replayed twice in that binding, and checks the ring,
+every slot, the combined map and the 2D maps bit-identical to serial eager frames.
+"""
+import os
+import subprocess
+import sys
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+CHILD = r"""
+import sys
+import torch  # noqa: F401  (first: its HIP runtime then serves libgvom_hip.so too)
+sys.path[:0] = [{pkg!r}, {root!r}]
+import numpy as np
+import gvom
+import gvom_native as nat
+import gvom_synth as synth
+
+built, running = nat.hip_versions()
+print("HIP built", built, "running", running, "runtimes", nat.hip_runtime_files(), flush=True)
+wl = synth.workload(2)
+scans = [st[0] for st in wl.steps]
+dev = [nat.DeviceArray(s.points) for s in scans]
+serial = gvom.Gvom(*wl.map_args)
+ovl = gvom.Gvom(*wl.map_args)
+ovl.set_overlap(True)
+
+
+def frame(g, k, shift):
+    s = scans[k % len(scans)]
+    g.process_pointcloud_device(dev[k % len(scans)], s.points.shape[0], s.ego + shift, s.transform)
+    g.combine_maps_device()
+
+
+def same(a, b, label):
+    for key in a:
+        assert np.array_equal(a[key], b[key]), (label, key)
+
+
+def compare(label):
+    assert serial.ring_state() == ovl.ring_state(), label
+    same(serial.combined_state(), ovl.combined_state(), label + " combined")
+    same(serial.maps_state(), ovl.maps_state(), label + " maps")
+    for slot in range(wl.map_args[4]):
+        a, b = serial.slot_state(slot), ovl.slot_state(slot)
+        assert (a is None) == (b is None), (label, slot)
+        if a is not None:
+            same(a, b, label + " slot %d" % slot)
+
+
+zero = np.zeros(3)
+for k in range(6):
+    frame(serial, k, zero)
+    frame(ovl, k, zero)
+compare("eager")
+print("eager overlapped frames identical", flush=True)
+ovl.sync()
+ovl.graph_begin(16)
+for k in range(4):
+    frame(ovl, k, zero)
+gr = ovl.graph_end()
+print("captured", flush=True)
+shift = np.array([0.4, -0.2, 0.1])
+for r in range(2):
+    egos = np.stack([scans[k].ego + r * shift for k in range(4)])
+    gr.launch(egos, np.stack([scans[k].transform for k in range(4)]))
+    for k in range(4):
+        frame(serial, k, r * shift)
+    compare("replay %d" % r)
+    print("replay", r, "identical", flush=True)
+print("ok", flush=True)
+"""
+
+
+def test_torch_runtime_binding_overlap_and_capture():
+    code = CHILD.format(pkg=os.path.join(ROOT, "g-vom_amd"), root=ROOT)
+    print("child: torch imported first, overlap on, eager + captured frames", flush=True)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert r.stdout.strip().endswith("ok")
