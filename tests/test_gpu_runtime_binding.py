@@ -89,3 +89,51 @@ def test_torch_runtime_binding_overlap_and_capture():
     print(r.stdout, flush=True)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     assert r.stdout.strip().endswith("ok")
+
+
+CHILD_AUTOGRAD = r"""
+import sys
+import torch
+sys.path[:0] = [{pkg!r}, {root!r}]
+import numpy as np
+import gvom_native as nat
+import gvom_synth as synth
+from gvom_torch import MapLoss, map_loss
+
+print("runtimes", nat.hip_runtime_files(), flush=True)
+s = synth.workload(5).steps[0][0]
+T0 = synth.perturb_transform(s.transform, np.random.default_rng(105)).astype(np.float64)
+pts = s.points.astype(np.float64)
+# autograd path: float64 CUDA points, 6-vector xi at 0
+p = torch.tensor(pts, device="cuda:0", requires_grad=True)
+xi = torch.zeros(6, dtype=torch.float64, requires_grad=True)
+L = map_loss(p, xi, T0, s.ego, MapLoss(0.4, 0.2, 256, 64))
+L.backward()
+gp = p.grad.detach().cpu().numpy()
+gx = xi.grad.numpy()
+# the same kernels through MapLoss directly on a DeviceArray
+ml = MapLoss(0.4, 0.2, 256, 64)
+dev = nat.DeviceArray(pts)
+gdev = nat.DeviceArray(np.zeros_like(pts))
+out = ml.forward(dev.ptr.value, pts.shape[0], T0, s.ego)
+gxi = ml.backward(1.0, gdev.ptr.value)
+g_direct = gdev.to_host()
+print("L", float(L), out[0], flush=True)
+assert abs(float(L) - out[0]) <= 1e-12 * abs(out[0])
+np.testing.assert_allclose(gx, gxi, rtol=1e-10, atol=1e-12 * np.abs(gxi).max())
+np.testing.assert_allclose(gp, g_direct, rtol=1e-10, atol=1e-12 * np.abs(g_direct).max())
+assert np.abs(gp).max() > 0
+print("ok", flush=True)
+"""
+
+
+def test_map_loss_autograd_with_torch_cuda_tensors():
+    """torch imported first (one HIP runtime for torch and the library): map_loss over a
+    float64 CUDA tensor gives the loss, the point gradient (a CUDA tensor) and the se(3)
+    gradient of direct MapLoss forward/backward calls on the same scan."""
+    code = CHILD_AUTOGRAD.format(pkg=os.path.join(ROOT, "g-vom_amd"), root=ROOT)
+    print("child: torch imported first, map_loss on cuda:0", flush=True)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert r.stdout.strip().endswith("ok")
