@@ -290,7 +290,7 @@ def main():
     ap.add_argument("--ray-config", default="auto",
                     help="'auto' (autotune on the first scan) or 'block,rays_per_thread,flags' "
                          "(flags: bit 0 LDS near-sensor box, bit 1 azimuth-sector ray order, bit 2 "
-                         "tiled scratch)")
+                         "tiled scratch, bit 3 spread wavefronts)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU only; runs on more than one GPU are not supported",
@@ -318,7 +318,7 @@ def main():
         ray_cfg = {"block": rc[0], "rays_per_thread": rc[1], "flags": rc[2], "autotuned_ms": round(rc[3], 4)}
     else:
         b, r, f = (int(v) for v in args.ray_config.split(","))
-        g.set_raycast_config(b, r, bool(f & 1), bool(f & 2), bool(f & 4))
+        g.set_raycast_config(b, r, bool(f & 1), bool(f & 2), bool(f & 4), bool(f & 8))
         ray_cfg = {"block": b, "rays_per_thread": r, "flags": f}
 
     def frame(k):

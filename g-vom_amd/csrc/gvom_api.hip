@@ -111,7 +111,7 @@ struct gvom_handle {
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> set_used;   // bit k set when event k of that set was recorded
     int cur_set = -1;
-    int ray_block = 128, ray_rpt = 1, ray_box = 0, ray_xcd = 0;
+    int ray_block = 128, ray_rpt = 1, ray_box = 0, ray_xcd = 0, ray_spread = 1;
     uint8_t *rsec = nullptr;
     uint32_t *rorder = nullptr;
     int8_t *grids = nullptr;   // OccupancyGrid payloads staged for host copies (5 * X bytes)
@@ -554,7 +554,7 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     stage_event(h, 1, fs);
     CKH(h, launch_raycast(dtype, h->world, h->valid, sd, S, NT, g, h->st, tiled ? h->tscratch : h->slot_tables,
                           h->ray_block, h->ray_rpt, tiled ? 0 : h->ray_box, tiled ? 1 : 0, -1,
-                          xcd ? h->rorder : nullptr, fs));
+                          xcd ? h->rorder : nullptr, h->ray_spread, fs));
     stage_event(h, 9, fs);
     if (tiled) CKH(h, launch_tile_convert(g, h->st, S, -1, h->tscratch, h->slot_tables, fs));
     stage_event(h, 2, fs);
@@ -1320,7 +1320,7 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
                         if (ok && !itile) ok = launch_fill_i32(scratch, 0, g.V, st) == hipSuccess;
                         ok = ok && launch_raycast(dtype, h->world, h->valid, h->sd, 1, n, g, h->st,
                                                   itile ? h->tscratch : scratch, blocks[ib], rpts[ir], ibox, itile, 0,
-                                                  iord ? h->rorder : nullptr, st) == hipSuccess;
+                                                  iord ? h->rorder : nullptr, h->ray_spread, st) == hipSuccess;
                         if (ok && itile)
                             ok = launch_tile_convert(g, h->st, 1, 0, h->tscratch, scratch, st) == hipSuccess;
                         ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
@@ -1360,7 +1360,7 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
         if (chosen) {
             chosen[0] = bb;
             chosen[1] = br;
-            chosen[2] = bx | (bo << 1) | (bt << 2);
+            chosen[2] = bx | (bo << 1) | (bt << 2) | (h->ray_spread << 3);
         }
         if (best_ms) *best_ms = best;
     }
@@ -1403,5 +1403,6 @@ extern "C" int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t ra
     h->ray_box = flags & 1;
     h->ray_xcd = (flags >> 1) & 1;
     h->ray_tiled = (flags >> 2) & 1;
+    h->ray_spread = (flags >> 3) & 1;
     return GVOM_OK;
 }

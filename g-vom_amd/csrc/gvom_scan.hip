@@ -328,7 +328,7 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
                                                  const ScanDesc *__restrict__ sd, int S, int64_t NT,
                                                  Geo g, const DevState *__restrict__ st,
                                                  int32_t *__restrict__ tables, int maxit, int rpt, int force_slot,
-                                                 const uint32_t *__restrict__ order) {
+                                                 const uint32_t *__restrict__ order, int spread) {
     extern __shared__ int32_t box[];
     // With `order` (XCD-aware schedule) block b traces chunk b / 8 of azimuth sector b % 8;
     // workgroups are dispatched to the 8 XCDs round-robin, so each XCD traces one sector.
@@ -366,7 +366,13 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
         __syncthreads();
     }
     for (int rep = 0; rep < rpt; ++rep) {
-        const int64_t i = RAY_INDEX((chunk * rpt + rep) * blockDim.x + threadIdx.x);
+        // spread: the block's wavefronts come from across the scan (wavefront w of pass rep of
+        // block b traces rays of global wavefront (rep * waves per block + w) * blocks + b), so
+        // the long-ray rings do not end up together on a few CUs; lanes stay adjacent rays
+        const int64_t j = (spread && !order)
+                              ? ((((int64_t)rep * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gridDim.x + chunk) << 6) + lane
+                              : (chunk * rpt + rep) * blockDim.x + threadIdx.x;
+        const int64_t i = RAY_INDEX(j);
         bool inbox_scan = false;
         Ray r;
         r.alive = false;
@@ -1099,7 +1105,7 @@ hipError_t launch_ray_order(const uint8_t *rsec, int64_t NT, DevState *st, uint3
 hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, const ScanDesc *sd,
                           int S, int64_t NT, const Geo &g, const DevState *st, int32_t *tables,
                           int block, int rpt, int use_box, int tiled, int force_slot, const uint32_t *order,
-                          hipStream_t s) {
+                          int spread, hipStream_t s) {
     if (NT <= 0) return hipSuccess;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
     if (tiled && use_box) return hipErrorInvalidValue;
@@ -1110,7 +1116,7 @@ hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, co
     const size_t lds = use_box ? BOX_N * sizeof(int32_t) : 0;
 #define RAY_LAUNCH(P, BOX, TL)                                                                              \
     k_raycast<P, BOX, TL><<<nb, block, BOX ? lds : 0, s>>>((const P *)world, valid, sd, S, NT, g, st, tables, \
-                                                           maxit, rpt, force_slot, order)
+                                                           maxit, rpt, force_slot, order, spread)
     if (dtype == GVOM_F32) {
         if (tiled) RAY_LAUNCH(float, false, true);
         else if (use_box) RAY_LAUNCH(float, true, false);

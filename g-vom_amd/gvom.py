@@ -197,7 +197,7 @@ class Gvom:
     def autotune(self, dev_points, n, ego_position, transform=None, dtype=np.float32):
         """Pick the ray-cast variant by timing it on one device-resident scan.  Returns
         (block, rays_per_thread, flags, best_ms); flags bit 0 LDS box, bit 1 sector order,
-        bit 2 tiled scratch."""
+        bit 2 tiled scratch, bit 3 spread (kept as set, not searched)."""
         ptr = dev_points.ptr if isinstance(dev_points, nat.DeviceArray) else ctypes.c_void_p(int(dev_points))
         ego = np.ascontiguousarray(np.asarray(ego_position, dtype=np.float64).reshape(3))
         T, tdt = _as_transform(transform)
@@ -217,10 +217,11 @@ class Gvom:
         nat.check(self._lib.gvom_probe_atomics(self._h, int(pattern), int(n_atomics), out), self._h)
         return out[0], int(out[1])
 
-    def set_raycast_config(self, block=128, rays_per_thread=1, use_box=False, sector_order=False, tiled=True):
-        """Ray-cast launch shape and variant (tiled scratch excludes the LDS box); a new
-        handle starts at the defaults."""
-        flags = (1 if use_box else 0) | (2 if sector_order else 0) | (4 if tiled else 0)
+    def set_raycast_config(self, block=128, rays_per_thread=1, use_box=False, sector_order=False, tiled=True,
+                           spread=True):
+        """Ray-cast launch shape and variant (tiled scratch excludes the LDS box; spread takes
+        each block's wavefronts from across the scan); a new handle starts at the defaults."""
+        flags = (1 if use_box else 0) | (2 if sector_order else 0) | (4 if tiled else 0) | (8 if spread else 0)
         nat.check(self._lib.gvom_set_raycast_config(self._h, int(block), int(rays_per_thread), flags), self._h)
 
     def set_overlap(self, enabled=True):

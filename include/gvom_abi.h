@@ -295,13 +295,15 @@ void gvom_graph_destroy(gvom_graph *g);
  *     heads of one wavefront step share fewer atomic requests) and its conversion;
  *   - each with the XCD-aware (azimuth-sector) ray order off/on,
  * and keeps the fastest.  chosen = {block, rays per thread, flags (bit 0 box, bit 1 sector
- * order, bit 2 tiled)}; best_ms = its time.  Synchronises. */
+ * order, bit 2 tiled, bit 3 spread)}; best_ms = its time.  The spread bit (wavefronts of a
+ * block taken from across the scan) is not searched: it is kept as set, since it slows the
+ * cast alone but speeds up a cast that overlaps a combine.  Synchronises. */
 int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int32_t dtype, const double ego[3],
                   const void *T, int32_t t_dtype, int32_t chosen[3], float *best_ms);
 /* Sets the ray-cast configuration directly (flags as for gvom_autotune; the box and the
- * tiled scratch exclude each other).  A new handle starts at {128, 1, tiled}, the fastest
- * configuration on the headline scan; the tiled scratch (min(buffer_size, 32) tables) is
- * allocated by the first process call or gvom_graph_begin. */
+ * tiled scratch exclude each other; bit 3 has no effect with bit 1).  A new handle starts at
+ * {128, 1, tiled + spread}; the tiled scratch (min(buffer_size, 32) tables) is allocated by
+ * the first process call or gvom_graph_begin. */
 int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t rays_per_thread, int32_t flags);
 
 /* Differentiable form (no reference counterpart).  Processes one scan like

@@ -137,17 +137,17 @@ def test_graph_replay_matches_eager(cfg4):
         print(f"graph replay {r} ok", flush=True)
 
 
-@pytest.mark.parametrize("flags", [4, 6, 1, 2, 3])
+@pytest.mark.parametrize("flags", [4, 12, 6, 1, 9, 2, 3, 8])
 def test_ray_cast_variants_equal(cfg4, flags):
-    """Every ray-cast variant the autotuner chooses from (tiled scratch, sector order, LDS
-    box) leaves slot tables and records bit-identical to the plain cast, for single scans
-    and ragged 3-scan batches, and so do the combined map and the 2D maps."""
+    """Every ray-cast variant (tiled scratch, sector order, LDS box, spread wavefronts) leaves
+    slot tables and records bit-identical to the plain cast, for single scans and ragged
+    3-scan batches, and so do the combined map and the 2D maps."""
     import gvom
     wl = cfg4
     a = gvom.Gvom(*wl.map_args)
-    a.set_raycast_config(256, 1, False, False, False)
+    a.set_raycast_config(256, 1, False, False, False, False)
     b = gvom.Gvom(*wl.map_args)
-    b.set_raycast_config(128, 2, bool(flags & 1), bool(flags & 2), bool(flags & 4))
+    b.set_raycast_config(128, 2, bool(flags & 1), bool(flags & 2), bool(flags & 4), bool(flags & 8))
     for k, step in enumerate(wl.steps):
         if k == 0:
             for s in step:
