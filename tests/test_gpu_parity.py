@@ -166,3 +166,29 @@ def test_vertical_and_lateral_motion(variant):
         compare_maps(g.maps_state(), o.combined, f"motion {variant} step {k}")
         assert_equal(f"motion {variant} step {k} origin_m", out_g[0], out_o[0])
         print(f"motion {variant} step {k}: cells {len(o.combined.hit)} ok", flush=True)
+
+
+def test_jumps_beyond_the_map():
+    """Ego jumps larger than the map's z extent (12.8 m) and its xy half-width (51.2 m), so
+    that whole sources of the combine fall outside the new map window, then back; 256 x 256 x
+    64 map, buffer 4, every slot, combined map and 2D map checked against the oracle."""
+    import gvom
+    wl = synth.workload(2)
+    s0 = wl.steps[0][0]
+    jumps = [(0.0, 0.0, 0.0), (0.0, 0.0, 14.0), (0.0, 0.0, -3.0), (60.0, -55.0, 0.0), (0.0, 0.0, 0.0),
+             (0.3, -0.2, 20.5)]
+    steps = [[synth.ScanInput(s0.points, s0.ego + np.array(d), s0.transform, k)] for k, d in enumerate(jumps)]
+    g = gvom.Gvom(*wl.map_args)
+    o = OracleGvom(*wl.map_args)
+    for k, step in enumerate(steps):
+        for s in step:
+            g.Process_pointcloud(s.points, s.ego, s.transform)
+            o.Process_pointcloud(s.points, s.ego, s.transform)
+            slot = o.last_buffer_index
+            compare_scan(g.slot_state(slot), o.slots[slot], f"jumps step {k} slot {slot}")
+        out_g = g.combine_maps()
+        out_o = o.combine_maps()
+        compare_combined(g.combined_state(), o.combined, f"jumps step {k} combined")
+        compare_maps(g.maps_state(), o.combined, f"jumps step {k}")
+        assert_equal(f"jumps step {k} origin_m", out_g[0], out_o[0])
+        print(f"jumps step {k}: cells {len(o.combined.hit)} ok", flush=True)
