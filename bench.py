@@ -221,7 +221,7 @@ def copy_calibration(mib, reps=5):
     print(json.dumps({"copy_calibration_bytes": n, "copies": reps}), flush=True)
 
 
-def diff_leg(reps):
+def diff_leg(reps, refine=True):
     import gvom_native as nat
     import gvom_synth
     from gvom_torch import MapLoss
@@ -244,6 +244,12 @@ def diff_leg(reps):
         loss.forward(dev.ptr.value, pts.shape[0], T, s.ego)
         loss.backward(1.0, gdev.ptr.value)
     t_fb = (time.perf_counter() - t0) / reps
+    timing = {"workload": "one 64x1024 scan (65,536 rays, 5% dropout), 256x256x64; L = sum lambda_min + "
+                          "0.1 * column-height variance",
+              "points": int(pts.shape[0]), "fwd_ms": round(1e3 * t_f, 3), "fwd_bwd_ms": round(1e3 * t_fb, 3),
+              "bwd_over_fwd": round(t_fb / t_f, 3)}
+    if not refine:
+        return timing
 
     # 20 refinement steps of a perturbed sensor extrinsic shared by 4 frames at known vehicle
     # poses (1 m apart, ~1 deg yaw drift).  The loss is invariant to a common translation and
@@ -265,10 +271,7 @@ def diff_leg(reps):
         dR = Ea[:3, :3] @ E_true[:3, :3].T
         ang = np.degrees(np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1)))
         return float(ang), float(np.linalg.norm(Ea[:3, 3] - E_true[:3, 3]))
-    return {"workload": "one 64x1024 scan (65,536 rays, 5% dropout), 256x256x64; L = sum lambda_min + "
-                        "0.1 * column-height variance",
-            "points": int(pts.shape[0]), "fwd_ms": round(1e3 * t_f, 3), "fwd_bwd_ms": round(1e3 * t_fb, 3),
-            "bwd_over_fwd": round(t_fb / t_f, 3),
+    return {**timing,
             "refine_extrinsic_4_frames_20_steps": {
                 "workload": "4 frames of 64x1024 scans at known vehicle poses, shared perturbed extrinsic; "
                             "rotation refined (Adam, 0.002 rad), translation held",
@@ -287,6 +290,8 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline only (no graph, batched or diff legs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--diff-only", type=int, default=0, metavar="REPS",
+                    help="run only the differentiable-loss leg with REPS timed calls (for profiling)")
     ap.add_argument("--no-overlap", action="store_true",
                     help="keep each frame's scan stage behind the previous combine (no second stream)")
     ap.add_argument("--ray-config", default="auto",
@@ -299,6 +304,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.diff_only:
+        print(json.dumps({"fwd_bwd": diff_leg(args.diff_only, refine=False)}), flush=True)
+        return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return

@@ -136,6 +136,7 @@ struct gvom_handle {
     // overlap off; eager calls still overlap.
     int runtime_version = 0;
     bool nested_capture = true;
+    int diff_geometry = 0;   // gvom_set_diff_geometry_only
 };
 
 #define CKH(h, expr)                                                                 \
@@ -423,8 +424,11 @@ static int ensure_tile_scratch(gvom_handle *h) {
 }
 
 // The scan stage for S scans (gvom.py:105-220 for each).
+// cast = false (gvom_diff_forward in geometry-only mode): no ray cast; the slot tables are
+// cleared instead, so the slot holds the occupied voxels and records without pass counts.
 static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets, int S, int dtype,
-                        const double *ego, const void *T, int t_dtype, bool on_device, bool may_overlap) {
+                        const double *ego, const void *T, int t_dtype, bool on_device, bool may_overlap,
+                        bool cast = true) {
     const Geo &g = h->g;
     h->diff_n = -1;   // the work buffers a pending gvom_diff_backward reads are reused below
     if (S < 1 || S > GVOM_MAX_BATCH || S > g.B) {
@@ -496,7 +500,7 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
         d.offset = offsets[s];
         d.count = offsets[s + 1] - offsets[s];
     }
-    const bool tiled = h->ray_tiled != 0;
+    const bool tiled = cast && h->ray_tiled != 0;
     if (tiled) {
         int rc_t = ensure_tile_scratch(h);
         if (rc_t) return rc_t;
@@ -534,7 +538,7 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
         CKH(h, hipMemcpyAsync(h->sd, sdh, sizeof(ScanDesc) * S, hipMemcpyHostToDevice, fs));
     }
     stage_event(h, 0, fs);
-    const bool xcd = h->ray_xcd != 0;
+    const bool xcd = cast && h->ray_xcd != 0;
     // the tiled path's conversion writes every word of the slot tables; the direct cast
     // counts into cleared tables
     if (!tiled) CKH(h, launch_clear_slots(h->slot_tables, h->st, S, g, fs));
@@ -552,9 +556,10 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     CKH(h, launch_mom_part(dtype, h->world, sk, sv, NT, sd, S, g, h->st, h->part, h->side));
     CKH(h, hipEventRecord(h->ev_join, h->side));
     stage_event(h, 1, fs);
-    CKH(h, launch_raycast(dtype, h->world, h->valid, sd, S, NT, g, h->st, tiled ? h->tscratch : h->slot_tables,
-                          h->ray_block, h->ray_rpt, tiled ? 0 : h->ray_box, tiled ? 1 : 0, -1,
-                          xcd ? h->rorder : nullptr, h->ray_spread, fs));
+    if (cast)
+        CKH(h, launch_raycast(dtype, h->world, h->valid, sd, S, NT, g, h->st, tiled ? h->tscratch : h->slot_tables,
+                              h->ray_block, h->ray_rpt, tiled ? 0 : h->ray_box, tiled ? 1 : 0, -1,
+                              xcd ? h->rorder : nullptr, h->ray_spread, fs));
     stage_event(h, 9, fs);
     if (tiled) CKH(h, launch_tile_convert(g, h->st, S, -1, h->tscratch, h->slot_tables, fs));
     stage_event(h, 2, fs);
@@ -1046,7 +1051,7 @@ extern "C" int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, 
     if (!h || !T || !ego || n < 0) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
     int64_t off[2] = {0, n};
-    int rc = process_impl(h, points, off, 1, GVOM_F64, ego, T, GVOM_F64, true, false);
+    int rc = process_impl(h, points, off, 1, GVOM_F64, ego, T, GVOM_F64, true, false, !h->diff_geometry);
     if (rc) return rc;
     const Geo &g = h->g;
     const int64_t X = g.X;
@@ -1087,6 +1092,13 @@ extern "C" int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, 
     h->diff_n = n;
     h->diff_slot = slot;
     h->diff_wh = w_height;
+    return GVOM_OK;
+}
+
+extern "C" int gvom_set_diff_geometry_only(gvom_handle *h, int32_t enabled) {
+    if (!h) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->diff_geometry = enabled != 0;
     return GVOM_OK;
 }
 

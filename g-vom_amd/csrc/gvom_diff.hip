@@ -135,7 +135,11 @@ __global__ __launch_bounds__(RB) void k_diff_eigen(const DevState *__restrict__ 
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// Per column: lowest occupied voxel, its argmin point, h_c; block partials of (K, sum h).
+// Per column, one wavefront (lane = z within a 64-voxel chunk): the lowest occupied voxel
+// by ballot, then its points' argmin of world z over the lanes (ties: first in sorted
+// order), h_c; per-block partials of (K, sum h) over the block's CW columns in order.
+constexpr int CW = RB / 64;   // columns per block
+
 __global__ __launch_bounds__(RB) void k_diff_columns(Geo g, const DevState *__restrict__ st, int slot,
                                                      const int32_t *__restrict__ table,
                                                      const int32_t *__restrict__ seg_start,
@@ -143,55 +147,98 @@ __global__ __launch_bounds__(RB) void k_diff_columns(Geo g, const DevState *__re
                                                      const double *__restrict__ world, double *__restrict__ colh,
                                                      int32_t *__restrict__ colpt, int32_t *__restrict__ pmark,
                                                      double *__restrict__ partK, double *__restrict__ partH) {
-    __shared__ double sh[RB / 64];
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double shK[CW], shH[CW];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t c = (int64_t)blockIdx.x * CW + w;
     double k = 0.0, hsum = 0.0;
     if (c < g.X) {
         const int32_t *col = table + (int64_t)cur_buf(st->flip, slot) * g.V + c * g.Z;
+        int zfound = -1;
+        int32_t tfound = 0;
+        for (int z0 = 0; z0 < g.Z && zfound < 0; z0 += 64) {
+            const int z = z0 + lane;
+            const int32_t t = z < g.Z ? col[z] : 0;
+            const uint64_t m = __ballot(t < 0);
+            if (m) {
+                const int l = __builtin_ctzll(m);
+                zfound = z0 + l;
+                tfound = __shfl(t, l);
+            }
+        }
         int32_t pt = -1;
         double h = 0.0;
-        for (int z = 0; z < g.Z; ++z) {
-            const int32_t t = col[z];
-            if (t >= 0) continue;
-            const int64_t seg = st->first_seg[0] + (int64_t)(t & 0x7FFFFFFF);
+        if (zfound >= 0) {
+            const int64_t seg = st->first_seg[0] + (int64_t)(tfound & 0x7FFFFFFF);
             const int32_t a = seg_start[seg], b = seg_start[seg + 1];
             double best = 0.0;
-            for (int32_t j = a; j < b; ++j) {
-                const int32_t i = (int32_t)svals[j];
-                const double wz = world[3 * (int64_t)i + 2];
-                if (pt < 0 || wz < best) {
+            int32_t bj = -1;
+            for (int32_t j = a + lane; j < b; j += 64) {
+                const double wz = world[3 * (int64_t)svals[j] + 2];
+                if (bj < 0 || wz < best) {
                     best = wz;
-                    pt = i;
+                    bj = j;
                 }
             }
-            h = best;
-            break;
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(best, off);
+                const int32_t oj = __shfl_xor(bj, off);
+                if (oj >= 0 && (bj < 0 || ob < best || (ob == best && oj < bj))) {
+                    best = ob;
+                    bj = oj;
+                }
+            }
+            if (bj >= 0) {
+                pt = (int32_t)svals[bj];
+                h = best;
+            }
         }
-        colpt[c] = pt;
-        colh[c] = h;
-        if (pt >= 0) {
-            pmark[pt] = (int32_t)c;
-            k = 1.0;
-            hsum = h;
+        if (lane == 0) {
+            colpt[c] = pt;
+            colh[c] = h;
+            if (pt >= 0) {
+                pmark[pt] = (int32_t)c;
+                k = 1.0;
+                hsum = h;
+            }
         }
     }
-    const double tk = block_sum(k, sh);
-    const double th = block_sum(hsum, sh);
+    if (lane == 0) {
+        shK[w] = k;
+        shH[w] = hsum;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        double tk = 0.0, th = 0.0;
+        for (int q = 0; q < CW; ++q) {
+            tk += shK[q];
+            th += shH[q];
+        }
         partK[blockIdx.x] = tk;
         partH[blockIdx.x] = th;
     }
 }
 
-// One block: ordered sum of n partials into *out.
-__global__ __launch_bounds__(RB) void k_diff_sum(const double *__restrict__ part, int64_t n, double *__restrict__ out) {
-    __shared__ double sh[RB / 64];
-    // each thread sums a contiguous run of partials, then the block combines in thread order
-    const int64_t per = (n + RB - 1) / RB;
+// Ordered sum of n partials in one block: each thread sums a contiguous run, then the block
+// combines in thread order (valid in thread 0).
+__device__ __forceinline__ double ordered_sum(const double *__restrict__ part, int64_t n, double *sh) {
+    const int64_t per = (n + blockDim.x - 1) / blockDim.x;
     double x = 0.0;
     for (int64_t i = threadIdx.x * per; i < (threadIdx.x + 1) * per && i < n; ++i) x += part[i];
-    const double t = block_sum(x, sh);
-    if (threadIdx.x == 0) *out = t;
+    return block_sum(x, sh);
+}
+
+// Up to 6 independent ordered sums, one block each.
+struct SumJobs {
+    const double *part[6];
+    int64_t n[6];
+    double *out[6];
+};
+
+__global__ __launch_bounds__(RB) void k_diff_sums(SumJobs jobs) {
+    __shared__ double sh[RB / 64];
+    const int k = blockIdx.x;
+    const double t = ordered_sum(jobs.part[k], jobs.n[k], sh);
+    if (threadIdx.x == 0) *jobs.out[k] = t;
 }
 
 // Per column: (h_c - mean)^2 partials.
@@ -210,9 +257,14 @@ __global__ __launch_bounds__(RB) void k_diff_colvar(Geo g, const double *__restr
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// Final scalars: sums = [sum lambda, K, sum h, sum (h - mean)^2, L, L_rough, L_height]
-__global__ void k_diff_loss(double *__restrict__ sums, double wh) {
+// Sum of the variance partials, then the final scalars:
+// sums = [sum lambda, K, sum h, sum (h - mean)^2, L, L_rough, L_height]
+__global__ __launch_bounds__(RB) void k_diff_loss(const double *__restrict__ part, int64_t n,
+                                                  double *__restrict__ sums, double wh) {
+    __shared__ double sh[RB / 64];
+    const double t = ordered_sum(part, n, sh);
     if (threadIdx.x != 0) return;
+    sums[3] = t;
     const double K = sums[1];
     const double var = K > 0 ? sums[3] / K : 0.0;
     sums[5] = sums[0];
@@ -306,7 +358,7 @@ inline unsigned nb(int64_t n, int per) { return (unsigned)((n + per - 1) / per >
 }  // namespace
 
 int64_t diff_partials(int64_t n_points, int64_t X, int64_t ncap) {
-    int64_t a = nb(n_points, RB), b = nb(X, RB), c = nb(ncap, RB);
+    int64_t a = nb(n_points, RB), b = nb(X, CW), c = nb(ncap, RB);
     int64_t m = a > b ? a : b;
     m = m > c ? m : c;
     return 6 * m + 16;
@@ -319,18 +371,28 @@ hipError_t launch_diff_forward(const Geo &g, const DevState *st, int slot, const
                                const SlotRec *recs, int64_t ncap, const int32_t *seg_start, const uint32_t *svals,
                                const double *world, int64_t n, void *dr, double *colh, int32_t *colpt,
                                int32_t *pmark, double *part, double *sums, double wh, hipStream_t s) {
-    const unsigned ne = nb(ncap, RB), nc = nb(g.X, RB);
+    // partials: [0, m) eigenvalues, then [m, 2m) and [2m, 3m) the column counts and heights;
+    // the variance partials reuse [0, m) once the first three sums are taken
+    const unsigned ne = nb(ncap, RB), ncw = nb(g.X, CW), nc = nb(g.X, RB);
+    const int64_t m = (diff_partials(n > 0 ? n : 1, g.X, ncap) - 16) / 6;
     hipError_t e = launch_fill_i32(pmark, -1, n > 0 ? n : 1, s);
     if (e != hipSuccess) return e;
     k_diff_eigen<<<ne, RB, 0, s>>>(st, slot, recs, ncap, (DiffRec *)dr, part);
-    k_diff_sum<<<1, RB, 0, s>>>(part, ne, sums + 0);
-    k_diff_columns<<<nc, RB, 0, s>>>(g, st, slot, table, seg_start, svals, world, colh, colpt, pmark, part,
-                                     part + nc);
-    k_diff_sum<<<1, RB, 0, s>>>(part, nc, sums + 1);
-    k_diff_sum<<<1, RB, 0, s>>>(part + nc, nc, sums + 2);
+    k_diff_columns<<<ncw, RB, 0, s>>>(g, st, slot, table, seg_start, svals, world, colh, colpt, pmark, part + m,
+                                      part + 2 * m);
+    SumJobs j3{};
+    j3.part[0] = part;
+    j3.n[0] = ne;
+    j3.out[0] = sums + 0;
+    j3.part[1] = part + m;
+    j3.n[1] = ncw;
+    j3.out[1] = sums + 1;
+    j3.part[2] = part + 2 * m;
+    j3.n[2] = ncw;
+    j3.out[2] = sums + 2;
+    k_diff_sums<<<3, RB, 0, s>>>(j3);
     k_diff_colvar<<<nc, RB, 0, s>>>(g, colh, colpt, sums, part);
-    k_diff_sum<<<1, RB, 0, s>>>(part, nc, sums + 3);
-    k_diff_loss<<<1, 64, 0, s>>>(sums, wh);
+    k_diff_loss<<<1, RB, 0, s>>>(part, nc, sums, wh);
     return hipGetLastError();
 }
 
@@ -343,6 +405,12 @@ hipError_t launch_diff_backward(const Geo &g, const DevState *st, int slot, cons
     const unsigned np = nb(n, RB);
     k_diff_backward<<<np, RB, 0, s>>>(g, st, slot, sd, world, valid, table, (const DiffRec *)dr, pmark, colh, sums,
                                       wh, dL, n, gsensor, gworld, part);
-    for (int k = 0; k < 6; ++k) k_diff_sum<<<1, RB, 0, s>>>(part + (int64_t)k * np, np, grad6 + k);
+    SumJobs j6{};
+    for (int k = 0; k < 6; ++k) {
+        j6.part[k] = part + (int64_t)k * np;
+        j6.n[k] = np;
+        j6.out[k] = grad6 + k;
+    }
+    k_diff_sums<<<6, RB, 0, s>>>(j6);
     return hipGetLastError();
 }

@@ -60,6 +60,7 @@ SIGNATURES = {
     "gvom_graph_destroy": (None, [_P]),
     "gvom_autotune": (ctypes.c_int, [_P, _P, _I64, _I32, _P, _P, _I32, _P, _P]),
     "gvom_hip_versions": (ctypes.c_int, [_P, _P]),
+    "gvom_set_diff_geometry_only": (ctypes.c_int, [_P, _I32]),
     "gvom_hip_runtime_path": (ctypes.c_char_p, []),
     "gvom_set_raycast_config": (ctypes.c_int, [_P, _I32, _I32, _I32]),
     "gvom_diff_backward": (ctypes.c_int, [_P, ctypes.c_double, _P, _P, _P]),

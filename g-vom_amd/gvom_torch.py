@@ -51,9 +51,11 @@ class MapLoss:
     """Differentiable map loss on single scans, backed by the HIP kernels."""
 
     def __init__(self, xy_resolution, z_resolution, xy_size, z_size, min_distance=1.0, xy_eigen_dist=1,
-                 z_eigen_dist=1, w_height=0.1):
+                 z_eigen_dist=1, w_height=0.1, geometry_only=True):
         self.g = gvom.Gvom(xy_resolution, z_resolution, xy_size, z_size, 1, min_distance, 0.5, 0.5, 0.3, 2.0, 4.0,
                            1.0, xy_eigen_dist, z_eigen_dist)
+        # the handle only evaluates the loss: no ray cast (pass counts are not part of it)
+        nat.check(self.g._lib.gvom_set_diff_geometry_only(self.g._h, 1 if geometry_only else 0), self.g._h)
         self.w_height = float(w_height)
         self.n = 0
 

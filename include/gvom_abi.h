@@ -319,6 +319,13 @@ int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t rays_per_thre
 int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, const double T[16], const double ego[3],
                       double w_height, double out[3]);
 
+/* With enabled != 0, gvom_diff_forward skips the ray cast: the ring slot it writes holds the
+ * scan's occupied voxels and records but no ray-pass counts (free space reads as unknown,
+ * record totals equal hit counts).  The loss and its gradient use neither, and are the same.
+ * For handles that only evaluate the loss; a handle whose ring also feeds gvom_combine_maps
+ * keeps the default (off).  No counterpart in the reference. */
+int gvom_set_diff_geometry_only(gvom_handle *h, int32_t enabled);
+
 /* Gradient of dL * L from the last gvom_diff_forward: grad_points (device, n x 3 f64,
  * sensor frame) and grad_world (device, n x 3 f64, world frame) may be NULL;
  * grad_xi (host, 6 doubles) receives dL/d(rho, phi) for T' = exp(xi^) T.  Synchronises. */

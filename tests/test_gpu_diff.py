@@ -144,3 +144,22 @@ def test_rigid_motion_invariances(scene):
     assert abs(gxi[5]) <= 1e-5 * tilt
     assert np.abs(gw.sum(0)).max() <= 1e-9 * np.abs(gw).sum()
     assert abs(np.cross(world, gw).sum(0)[2]) <= 1e-5 * tilt
+
+
+def test_geometry_only_matches_full_scan_stage(scene):
+    """The loss handle skips the ray cast (gvom_set_diff_geometry_only); a handle that runs
+    the full scan stage gives the same loss and the same gradients, bit for bit."""
+    import gvom_native as nat
+    from gvom_torch import MapLoss
+    pts, T, ego, _ = scene
+    res = []
+    for geo in (True, False):
+        loss = MapLoss(XYR, ZR, XY, Z, 1.0, 1, 1, w_height=0.1, geometry_only=geo)
+        dev = nat.DeviceArray(pts)
+        gw = nat.DeviceArray(np.zeros_like(pts))
+        out = loss.forward(dev.ptr.value, pts.shape[0], T, ego)
+        gxi = loss.backward(1.0, None, gw.ptr.value)
+        res.append((out, gxi, gw.to_host()))
+    print(f"geometry-only L {res[0][0][0]!r}, full {res[1][0][0]!r}", flush=True)
+    for a, b in zip(res[0], res[1]):
+        np.testing.assert_array_equal(a, b)
