@@ -283,7 +283,7 @@ __global__ __launch_bounds__(RB) void k_diff_backward(Geo g, const DevState *__r
                                                       const double *__restrict__ colh, const double *__restrict__ sums,
                                                       double wh, double dL, int64_t n, double *__restrict__ gsensor,
                                                       double *__restrict__ gworld, double *__restrict__ part6) {
-    __shared__ double sh[RB / 64];
+    __shared__ double sh6[6][RB / 64];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double gx = 0.0, gy = 0.0, gz = 0.0, px = 0.0, py = 0.0, pz = 0.0;
     if (i < n && valid[i]) {
@@ -301,6 +301,35 @@ __global__ __launch_bounds__(RB) void k_diff_backward(Geo g, const DevState *__r
             f2 <= g.Z + g.ez) {
             const int64_t b0 = (int64_t)f0, b1 = (int64_t)f1, b2 = (int64_t)f2;
             const int32_t *tab = table + (int64_t)cur_buf(st->flip, slot) * g.V;
+            if (g.ex == 1 && g.ez == 1) {
+                // the 27 table words first, then the records of three neighbours at a time, so
+                // the gathers overlap; terms are added in the loop order of the general case
+                int32_t t[27];
+#pragma unroll
+                for (int k = 0; k < 27; ++k) {
+                    const int64_t ux = b0 + k / 9 - 1, uy = b1 + (k / 3) % 3 - 1, uz = b2 + k % 3 - 1;
+                    const bool in = ux >= 0 && ux < g.XY && uy >= 0 && uy < g.XY && uz >= 0 && uz < g.Z;
+                    t[k] = in ? tab[(ux * g.XY + uy) * g.Z + uz] : 0;
+                }
+#pragma unroll
+                for (int k0 = 0; k0 < 27; k0 += 3) {
+                    DiffRec r[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) r[k] = dr[t[k0 + k] < 0 ? (t[k0 + k] & 0x7FFFFFFF) : 0];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        if (t[k0 + k] >= 0) continue;
+                        const int kk = k0 + k;
+                        const double l0 = q0 - (double)(b0 + kk / 9 - 1) - r[k].m[0],
+                                     l1 = q1 - (double)(b1 + (kk / 3) % 3 - 1) - r[k].m[1],
+                                     l2 = q2 - (double)(b2 + kk % 3 - 1) - r[k].m[2];
+                        const double s = r[k].coef * (r[k].v[0] * l0 + r[k].v[1] * l1 + r[k].v[2] * l2);
+                        a0 += s * r[k].v[0];
+                        a1 += s * r[k].v[1];
+                        a2 += s * r[k].v[2];
+                    }
+                }
+            } else
             for (int64_t ux = b0 - g.ex; ux <= b0 + g.ex; ++ux) {
                 if (ux < 0 || ux >= g.XY) continue;
                 for (int64_t uy = b1 - g.ex; uy <= b1 + g.ex; ++uy) {
@@ -346,10 +375,20 @@ __global__ __launch_bounds__(RB) void k_diff_backward(Geo g, const DevState *__r
         if (gworld) gworld[3 * i] = gworld[3 * i + 1] = gworld[3 * i + 2] = 0.0;
         if (gsensor) gsensor[3 * i] = gsensor[3 * i + 1] = gsensor[3 * i + 2] = 0.0;
     }
-    const double vals[6] = {gx, gy, gz, py * gz - pz * gy, pz * gx - px * gz, px * gy - py * gx};
-    for (int k = 0; k < 6; ++k) {
-        const double t = block_sum(vals[k], sh);
-        if (threadIdx.x == 0) part6[(int64_t)k * gridDim.x + blockIdx.x] = t;
+    // the six block partials in one pass (same order of additions as block_sum)
+    double vals[6] = {gx, gy, gz, py * gz - pz * gy, pz * gx - px * gz, px * gy - py * gx};
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        for (int off = 32; off > 0; off >>= 1) vals[k] += __shfl_xor(vals[k], off);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sh6[k][w] = vals[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double t = 0.0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += sh6[threadIdx.x][q];
+        part6[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
     }
 }
 
