@@ -307,7 +307,8 @@ int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int32_t dtype, 
 int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t rays_per_thread, int32_t flags);
 
 /* Differentiable form (no reference counterpart).  Processes one scan like
- * gvom_process_pointcloud into the next ring slot (points: device array of n x 3
+ * gvom_process_pointcloud into the next ring slot (without the ray cast when
+ * gvom_set_diff_geometry_only is on; points: device array of n x 3
  * float64 in the sensor frame; T: host 4x4 float64 sensor-to-world; ego: host 3
  * doubles) and evaluates
  *   L = sum over occupied voxels of the smallest eigenvalue of the voxel's
