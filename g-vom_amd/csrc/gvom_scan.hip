@@ -1147,7 +1147,8 @@ __global__ __launch_bounds__(256) void k_tile_convert(Geo g, const DevState *__r
     int4 *src = reinterpret_cast<int4 *>(scratch + (int64_t)s * tg.vt + (int64_t)tcol * Z * 16);
     for (int k = threadIdx.x; k < Z * 4; k += 256) {
         const int4 v = src[k];
-        src[k] = make_int4(0, 0, 0, 0);
+        // words no ray reached are zero already: skipping their store saves HBM writes
+        if (v.x | v.y | v.z | v.w) src[k] = make_int4(0, 0, 0, 0);
         int32_t *d = sh + (k >> 2) * 17 + (k & 3) * 4;
         d[0] = v.x;
         d[1] = v.y;
