@@ -163,3 +163,41 @@ def test_geometry_only_matches_full_scan_stage(scene):
     print(f"geometry-only L {res[0][0][0]!r}, full {res[1][0][0]!r}", flush=True)
     for a, b in zip(res[0], res[1]):
         np.testing.assert_array_equal(a, b)
+
+
+def test_forward_backward_other_map_shape():
+    """A 64 x 64 x 32 map (z_size != 64: the one-column-per-wavefront column pass) on a
+    16 x 512 scan: loss and gradients against the FP64 reference with the same assignment."""
+    import ctypes
+    import torch
+    import gvom_native as nat
+    from gvom_torch import MapLoss
+    from oracle import diff_ref
+    s = synth.workload(1).steps[0][0]
+    T = synth.perturb_transform(s.transform, np.random.default_rng(7)).astype(np.float64)
+    pts = s.points.astype(np.float64)
+    xy, z = 64, 32
+    loss = MapLoss(XYR, ZR, xy, z, 1.0, 1, 1, w_height=0.1)
+    dev = nat.DeviceArray(pts)
+    gw = nat.DeviceArray(np.zeros_like(pts))
+    out = loss.forward(dev.ptr.value, pts.shape[0], T, s.ego)
+    gxi = loss.backward(1.0, None, gw.ptr.value)
+    gw = gw.to_host()
+    world = _world(pts, T)
+    valid = np.isfinite(world).all(1) & ((world ** 2).sum(1) >= 1.0)
+    origin = np.floor(np.array([s.ego[0] / XYR - xy / 2, s.ego[1] / XYR - xy / 2, s.ego[2] / ZR - z / 2]))
+    a = diff_ref.assign(world, valid, origin, XYR, ZR, xy, z)
+    w = torch.tensor(world, requires_grad=True)
+    L, Lr, Lh, C = diff_ref.loss(w, a, covariances=True)
+    L.backward()
+    g_ref = w.grad.numpy()
+    tr_sum = float(torch.diagonal(C.detach(), dim1=1, dim2=2).sum())
+    print(f"64x64x32: L {out[0]!r} ref {L.item()!r}; columns {len(a.col_pts)}", flush=True)
+    assert len(a.col_pts) > 0
+    assert abs(out[1] - Lr.item()) <= 1e-8 * tr_sum
+    np.testing.assert_allclose(out[2], Lh.item(), rtol=1e-12, atol=1e-15)
+    scale = np.abs(g_ref).max()
+    np.testing.assert_allclose(gw, g_ref, rtol=0, atol=1e-4 * scale)
+    assert np.isclose(gw, g_ref, rtol=1e-6, atol=1e-9 * scale).mean() >= 0.999
+    xi_ref = np.concatenate([g_ref.sum(0), np.cross(world, g_ref).sum(0)])
+    np.testing.assert_allclose(gxi, xi_ref, rtol=1e-4, atol=1e-6 * np.abs(xi_ref).max())
