@@ -277,3 +277,45 @@ def test_uneven_cadence_overlap_serial_oracle(cfg):
             compare_combined(g.combined_state(), o.combined, f"{label} {mode} combined")
             compare_maps(g.maps_state(), o.combined, f"{label} {mode}")
         print(f"{label}: cells {len(o.combined.hit)} ok", flush=True)
+
+
+def test_largest_ring_and_batch():
+    """The largest ring (buffer 30) filled by one 30-scan batch, then a 7-scan batch that
+    wraps it, each followed by a combine: slots, combined map and 2D maps bit-identical to
+    the same scans sent one call at a time, and equal to the oracle (64 x 64 x 32 map, the
+    16 x 512 scan with the ego moved per scan)."""
+    import gvom
+    wl = synth.workload(1)
+    s0 = wl.steps[0][0]
+    args = synth.gvom_args(0.4, 0.2, 64, 32, 30)
+    rng = np.random.default_rng(30)
+    egos = [s0.ego + np.array([rng.uniform(-1.5, 1.5), rng.uniform(-1.5, 1.5), rng.uniform(-0.5, 0.5)])
+            for _ in range(37)]
+    seq = gvom.Gvom(*args)
+    bat = gvom.Gvom(*args)
+    o = OracleGvom(*args)
+    for lo, hi in ((0, 30), (30, 37)):
+        n = hi - lo
+        pts = np.concatenate([s0.points] * n)
+        off = np.arange(n + 1) * s0.points.shape[0]
+        bat.process_batch(pts, off, np.stack(egos[lo:hi]), np.stack([s0.transform] * n))
+        for k in range(lo, hi):
+            seq.Process_pointcloud(s0.points, egos[k], s0.transform)
+            o.Process_pointcloud(s0.points, egos[k], s0.transform)
+        assert bat.ring_state() == seq.ring_state()
+        for slot in range(30):
+            a, b = seq.slot_state(slot), bat.slot_state(slot)
+            assert (a is None) == (b is None)
+            if a is not None:
+                _state_equal(a, b, f"batch {lo}-{hi} slot {slot}")
+        compare_scan(bat.slot_state(o.last_buffer_index), o.slots[o.last_buffer_index], f"batch {lo}-{hi} newest")
+        out_b = bat.combine_maps()
+        seq.combine_maps()
+        out_o = o.combine_maps()
+        _state_equal(seq.combined_state(), bat.combined_state(), f"batch {lo}-{hi} combined")
+        _state_equal(seq.maps_state(), bat.maps_state(), f"batch {lo}-{hi} maps")
+        compare_combined(bat.combined_state(), o.combined, f"batch {lo}-{hi} combined vs oracle")
+        compare_maps(bat.maps_state(), o.combined, f"batch {lo}-{hi}")
+        for name, a, b in zip(("pos", "neg", "vis"), (out_b[1], out_b[2], out_b[4]), (out_o[1], out_o[2], out_o[4])):
+            assert_equal(f"batch {lo}-{hi} {name}", a, b)
+        print(f"ring 30, batch {lo}-{hi}: cells {len(o.combined.hit)} ok", flush=True)
