@@ -346,7 +346,7 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
         CKH(h, dalloc(&h->hist, 2 * h->hwords));
         CKH(h, hipMemsetAsync(h->hist, 0, sizeof(uint32_t) * 2 * h->hwords, h->stream));
         CKH(h, hipStreamSynchronize(h->stream));
-        CKH(h, dalloc(&h->blk, (pc / 2048 + 2) * (2 + 3 * GVOM_MAX_BATCH)));
+        CKH(h, dalloc(&h->blk, segment_block_words(pc)));
         CKH(h, dalloc(&h->seg_key, pc));
         CKH(h, dalloc(&h->seg_start, pc + 1));
         CKH(h, dalloc(&h->mom, pc));
@@ -552,7 +552,8 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     uint32_t *sk = h->k0, *sv = h->v0;
     CKH(h, radix_sort(h->k0, h->v0, h->k1, h->v1, NT, g.VB + sbits, h->hist, h->hwords, &sk, &sv, h->side));
     h->sorted_vals = sv;
-    CKH(h, launch_segments(sk, NT, h->seg_start, h->seg_key, h->blk, S, g, h->st, h->side));
+    CKH(h, launch_segments(sk, NT, h->seg_start, h->seg_key, h->blk, segment_block_words(h->ptcap), S, g, h->st,
+                           h->side));
     CKH(h, launch_mom_part(dtype, h->world, sk, sv, NT, sd, S, g, h->st, h->part, h->side));
     CKH(h, hipEventRecord(h->ev_join, h->side));
     stage_event(h, 1, fs);

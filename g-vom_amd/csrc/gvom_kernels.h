@@ -32,9 +32,11 @@ hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, co
 hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits,
                       uint32_t *hist, int64_t hwords, uint32_t **kout, uint32_t **vout, hipStream_t s);
 int64_t radix_hist_entries(int64_t n);
+// words of the segment kernels' per-block counts (launch_segments' blk) for n sorted keys
+int64_t segment_block_words(int64_t n);
 // Segments of equal keys in the sorted array, per-scan bounds and slot cell counts.
 hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start, uint32_t *seg_key,
-                           uint32_t *blk, int S, const Geo &g, DevState *st, hipStream_t s);
+                           uint32_t *blk, int64_t blk_words, int S, const Geo &g, DevState *st, hipStream_t s);
 // Per-segment moments (chunk partials, then per-segment sums) + slot table occupancy
 // marks + hit/total/min-height records.  All scan-stage writes to slot tables and records go
 // to the buffers new_buf(flip, slot) of the slots the batch fills.

@@ -558,7 +558,10 @@ __device__ __forceinline__ bool is_head(const uint32_t *__restrict__ k, int64_t 
     return key != GVOM_NOKEY && (i == 0 || k[i - 1] != key);
 }
 
-// Per block of RS_TILE sorted keys: [heads, valid keys, then per scan s (in-map heads,
+// Segment kernels: blocks of 256 threads x 8 sorted keys, independent of the sort's tile.
+constexpr int SEG_TILE = 256 * 8;
+
+// Per block of SEG_TILE sorted keys: [heads, valid keys, then per scan s (in-map heads,
 // halo heads, valid keys)], W = 2 + 3 * S words.
 __global__ __launch_bounds__(256) void k_seg_count(const uint32_t *__restrict__ k, int64_t n, int S, Geo g,
                                                    uint32_t *__restrict__ blk) {
@@ -566,7 +569,7 @@ __global__ __launch_bounds__(256) void k_seg_count(const uint32_t *__restrict__ 
     const int W = 2 + 3 * S;
     if (threadIdx.x < W) c[threadIdx.x] = 0;
     __syncthreads();
-    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    int64_t base = (int64_t)blockIdx.x * SEG_TILE;
     uint32_t heads = 0, valid = 0;
     for (int j = 0; j < 8; ++j) {
         int64_t i = base + threadIdx.x * 8 + j;
@@ -646,7 +649,7 @@ __global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *__restrict_
     for (int b = t; b < (int)blockIdx.x; b += 256) pre += blk[(int64_t)b * W];
     for (int off = 32; off > 0; off >>= 1) pre += __shfl_down(pre, off);
     if (lane == 0) wpre[w] = pre;
-    int64_t base = (int64_t)blockIdx.x * RS_TILE + t * 8;
+    int64_t base = (int64_t)blockIdx.x * SEG_TILE + t * 8;
     uint32_t flags = 0, cnt = 0;
     for (int j = 0; j < 8; ++j) {
         int64_t i = base + j;
@@ -1191,6 +1194,8 @@ int64_t tile_scratch_words(const Geo &g) { return tile_geo(g).vt; }
 
 int64_t radix_hist_entries(int64_t n) { return 256 * (int64_t)nblocks(n, RS_TILE) + 256; }
 
+int64_t segment_block_words(int64_t n) { return ((int64_t)nblocks(n, SEG_TILE) + 1) * (2 + 3 * GVOM_MAX_BATCH); }
+
 hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits,
                       uint32_t *hist, int64_t hwords, uint32_t **kout, uint32_t **vout, hipStream_t s) {
     *kout = k0;
@@ -1219,8 +1224,9 @@ hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, in
 }
 
 hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start, uint32_t *seg_key,
-                           uint32_t *blk, int S, const Geo &g, DevState *st, hipStream_t s) {
-    int nblk = n > 0 ? (int)nblocks(n, RS_TILE) : 0;
+                           uint32_t *blk, int64_t blk_words, int S, const Geo &g, DevState *st, hipStream_t s) {
+    if (segment_block_words(n) > blk_words) return hipErrorInvalidValue;   // blk too small for n keys
+    int nblk = n > 0 ? (int)nblocks(n, SEG_TILE) : 0;
     if (n > 0) {
         k_seg_count<<<nblk, 256, 0, s>>>(skeys, n, S, g, blk);
         k_seg_compact<<<nblk, 256, 0, s>>>(skeys, n, blk, S, g, seg_start, seg_key, st);
