@@ -106,6 +106,20 @@ struct gvom_handle {
     ScanDesc *graph_sd = nullptr;   // device pool the captured kernels read descriptors from
     int64_t graph_sd_cap = 0;
     int mirror_next = 0, mirror_last = 0, mirror_filled = 0, mirror_parity = 0, mirror_have = 0;
+    // Bumped whenever a device buffer a captured sequence may reference is reallocated; a graph
+    // captured under an older generation is refused instead of replayed against freed memory.
+    int64_t buf_gen = 0;
+    // Internal captures read their one descriptor from here instead of the graph pool.
+    ScanDesc *cap_sd_dev = nullptr;
+    // gvom_diff_forward's cached launch sequence (geometry-only, one-slot handles): captured on
+    // the second call with the same points, count and weight, then replayed with a new
+    // descriptor.  dk_*: the previous call's key.
+    hipGraph_t dg_graph = nullptr;
+    hipGraphExec_t dg_exec = nullptr;
+    ScanDesc *dg_sd = nullptr;
+    const void *dg_pts = nullptr, *dk_pts = nullptr;
+    int64_t dg_n = -1, dk_n = -1, dg_gen = -1;
+    double dg_wh = 0.0, dk_wh = 0.0;
 
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -157,6 +171,9 @@ static void free_all(gvom_handle *h) {
                     h->occ_bits, h->zero_page};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (h->dg_exec) (void)hipGraphExecDestroy(h->dg_exec);
+    if (h->dg_graph) (void)hipGraphDestroy(h->dg_graph);
+    if (h->dg_sd) (void)hipFree(h->dg_sd);
     for (auto &e : h->ev_pool)
         if (e) (void)hipEventDestroy(e);
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
@@ -318,6 +335,7 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
         }
         h->slot_recs = nr;
         h->ncap = nc;
+        ++h->buf_gen;
     }
     // Always hold work buffers, also for an empty first scan: the segment pass writes its
     // end sentinel even when there are no points.
@@ -354,6 +372,7 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
         CKH(h, dalloc(&h->rsec, pc));
         CKH(h, dalloc(&h->rorder, pc));
         h->ptcap = pc;
+        ++h->buf_gen;
     }
     return GVOM_OK;
 }
@@ -417,6 +436,7 @@ static int ensure_tile_scratch(gvom_handle *h) {
         return GVOM_ESTATE;
     }
     CKH(h, dalloc(&h->tscratch, words * n));
+    ++h->buf_gen;
     CKH(h, hipMemsetAsync(h->tscratch, 0, (size_t)words * n * sizeof(int32_t), h->stream));
     CKH(h, mark_back(h));
     h->tscratch_scans = n;
@@ -521,15 +541,19 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
             h->err = "only device-pointer process calls can be captured";
             return GVOM_ESTATE;
         }
-        // descriptors come from the graph pool, written by gvom_graph_launch before each replay
-        const int64_t at = (int64_t)h->cap_sd.size();
-        if (at + S > h->graph_sd_cap) {
-            h->err = "too many scans in one captured sequence";
-            return GVOM_ECAP;
+        if (h->cap_sd_dev) {
+            sd = h->cap_sd_dev;   // an internal capture (one scan), its descriptor written before each replay
+        } else {
+            // descriptors come from the graph pool, written by gvom_graph_launch before each replay
+            const int64_t at = (int64_t)h->cap_sd.size();
+            if (at + S > h->graph_sd_cap) {
+                h->err = "too many scans in one captured sequence";
+                return GVOM_ECAP;
+            }
+            sd = h->graph_sd + at;
+            for (int s = 0; s < S; ++s) h->cap_sd.push_back(sdh[s]);
+            h->cap_ops.push_back(S);
         }
-        sd = h->graph_sd + at;
-        for (int s = 0; s < S; ++s) h->cap_sd.push_back(sdh[s]);
-        h->cap_ops.push_back(S);
     } else {
         if (!on_device && NT > 0) {
             CKH(h, hipMemcpyAsync(h->pts_in, xyz, (size_t)NT * 3 * esz, hipMemcpyHostToDevice, fs));
@@ -1047,41 +1071,155 @@ extern "C" int gvom_stage_times(gvom_handle *h, float *ms, int32_t n) {
 }
 
 // ---- differentiable form -------------------------------------------------------------
-extern "C" int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, const double T[16],
-                                 const double ego[3], double w_height, double out[3]) {
-    if (!h || !T || !ego || n < 0) return GVOM_EINVAL;
-    std::lock_guard<std::mutex> lk(h->mu);
-    int64_t off[2] = {0, n};
-    int rc = process_impl(h, points, off, 1, GVOM_F64, ego, T, GVOM_F64, true, false, !h->diff_geometry);
-    if (rc) return rc;
-    const Geo &g = h->g;
-    const int64_t X = g.X;
+// The loss kernels' buffers for n points (a reallocation invalidates captured sequences).
+static int diff_buffers(gvom_handle *h, int64_t n) {
+    const int64_t X = h->g.X;
     if (!h->diff_colh) {
         CKH(h, dalloc(&h->diff_colh, X));
         CKH(h, dalloc(&h->diff_colpt, X));
         CKH(h, dalloc(&h->diff_sums, 16));
+        ++h->buf_gen;
     }
     if (h->diff_rec_cap < h->ncap) {
         if (h->diff_rec) CKH(h, hipFree(h->diff_rec));
         CKH(h, hipMalloc(&h->diff_rec, (size_t)h->ncap * (size_t)diff_rec_bytes()));
         h->diff_rec_cap = h->ncap;
+        ++h->buf_gen;
     }
     const int64_t need_pt = n > 0 ? n : 1;
     if (h->diff_pt_cap < need_pt) {
         if (h->diff_pmark) CKH(h, hipFree(h->diff_pmark));
         CKH(h, dalloc(&h->diff_pmark, need_pt));
         h->diff_pt_cap = need_pt;
+        ++h->buf_gen;
     }
     const int64_t np = diff_partials(need_pt, X, h->ncap);
     if (h->diff_part_cap < np) {
         if (h->diff_part) CKH(h, hipFree(h->diff_part));
         CKH(h, dalloc(&h->diff_part, np));
         h->diff_part_cap = np;
+        ++h->buf_gen;
+    }
+    return GVOM_OK;
+}
+
+static hipError_t enqueue_diff_forward(gvom_handle *h, int64_t n, double w_height) {
+    return launch_diff_forward(h->g, h->st, h->last_slot, h->slot_tables, h->slot_recs, h->ncap, h->seg_start,
+                               h->sorted_vals, (const double *)h->world, n, h->diff_rec, h->diff_colh, h->diff_colpt,
+                               h->diff_pmark, h->diff_part, h->diff_sums, w_height, h->stream);
+}
+
+// Captures the scan stage (no ray cast) and the loss kernels for these points into
+// h->dg_exec, reading the scan descriptor from h->dg_sd.  Nothing runs; the host mirror is
+// left as it was.  On any failure no graph is kept and the caller runs the eager path.
+static void capture_diff_graph(gvom_handle *h, const void *points, int64_t n, const double T[16],
+                               const double ego[3], double w_height) {
+    if (h->dg_exec) (void)hipGraphExecDestroy(h->dg_exec);
+    if (h->dg_graph) (void)hipGraphDestroy(h->dg_graph);
+    h->dg_exec = nullptr;
+    h->dg_graph = nullptr;
+    if (!h->dg_sd && dalloc(&h->dg_sd, 1) != hipSuccess) return;
+    // the previous call had the same point count, so every buffer already holds it
+    if (!h->diff_colh || h->diff_pt_cap < n || h->diff_rec_cap < h->ncap ||
+        h->diff_part_cap < diff_partials(n, h->g.X, h->ncap) || h->ptcap < n)
+        return;
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return;
+    const int m_next = h->next_slot, m_last = h->last_slot, m_filled = h->filled_mask;
+    double m_origin[3], m_ego[3];
+    for (int k = 0; k < 3; ++k) {
+        m_origin[k] = h->slot_origin[0][k];
+        m_ego[k] = h->latest_ego[k];
+    }
+    const std::string m_err = h->err;
+    if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return;
+    h->capturing = true;
+    h->cap_sd_dev = h->dg_sd;
+    int64_t off[2] = {0, n};
+    const int rc = process_impl(h, points, off, 1, GVOM_F64, ego, T, GVOM_F64, true, false, false);
+    const hipError_t e = rc ? hipErrorInvalidValue : enqueue_diff_forward(h, n, w_height);
+    hipGraph_t graph = nullptr;
+    const hipError_t ee = hipStreamEndCapture(h->stream, &graph);
+    h->capturing = false;
+    h->cap_sd_dev = nullptr;
+    (void)mark_back(h);   // ev_back was recorded inside the capture
+    h->next_slot = m_next;
+    h->last_slot = m_last;
+    h->filled_mask = m_filled;
+    for (int k = 0; k < 3; ++k) {
+        h->slot_origin[0][k] = m_origin[k];
+        h->latest_ego[k] = m_ego[k];
+    }
+    h->err = m_err;
+    hipGraphExec_t exec = nullptr;
+    if (rc || e != hipSuccess || ee != hipSuccess || !graph ||
+        hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return;
+    }
+    h->dg_graph = graph;
+    h->dg_exec = exec;
+    h->dg_pts = points;
+    h->dg_n = n;
+    h->dg_wh = w_height;
+    h->dg_gen = h->buf_gen;
+}
+
+extern "C" int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, const double T[16],
+                                 const double ego[3], double w_height, double out[3]) {
+    if (!h || !T || !ego || n < 0) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (h->capturing) {
+        h->err = "gvom_diff_forward synchronises and cannot be captured";
+        return GVOM_ESTATE;
+    }
+    const Geo &g = h->g;
+    // A repeated call (same device points, count and weight) on a geometry-only one-slot
+    // handle replays a captured launch sequence with the new transform and ego: the slot the
+    // loss reads is then always slot 0, and no ray cast (no launch configuration) is involved.
+    const bool graphable = h->diff_geometry && g.B == 1 && !h->timing && n > 0 && points;
+    const bool repeat = points == h->dk_pts && n == h->dk_n && w_height == h->dk_wh;
+    h->dk_pts = points;
+    h->dk_n = n;
+    h->dk_wh = w_height;
+    auto cached = [&]() {
+        return h->dg_exec && h->dg_pts == points && h->dg_n == n && h->dg_wh == w_height && h->dg_gen == h->buf_gen;
+    };
+    if (graphable && repeat && !cached()) capture_diff_graph(h, points, n, T, ego, w_height);
+    if (graphable && repeat && cached()) {
+        for (int k = 0; k < 3; ++k) {
+            const double res = k < 2 ? g.xy_res : g.z_res;
+            if (!std::isfinite(ego[k]) || std::fabs(ego[k] / res) >= 4194304.0) {
+                h->err = "ego position must be finite with |ego / resolution| < 2^22";
+                return GVOM_EINVAL;
+            }
+        }
+        ScanDesc d;
+        std::memset(&d, 0, sizeof(d));
+        d.t_mode = GVOM_F64;
+        for (int k = 0; k < 12; ++k) d.T[k] = T[k];
+        for (int k = 0; k < 3; ++k) d.ego[k] = ego[k];
+        d.offset = 0;
+        d.count = n;
+        h->diff_n = -1;
+        CKH(h, hipMemcpyAsync(h->dg_sd, &d, sizeof(d), hipMemcpyHostToDevice, h->stream));
+        CKH(h, hipGraphLaunch(h->dg_exec, h->stream));
+        CKH(h, mark_back(h));
+        // the ring bookkeeping of one process call on a one-slot ring
+        h->filled_mask |= 1;
+        host_origin(g, ego, h->slot_origin[0]);
+        h->last_slot = 0;
+        h->next_slot = 0;
+        for (int k = 0; k < 3; ++k) h->latest_ego[k] = ego[k];
+    } else {
+        int64_t off[2] = {0, n};
+        int rc = process_impl(h, points, off, 1, GVOM_F64, ego, T, GVOM_F64, true, false, !h->diff_geometry);
+        if (rc) return rc;
+        rc = diff_buffers(h, n);
+        if (rc) return rc;
+        CKH(h, enqueue_diff_forward(h, n, w_height));
     }
     const int slot = h->last_slot;
-    CKH(h, launch_diff_forward(g, h->st, slot, h->slot_tables, h->slot_recs, h->ncap,
-                               h->seg_start, h->sorted_vals, (const double *)h->world, n, h->diff_rec, h->diff_colh,
-                               h->diff_colpt, h->diff_pmark, h->diff_part, h->diff_sums, w_height, h->stream));
     double sums[8];
     CKH(h, hipMemcpyAsync(sums, h->diff_sums, sizeof(sums), hipMemcpyDeviceToHost, h->stream));
     CKH(h, hipStreamSynchronize(h->stream));
@@ -1129,6 +1267,7 @@ struct gvom_graph {
     hipGraphExec_t exec = nullptr;
     std::vector<int> ops;
     std::vector<ScanDesc> sd;
+    int64_t gen = -1;   // the handle's buffer generation at capture
 };
 
 extern "C" int gvom_graph_begin(gvom_handle *h, int32_t max_scans) {
@@ -1150,6 +1289,7 @@ extern "C" int gvom_graph_begin(gvom_handle *h, int32_t max_scans) {
         if (h->graph_sd) CKH(h, hipFree(h->graph_sd));
         CKH(h, dalloc(&h->graph_sd, max_scans));
         h->graph_sd_cap = max_scans;
+        ++h->buf_gen;
     }
     CKH(h, hipStreamSynchronize(h->stream));
     h->cap_ops.clear();
@@ -1194,6 +1334,7 @@ extern "C" int gvom_graph_end(gvom_handle *h, gvom_graph **out) {
         return GVOM_EHIP;
     }
     gr->ops = h->cap_ops;
+    gr->gen = h->buf_gen;
     gr->sd = h->cap_sd;
     *out = gr;
     return GVOM_OK;
@@ -1211,6 +1352,11 @@ extern "C" int gvom_graph_launch(gvom_handle *h, gvom_graph *gr, const double *e
         return GVOM_ESTATE;
     }
     const int n = (int)gr->sd.size();
+    if (gr->gen != h->buf_gen) {
+        h->err = "device buffers were reallocated after this sequence was captured (a larger scan, batch or "
+                 "graph); capture it again";
+        return GVOM_ESTATE;
+    }
     if (n > h->graph_sd_cap) {
         h->err = "graph descriptor pool smaller than the graph";
         return GVOM_ESTATE;

@@ -283,7 +283,9 @@ int gvom_graph_end(gvom_handle *h, gvom_graph **out);
 /* Number of scans recorded in a captured sequence. */
 int gvom_graph_scans(const gvom_graph *g);
 /* Replays the sequence on the handle's stream with new ego positions (scans x 3 doubles)
- * and transforms (scans x 16 values of t_dtype, or NULL).  Does not synchronise. */
+ * and transforms (scans x 16 values of t_dtype, or NULL).  Does not synchronise.  Refused
+ * (GVOM_ESTATE) when the handle reallocated device buffers after the capture (a larger scan,
+ * batch or graph): capture the sequence again. */
 int gvom_graph_launch(gvom_handle *h, gvom_graph *g, const double *ego, const void *T, int32_t t_dtype);
 void gvom_graph_destroy(gvom_graph *g);
 
@@ -324,7 +326,10 @@ int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, const doubl
  * scan's occupied voxels and records but no ray-pass counts (free space reads as unknown,
  * record totals equal hit counts).  The loss and its gradient use neither, and are the same.
  * For handles that only evaluate the loss; a handle whose ring also feeds gvom_combine_maps
- * keeps the default (off).  No counterpart in the reference. */
+ * keeps the default (off).  On such a handle with buffer_size 1, a gvom_diff_forward repeated
+ * with the same device points, count and w_height captures its launch sequence once and then
+ * replays it with the new transform and ego (same results, fewer launches).  No counterpart
+ * in the reference. */
 int gvom_set_diff_geometry_only(gvom_handle *h, int32_t enabled);
 
 /* Gradient of dL * L from the last gvom_diff_forward: grad_points (device, n x 3 f64,

@@ -201,3 +201,34 @@ def test_forward_backward_other_map_shape():
     assert np.isclose(gw, g_ref, rtol=1e-6, atol=1e-9 * scale).mean() >= 0.999
     xi_ref = np.concatenate([g_ref.sum(0), np.cross(world, g_ref).sum(0)])
     np.testing.assert_allclose(gxi, xi_ref, rtol=1e-4, atol=1e-6 * np.abs(xi_ref).max())
+
+
+def test_repeated_forward_replays_match_eager(scene):
+    """A loss handle captures its launch sequence on the second forward with the same device
+    points, count and weight, and replays it after that.  Every forward and backward of the
+    sequence below (new transforms, a change of weight that falls back to eager calls, and
+    back) equals a fresh handle's eager call bit for bit."""
+    import gvom_native as nat
+    from gvom_torch import MapLoss, se3_exp
+    pts, T, ego, _ = scene
+    n = pts.shape[0]
+    rng = np.random.default_rng(11)
+    dev = nat.DeviceArray(pts)
+    gw = nat.DeviceArray(np.zeros_like(pts))
+    loss = MapLoss(XYR, ZR, XY, Z, 1.0, 1, 1, w_height=0.1)
+    plan = [(0.1, T)] + [(0.1, se3_exp(0.003 * rng.normal(size=6)) @ T) for _ in range(3)]
+    plan += [(0.2, T), (0.2, plan[1][1]), (0.1, plan[2][1]), (0.1, T)]
+    for k, (wh, Tk) in enumerate(plan):
+        loss.w_height = wh
+        out = loss.forward(dev.ptr.value, n, Tk, ego)
+        gxi = loss.backward(1.0, None, gw.ptr.value)
+        g1 = gw.to_host()
+        fresh = MapLoss(XYR, ZR, XY, Z, 1.0, 1, 1, w_height=wh)
+        dev2 = nat.DeviceArray(pts)
+        gw2 = nat.DeviceArray(np.zeros_like(pts))
+        out2 = fresh.forward(dev2.ptr.value, n, Tk, ego)
+        gxi2 = fresh.backward(1.0, None, gw2.ptr.value)
+        np.testing.assert_array_equal(out, out2)
+        np.testing.assert_array_equal(gxi, gxi2)
+        np.testing.assert_array_equal(g1, gw2.to_host())
+        print(f"call {k} (w_height {wh}): L {out[0]!r} equal to a fresh handle's", flush=True)

@@ -82,3 +82,34 @@ def test_rejected_calls_leave_the_handle_intact():
         _same(g.combined_state(), ref.combined_state(), f"frame {k} combined")
         _same(g.maps_state(), ref.maps_state(), f"frame {k} maps")
     print("handle unchanged by the rejected calls", flush=True)
+
+
+def test_graph_refused_after_buffers_grow():
+    """A captured sequence refers to the handle's device buffers: after an eager call that
+    reallocates them (a scan larger than any before) its replay is refused with an error,
+    and a sequence captured afterwards replays normally."""
+    import gvom
+    import gvom_native as nat
+    wl = synth.workload(2)
+    s = wl.steps[0][0]
+    g = gvom.Gvom(*wl.map_args)
+    dev = nat.DeviceArray(s.points)
+    g.process_pointcloud_device(dev, s.points.shape[0], s.ego, s.transform)
+    g.combine_maps_device()
+    g.sync()
+    g.graph_begin(1)
+    g.process_pointcloud_device(dev, s.points.shape[0], s.ego, s.transform)
+    g.combine_maps_device()
+    gr = g.graph_end()
+    big = np.concatenate([s.points] * 4)   # more points than the work buffers hold
+    g.Process_pointcloud(big, s.ego, s.transform)
+    with pytest.raises(nat.GvomError, match="reallocated after this sequence was captured"):
+        gr.launch(s.ego[None, :], s.transform[None])
+    g.sync()
+    g.graph_begin(1)
+    g.process_pointcloud_device(dev, s.points.shape[0], s.ego, s.transform)
+    g.combine_maps_device()
+    gr2 = g.graph_end()
+    gr2.launch(s.ego[None, :], s.transform[None])
+    g.sync()
+    print("stale graph refused, recaptured graph replayed", flush=True)
