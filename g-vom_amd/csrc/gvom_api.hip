@@ -30,8 +30,8 @@ hipError_t launch_probe_atomics(int32_t *table, uint32_t V, int blocks, int per_
 hipError_t launch_diff_backward(const Geo &g, const DevState *st, int slot, const ScanDesc *sd,
                                 const double *world, const uint8_t *valid, const int32_t *table, const void *dr,
                                 const int32_t *pmark, const double *colh, const double *sums, double wh, double dL,
-                                int64_t n, double *gsensor, double *gworld, double *part, double *grad6,
-                                hipStream_t s);
+                                const double *dL_dev, int64_t n, double *gsensor, double *gworld, double *part,
+                                double *grad6, hipStream_t s);
 
 namespace {
 thread_local std::string g_create_error;
@@ -52,6 +52,9 @@ struct gvom_handle {
     // the ray cast; forked after k_prepare and joined before the segment sums
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // caller-stream hand-off of the stream-ordered entry points: ev_in on the caller's stream,
+    // waited for by the handle's stream; ev_out the other way
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
     std::mutex mu;
     std::string err;
 
@@ -98,6 +101,9 @@ struct gvom_handle {
     int64_t diff_n = -1;
     int diff_slot = 0;
     double diff_wh = 0.0;
+    // the scan descriptor the last forward's kernels read: h->sd after an eager forward,
+    // h->dg_sd after a replay of the captured forward (the backward reads its transform and ego)
+    const ScanDesc *diff_sd = nullptr;
 
     // hipGraph capture: ops recorded while capturing (process = S scans, combine = -1)
     bool capturing = false;
@@ -178,6 +184,8 @@ static void free_all(gvom_handle *h) {
         if (e) (void)hipEventDestroy(e);
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->ev_in) (void)hipEventDestroy(h->ev_in);
+    if (h->ev_out) (void)hipEventDestroy(h->ev_out);
     if (h->ev_front) (void)hipEventDestroy(h->ev_front);
     if (h->ev_back) (void)hipEventDestroy(h->ev_back);
     if (h->side) (void)hipStreamDestroy(h->side);
@@ -256,6 +264,8 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     CKC(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
     CKC(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
     CKC(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    CKC(hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming));
+    CKC(hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming));
     CKC(hipStreamCreateWithFlags(&h->front, hipStreamNonBlocking));
     CKC(hipEventCreateWithFlags(&h->ev_front, hipEventDisableTiming));
     CKC(hipEventCreateWithFlags(&h->ev_back, hipEventDisableTiming));
@@ -326,6 +336,7 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
         if (nc > g.V) nc = g.V;
         SlotRec *nr = nullptr;
         CKH(h, dalloc(&nr, nc * 2 * g.B));   // two buffers per ring slot
+        ++h->buf_gen;
         if (h->slot_recs) {
             for (int s = 0; s < 2 * g.B; ++s)
                 CKH(h, hipMemcpyAsync(nr + (int64_t)s * nc, h->slot_recs + (int64_t)s * h->ncap,
@@ -335,7 +346,6 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
         }
         h->slot_recs = nr;
         h->ncap = nc;
-        ++h->buf_gen;
     }
     // Always hold work buffers, also for an empty first scan: the segment pass writes its
     // end sentinel even when there are no points.
@@ -348,10 +358,19 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
             return GVOM_EINVAL;
         }
         CKH(h, hipStreamSynchronize(h->stream));
-        void *ptrs[] = {h->pts_in, h->world, h->valid, h->k0, h->v0, h->k1, h->v1, h->hist, h->blk,
-                        h->seg_key, h->seg_start, h->mom, h->part, h->rsec, h->rorder};
-        for (void *q : ptrs)
-            if (q) CKH(h, hipFree(q));
+        // Bumped before anything is freed: a captured sequence is refused from here on, also
+        // when a reallocation below fails and leaves the buffers missing.
+        ++h->buf_gen;
+        h->ptcap = 0;
+        void **ptrs[] = {(void **)&h->pts_in, (void **)&h->world, (void **)&h->valid, (void **)&h->k0,
+                         (void **)&h->v0, (void **)&h->k1, (void **)&h->v1, (void **)&h->hist, (void **)&h->blk,
+                         (void **)&h->seg_key, (void **)&h->seg_start, (void **)&h->mom, (void **)&h->part,
+                         (void **)&h->rsec, (void **)&h->rorder};
+        for (void **q : ptrs) {
+            void *old = *q;
+            *q = nullptr;
+            if (old) CKH(h, hipFree(old));
+        }
         CKH(h, hipMalloc(&h->pts_in, (size_t)pc * 3 * sizeof(double)));
         CKH(h, hipMalloc(&h->world, (size_t)pc * 3 * sizeof(double)));
         CKH(h, dalloc(&h->valid, pc));
@@ -372,7 +391,6 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
         CKH(h, dalloc(&h->rsec, pc));
         CKH(h, dalloc(&h->rorder, pc));
         h->ptcap = pc;
-        ++h->buf_gen;
     }
     return GVOM_OK;
 }
@@ -450,7 +468,6 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
                         const double *ego, const void *T, int t_dtype, bool on_device, bool may_overlap,
                         bool cast = true) {
     const Geo &g = h->g;
-    h->diff_n = -1;   // the work buffers a pending gvom_diff_backward reads are reused below
     if (S < 1 || S > GVOM_MAX_BATCH || S > g.B) {
         h->err = "number of scans must be in [1, min(32, buffer_size)]";
         return GVOM_EINVAL;
@@ -494,7 +511,10 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
         h->err = "batch too large for 32-bit sort keys on this map size";
         return GVOM_EINVAL;
     }
+    // A rejected call leaves a pending gvom_diff_backward valid unless its buffers moved.
+    const int64_t gen0 = h->buf_gen;
     int rc = ensure_capacity(h, max_scan, NT);
+    if (h->buf_gen != gen0) h->diff_n = -1;
     if (rc) return rc;
     if (!h->slot_recs || !h->pts_in || !h->world || !h->valid || !h->k0 || !h->v0 || !h->k1 || !h->v1 ||
         !h->hist || !h->blk || !h->seg_key || !h->seg_start || !h->mom || !h->part || !h->rsec || !h->rorder || h->ptcap < NT || h->ptcap < 1) {
@@ -523,6 +543,7 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     const bool tiled = cast && h->ray_tiled != 0;
     if (tiled) {
         int rc_t = ensure_tile_scratch(h);
+        if (h->buf_gen != gen0) h->diff_n = -1;
         if (rc_t) return rc_t;
     }
     hipStream_t st = h->stream;
@@ -561,6 +582,7 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
         }
         CKH(h, hipMemcpyAsync(h->sd, sdh, sizeof(ScanDesc) * S, hipMemcpyHostToDevice, fs));
     }
+    h->diff_n = -1;   // every check passed: the work buffers a pending gvom_diff_backward reads are reused below
     stage_event(h, 0, fs);
     const bool xcd = cast && h->ray_xcd != 0;
     // the tiled path's conversion writes every word of the slot tables; the direct cast
@@ -1165,10 +1187,24 @@ static void capture_diff_graph(gvom_handle *h, const void *points, int64_t n, co
     h->dg_gen = h->buf_gen;
 }
 
-extern "C" int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, const double T[16],
-                                 const double ego[3], double w_height, double out[3]) {
-    if (!h || !T || !ego || n < 0) return GVOM_EINVAL;
-    std::lock_guard<std::mutex> lk(h->mu);
+// The handle's stream waits for everything enqueued on the caller's stream so far.
+static int join_caller(gvom_handle *h, hipStream_t caller) {
+    CKH(h, hipEventRecord(h->ev_in, caller));
+    CKH(h, hipStreamWaitEvent(h->stream, h->ev_in, 0));
+    return GVOM_OK;
+}
+
+// The caller's stream waits for everything enqueued on the handle's stream so far.
+static int release_caller(gvom_handle *h, hipStream_t caller) {
+    CKH(h, hipEventRecord(h->ev_out, h->stream));
+    CKH(h, hipStreamWaitEvent(caller, h->ev_out, 0));
+    return GVOM_OK;
+}
+
+// out (host) or out_dev (device): {L, L_rough, w_height * Var}.  With `async`, the work is
+// ordered after the caller's stream and the caller's stream after it, with no host sync.
+static int diff_forward_impl(gvom_handle *h, const void *points, int64_t n, const double T[16], const double ego[3],
+                             double w_height, double *out, double *out_dev, bool async, hipStream_t caller) {
     if (h->capturing) {
         h->err = "gvom_diff_forward synchronises and cannot be captured";
         return GVOM_ESTATE;
@@ -1179,13 +1215,15 @@ extern "C" int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, 
     // loss reads is then always slot 0, and no ray cast (no launch configuration) is involved.
     const bool graphable = h->diff_geometry && g.B == 1 && !h->timing && n > 0 && points;
     const bool repeat = points == h->dk_pts && n == h->dk_n && w_height == h->dk_wh;
-    h->dk_pts = points;
-    h->dk_n = n;
-    h->dk_wh = w_height;
     auto cached = [&]() {
         return h->dg_exec && h->dg_pts == points && h->dg_n == n && h->dg_wh == w_height && h->dg_gen == h->buf_gen;
     };
     if (graphable && repeat && !cached()) capture_diff_graph(h, points, n, T, ego, w_height);
+    const ScanDesc *used_sd = h->sd;
+    if (async) {
+        const int rc = join_caller(h, caller);
+        if (rc) return rc;
+    }
     if (graphable && repeat && cached()) {
         for (int k = 0; k < 3; ++k) {
             const double res = k < 2 ? g.xy_res : g.z_res;
@@ -1205,6 +1243,7 @@ extern "C" int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, 
         CKH(h, hipMemcpyAsync(h->dg_sd, &d, sizeof(d), hipMemcpyHostToDevice, h->stream));
         CKH(h, hipGraphLaunch(h->dg_exec, h->stream));
         CKH(h, mark_back(h));
+        used_sd = h->dg_sd;
         // the ring bookkeeping of one process call on a one-slot ring
         h->filled_mask |= 1;
         host_origin(g, ego, h->slot_origin[0]);
@@ -1220,18 +1259,44 @@ extern "C" int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, 
         CKH(h, enqueue_diff_forward(h, n, w_height));
     }
     const int slot = h->last_slot;
-    double sums[8];
-    CKH(h, hipMemcpyAsync(sums, h->diff_sums, sizeof(sums), hipMemcpyDeviceToHost, h->stream));
-    CKH(h, hipStreamSynchronize(h->stream));
-    if (out) {
-        out[0] = sums[4];
-        out[1] = sums[5];
-        out[2] = sums[6];
+    if (async) {
+        if (out_dev)
+            CKH(h, hipMemcpyAsync(out_dev, h->diff_sums + 4, 3 * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        const int rc = release_caller(h, caller);
+        if (rc) return rc;
+    } else {
+        double sums[8];
+        CKH(h, hipMemcpyAsync(sums, h->diff_sums, sizeof(sums), hipMemcpyDeviceToHost, h->stream));
+        CKH(h, hipStreamSynchronize(h->stream));
+        if (out) {
+            out[0] = sums[4];
+            out[1] = sums[5];
+            out[2] = sums[6];
+        }
     }
     h->diff_n = n;
     h->diff_slot = slot;
     h->diff_wh = w_height;
+    h->diff_sd = used_sd;
+    // the repeat key changes only when the call succeeded
+    h->dk_pts = points;
+    h->dk_n = n;
+    h->dk_wh = w_height;
     return GVOM_OK;
+}
+
+extern "C" int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, const double T[16],
+                                 const double ego[3], double w_height, double out[3]) {
+    if (!h || !T || !ego || n < 0) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    return diff_forward_impl(h, points, n, T, ego, w_height, out, nullptr, false, nullptr);
+}
+
+extern "C" int gvom_diff_forward_async(gvom_handle *h, const void *points, int64_t n, const double T[16],
+                                       const double ego[3], double w_height, double *out_dev, void *stream) {
+    if (!h || !T || !ego || n < 0) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    return diff_forward_impl(h, points, n, T, ego, w_height, nullptr, out_dev, true, (hipStream_t)stream);
 }
 
 extern "C" int gvom_set_diff_geometry_only(gvom_handle *h, int32_t enabled) {
@@ -1241,24 +1306,48 @@ extern "C" int gvom_set_diff_geometry_only(gvom_handle *h, int32_t enabled) {
     return GVOM_OK;
 }
 
-extern "C" int gvom_diff_backward(gvom_handle *h, double dL, double *grad_points, double *grad_world,
-                                  double grad_xi[6]) {
-    if (!h) return GVOM_EINVAL;
-    std::lock_guard<std::mutex> lk(h->mu);
-    if (h->diff_n < 0) {
+// dL_dev (device scalar) replaces dL when not null; grad_xi (host) or grad_xi_dev (device).
+static int diff_backward_impl(gvom_handle *h, double dL, const double *dL_dev, double *grad_points,
+                              double *grad_world, double *grad_xi, double *grad_xi_dev, bool async,
+                              hipStream_t caller) {
+    if (h->diff_n < 0 || !h->diff_sd) {
         h->err = "gvom_diff_backward needs a preceding gvom_diff_forward";
         return GVOM_ESTATE;
     }
+    if (async) {
+        const int rc = join_caller(h, caller);
+        if (rc) return rc;
+    }
     double *g6 = h->diff_sums + 8;
-    CKH(h, launch_diff_backward(h->g, h->st, h->diff_slot, h->sd, (const double *)h->world, h->valid, h->slot_tables,
-                                h->diff_rec, h->diff_pmark, h->diff_colh, h->diff_sums, h->diff_wh, dL, h->diff_n,
-                                grad_points, grad_world, h->diff_part, g6, h->stream));
+    CKH(h, launch_diff_backward(h->g, h->st, h->diff_slot, h->diff_sd, (const double *)h->world, h->valid, h->slot_tables,
+                                h->diff_rec, h->diff_pmark, h->diff_colh, h->diff_sums, h->diff_wh, dL, dL_dev,
+                                h->diff_n, grad_points, grad_world, h->diff_part, g6, h->stream));
+    if (async) {
+        if (grad_xi_dev)
+            CKH(h, hipMemcpyAsync(grad_xi_dev, g6, 6 * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        return release_caller(h, caller);
+    }
     double gx[6];
     CKH(h, hipMemcpyAsync(gx, g6, sizeof(gx), hipMemcpyDeviceToHost, h->stream));
     CKH(h, hipStreamSynchronize(h->stream));
     if (grad_xi)
         for (int k = 0; k < 6; ++k) grad_xi[k] = gx[k];
     return GVOM_OK;
+}
+
+extern "C" int gvom_diff_backward(gvom_handle *h, double dL, double *grad_points, double *grad_world,
+                                  double grad_xi[6]) {
+    if (!h) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    return diff_backward_impl(h, dL, nullptr, grad_points, grad_world, grad_xi, nullptr, false, nullptr);
+}
+
+extern "C" int gvom_diff_backward_async(gvom_handle *h, const double *dL_dev, double *grad_points,
+                                        double *grad_world, double *grad_xi_dev, void *stream) {
+    if (!h || !dL_dev) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    return diff_backward_impl(h, 0.0, dL_dev, grad_points, grad_world, nullptr, grad_xi_dev, true,
+                              (hipStream_t)stream);
 }
 
 // ---- hipGraph capture of a launch sequence --------------------------------------------

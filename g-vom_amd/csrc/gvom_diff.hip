@@ -281,7 +281,8 @@ __global__ __launch_bounds__(RB) void k_diff_backward(Geo g, const DevState *__r
                                                       const int32_t *__restrict__ table,
                                                       const DiffRec *__restrict__ dr, const int32_t *__restrict__ pmark,
                                                       const double *__restrict__ colh, const double *__restrict__ sums,
-                                                      double wh, double dL, int64_t n, double *__restrict__ gsensor,
+                                                      double wh, double dL_arg, const double *__restrict__ dL_dev,
+                                                      int64_t n, double *__restrict__ gsensor,
                                                       double *__restrict__ gworld, double *__restrict__ part6) {
     __shared__ double sh6[6][RB / 64];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -357,6 +358,7 @@ __global__ __launch_bounds__(RB) void k_diff_backward(Geo g, const DevState *__r
             const double K = sums[1], mean = sums[2] / K;
             gz += wh * 2.0 * (colh[c] - mean) / K;
         }
+        const double dL = dL_dev ? *dL_dev : dL_arg;
         gx *= dL;
         gy *= dL;
         gz *= dL;
@@ -439,11 +441,11 @@ hipError_t launch_diff_forward(const Geo &g, const DevState *st, int slot, const
 hipError_t launch_diff_backward(const Geo &g, const DevState *st, int slot, const ScanDesc *sd,
                                 const double *world, const uint8_t *valid, const int32_t *table, const void *dr,
                                 const int32_t *pmark, const double *colh, const double *sums, double wh, double dL,
-                                int64_t n, double *gsensor, double *gworld, double *part, double *grad6,
-                                hipStream_t s) {
+                                const double *dL_dev, int64_t n, double *gsensor, double *gworld, double *part,
+                                double *grad6, hipStream_t s) {
     const unsigned np = nb(n, RB);
     k_diff_backward<<<np, RB, 0, s>>>(g, st, slot, sd, world, valid, table, (const DiffRec *)dr, pmark, colh, sums,
-                                      wh, dL, n, gsensor, gworld, part);
+                                      wh, dL, dL_dev, n, gsensor, gworld, part);
     SumJobs j6{};
     for (int k = 0; k < 6; ++k) {
         j6.part[k] = part + (int64_t)k * np;

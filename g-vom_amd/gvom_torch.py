@@ -69,6 +69,24 @@ class MapLoss:
         self.n = int(n)
         return out
 
+    def forward_async(self, points_ptr, n, T, ego, out_ptr, stream):
+        """Stream-ordered forward: ordered after the work on `stream` (a HIP stream handle,
+        0 = null stream), which is ordered after it; writes {L, L_rough, w Var} to the device
+        array at out_ptr (3 float64).  No host synchronisation."""
+        T = np.ascontiguousarray(np.asarray(T, np.float64).reshape(4, 4))
+        ego = np.ascontiguousarray(np.asarray(ego, np.float64).reshape(3))
+        nat.check(self.g._lib.gvom_diff_forward_async(self.g._h, ctypes.c_void_p(int(points_ptr)), int(n), nat.ptr(T),
+                                                      nat.ptr(ego), self.w_height, ctypes.c_void_p(int(out_ptr)),
+                                                      ctypes.c_void_p(int(stream))), self.g._h)
+        self.n = int(n)
+
+    def backward_async(self, dL_ptr, grad_points_ptr, grad_world_ptr, grad_xi_ptr, stream):
+        """Stream-ordered backward; dL_ptr: device float64 scalar, grad_xi_ptr: device 6 float64."""
+        def vp(x):
+            return None if x is None else ctypes.c_void_p(int(x))
+        nat.check(self.g._lib.gvom_diff_backward_async(self.g._h, vp(dL_ptr), vp(grad_points_ptr), vp(grad_world_ptr),
+                                                       vp(grad_xi_ptr), ctypes.c_void_p(int(stream))), self.g._h)
+
     def backward(self, dL=1.0, grad_points_ptr=None, grad_world_ptr=None):
         gxi = np.zeros(6)
         gp = None if grad_points_ptr is None else ctypes.c_void_p(int(grad_points_ptr))
@@ -86,23 +104,34 @@ def _function():
                             f"({', '.join(runtimes)}); import torch before gvom so that both bind to one")
 
     class _MapLossFn(torch.autograd.Function):
+        """The library's stream-ordered entry points on torch's current stream: the loss is a
+        0-d float64 tensor on the points' device, and neither direction synchronises the host
+        (only xi's host copy, for the transform, does when xi lives on the device)."""
+
         @staticmethod
         def forward(ctx, points, xi, T0, ego, loss):
             if points.dtype != torch.float64 or not points.is_cuda or not points.is_contiguous():
                 raise ValueError("points must be a contiguous float64 CUDA tensor (N, 3)")
             T = se3_exp(xi.detach().cpu().numpy()) @ np.asarray(T0, np.float64)
-            torch.cuda.synchronize(points.device)   # the library runs on its own stream
-            out = loss.forward(points.data_ptr(), points.shape[0], T, ego)
+            stream = torch.cuda.current_stream(points.device).cuda_stream
+            out = torch.empty(3, dtype=torch.float64, device=points.device)
+            loss.forward_async(points.data_ptr(), points.shape[0], T, ego, out.data_ptr(), stream)
             ctx.loss = loss
+            ctx.xi_device = xi.device
+            ctx.xi_dtype = xi.dtype
             ctx.save_for_backward(points)
-            return torch.tensor(out[0], dtype=torch.float64)
+            return out[0].clone()
 
         @staticmethod
         def backward(ctx, gL):
             (points,) = ctx.saved_tensors
             gp = torch.empty_like(points) if ctx.needs_input_grad[0] else None
-            gxi = ctx.loss.backward(float(gL), None if gp is None else gp.data_ptr())
-            return gp, torch.from_numpy(gxi), None, None, None
+            gL = gL.to(device=points.device, dtype=torch.float64).contiguous()
+            gxi = torch.empty(6, dtype=torch.float64, device=points.device)
+            stream = torch.cuda.current_stream(points.device).cuda_stream
+            ctx.loss.backward_async(gL.data_ptr(), None if gp is None else gp.data_ptr(), None, gxi.data_ptr(),
+                                    stream)
+            return gp, gxi.to(device=ctx.xi_device, dtype=ctx.xi_dtype), None, None, None
 
     return _MapLossFn
 

@@ -337,6 +337,18 @@ int gvom_set_diff_geometry_only(gvom_handle *h, int32_t enabled);
  * grad_xi (host, 6 doubles) receives dL/d(rho, phi) for T' = exp(xi^) T.  Synchronises. */
 int gvom_diff_backward(gvom_handle *h, double dL, double *grad_points, double *grad_world, double grad_xi[6]);
 
+/* Stream-ordered forms of gvom_diff_forward / gvom_diff_backward for an autograd boundary
+ * (no reference counterpart; the reference hands arrays to the device at gvom.py:115).
+ * `stream` is the caller's HIP stream (NULL = the null stream): the handle's work waits for
+ * everything enqueued on it so far, and it waits for the handle's work, with no host
+ * synchronisation.  out_dev: device, 3 doubles {L, L_rough, w_height * Var} (may be NULL);
+ * dL_dev: device scalar read by the backward kernel; grad_xi_dev: device, 6 doubles (may be
+ * NULL).  Results equal the synchronous forms' bit for bit. */
+int gvom_diff_forward_async(gvom_handle *h, const void *points, int64_t n, const double T[16],
+                            const double ego[3], double w_height, double *out_dev, void *stream);
+int gvom_diff_backward_async(gvom_handle *h, const double *dL_dev, double *grad_points, double *grad_world,
+                             double *grad_xi_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,16 @@ typedef struct {
 #define ORC_F64 1
 #define ORC_NONE (-1)
 
+/* Planted mistakes, for the pinning tests only: they show that each closed-form pin fails
+ * when the oracle makes the mistake it guards against.  0 (the default) = none;
+ *   1 = transform with the transposed rotation (gvom.py:1126-1134),
+ *   2 = combine shift with the sign of d reversed (gvom.py:1017-1026, 1044-1053),
+ *   3 = inferred height one voxel too high (gvom.py:587),
+ *   4 = robot-radius test on the cell centre instead of its corner (gvom.py:566-567),
+ *   5 = transform without the translation column (gvom.py:1128-1134). */
+static int g_planted = 0;
+void orc_set_planted_mistake(int m) { g_planted = m; }
+
 static inline double ld(const void *base, int dt, int64_t i) {
     return dt == ORC_F32 ? (double)((const float *)base)[i] : ((const double *)base)[i];
 }
@@ -67,7 +77,15 @@ static void transform_points(void *pts, int dt, int64_t n, const void *T, int td
     for (int64_t i = 0; i < n; ++i) {
         if (dt == ORC_F32 && tdt == ORC_F32) {
             float *q = (float *)pts + 3 * i;
-            const float *t = (const float *)T;
+            float t[12];
+            for (int k = 0; k < 12; ++k) t[k] = ((const float *)T)[k];
+            if (g_planted == 1) {
+                float u[12];
+                for (int k = 0; k < 12; ++k) u[k] = t[k];
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) t[4 * r + c] = u[4 * c + r];
+            }
+            if (g_planted == 5) t[3] = t[7] = t[11] = 0.0f;
             float x = q[0], y = q[1], z = q[2];
             float r0 = x * t[0] + y * t[1] + z * t[2] + t[3];
             float r1 = x * t[4] + y * t[5] + z * t[6] + t[7];
@@ -77,6 +95,13 @@ static void transform_points(void *pts, int dt, int64_t n, const void *T, int td
             double x = ld(pts, dt, 3 * i), y = ld(pts, dt, 3 * i + 1), z = ld(pts, dt, 3 * i + 2);
             double t[12];
             for (int k = 0; k < 12; ++k) t[k] = ld(T, tdt, k);
+            if (g_planted == 1) {
+                double u[12];
+                for (int k = 0; k < 12; ++k) u[k] = t[k];
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) t[4 * r + c] = u[4 * c + r];
+            }
+            if (g_planted == 5) t[3] = t[7] = t[11] = 0.0;
             double r0 = x * t[0] + y * t[1] + z * t[2] + t[3];
             double r1 = x * t[4] + y * t[5] + z * t[6] + t[7];
             double r2 = x * t[8] + y * t[9] + z * t[10] + t[11];
@@ -327,6 +352,11 @@ static void combine_indices(const orc_params *p, const double co[3], int32_t *co
                             const int32_t *old, const double oo[3], int64_t *counter, int is_old) {
     const int64_t XY = p->xy_size, Z = p->z_size;
     double dx = co[0] - oo[0], dy = co[1] - oo[1], dz = co[2] - oo[2];
+    if (g_planted == 2) {
+        dx = -dx;
+        dy = -dy;
+        dz = -dz;
+    }
     for (int64_t z = 0; z < Z; ++z)
         for (int64_t y = 0; y < XY; ++y)
             for (int64_t x = 0; x < XY; ++x) {
@@ -384,6 +414,11 @@ static void combine_metrics_d(const orc_params *p, const double co[3], const int
                               const int32_t *ohit, const int32_t *otot, const float *omh) {
     const int64_t XY = p->xy_size, Z = p->z_size;
     double dx = co[0] - oo[0], dy = co[1] - oo[1], dz = co[2] - oo[2];
+    if (g_planted == 2) {
+        dx = -dx;
+        dy = -dy;
+        dz = -dz;
+    }
     for (int64_t z = 0; z < Z; ++z)
         for (int64_t y = 0; y < XY; ++y)
             for (int64_t x = 0; x < XY; ++x) {
@@ -403,6 +438,11 @@ static void combine_metrics_f(const orc_params *p, const double co[3], const int
                               const int32_t *ohit, const int32_t *otot, const float *omh) {
     const int64_t XY = p->xy_size, Z = p->z_size;
     double dx = co[0] - oo[0], dy = co[1] - oo[1], dz = co[2] - oo[2];
+    if (g_planted == 2) {
+        dx = -dx;
+        dy = -dy;
+        dz = -dz;
+    }
     for (int64_t z = 0; z < Z; ++z)
         for (int64_t y = 0; y < XY; ++y)
             for (int64_t x = 0; x < XY; ++x) {
@@ -622,8 +662,9 @@ int64_t orc_combine(const orc_params *p, int32_t nslots, const orc_source *slots
             int64_t c = x * XY + y;
             height[c] = -1000.0;
             inferred[c] = -1000.0;
-            double xp = ((co[0] + (double)x) * xyr) - ego[0];
-            double yp = ((co[1] + (double)y) * xyr) - ego[1];
+            const double cc = g_planted == 4 ? 0.5 : 0.0;
+            double xp = ((co[0] + (double)x + cc) * xyr) - ego[0];
+            double yp = ((co[1] + (double)y + cc) * xyr) - ego[1];
             if (xp * xp + yp * yp <= p->robot_radius * p->robot_radius)
                 height[c] = ego[2] - p->ground_to_lidar_height;
             for (int64_t z = 0; z < Z; ++z) {
@@ -636,7 +677,7 @@ int64_t orc_combine(const orc_params *p, int32_t nslots, const orc_source *slots
             for (int64_t z = 0; z < Z; ++z) {
                 int32_t id = comb[ref_lin(x, y, z, XY)];
                 if (id < -1) {
-                    inferred[c] = ((double)z + co[2]) * zr;
+                    inferred[c] = ((double)(g_planted == 3 ? z + 1 : z) + co[2]) * zr;
                     break;
                 }
             }

@@ -70,8 +70,37 @@ def load_library():
     lib.orc_eigenvalues.argtypes = [P, P, ctypes.c_int64]
     lib.orc_slope_guess.restype = None
     lib.orc_slope_guess.argtypes = [P, P, P, P, P, P, P]
+    lib.orc_set_planted_mistake.restype = None
+    lib.orc_set_planted_mistake.argtypes = [ctypes.c_int]
     _LIB = lib
     return lib
+
+
+# Planted mistakes (pinning tests only): a pin is shown to fail when the oracle makes the
+# mistake it guards against.  1-5 live in the C restatement (see gvom_oracle.c), 6-7 in
+# the debug exporters below: 6 = hit ratio inverted (total / hit), 7 = planarity column
+# taken as lambda0 - lambda2 (gvom.py:492-499), 8 = height-map rows indexed x * xy + y
+# instead of x + y * xy (gvom.py:460, 475).
+_PLANTED = 0
+
+
+class planted_mistake:
+    """Context manager: run the oracle with planted mistake m (0 = none)."""
+
+    def __init__(self, m):
+        self.m = int(m)
+
+    def __enter__(self):
+        global _PLANTED
+        _PLANTED = self.m
+        load_library().orc_set_planted_mistake(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        global _PLANTED
+        _PLANTED = 0
+        load_library().orc_set_planted_mistake(0)
+        return False
 
 
 def _ptr(a):
@@ -269,10 +298,13 @@ def debug_voxel_map(cm: CombinedMap, xy, z, xyr, zr):
     out[ids, 0] = (x + cm.origin[0]) * xyr
     out[ids, 1] = (y + cm.origin[1]) * xyr
     out[ids, 2] = (zz + cm.origin[2]) * zr
-    out[ids, 3] = cm.hit[ids].astype(np.float64) / cm.total[ids].astype(np.float64)
+    if _PLANTED == 6:
+        out[ids, 3] = cm.total[ids].astype(np.float64) / cm.hit[ids].astype(np.float64)
+    else:
+        out[ids, 3] = cm.hit[ids].astype(np.float64) / cm.total[ids].astype(np.float64)
     out[ids, 4] = cm.hit[ids]
     e = cm.eigen[ids]
-    out[ids, 5] = e[:, 0] - e[:, 1]
+    out[ids, 5] = e[:, 0] - (e[:, 2] if _PLANTED == 7 else e[:, 1])
     out[ids, 6] = e[:, 1] - e[:, 2]
     out[ids, 7] = e[:, 2]
     return out
@@ -282,7 +314,7 @@ def debug_height_map(cm: CombinedMap, xy, xyr, zr):
     """__make_height_map_pointcloud (gvom.py:456-468): row index x + y*xy."""
     out = np.zeros((xy * xy, 7), np.float32)
     x, y = np.meshgrid(np.arange(xy), np.arange(xy), indexing="ij")
-    index = (x + y * xy).ravel()
+    index = (x * xy + y if _PLANTED == 8 else x + y * xy).ravel()
     out[index, 0] = ((x + cm.origin[0]) * xyr).ravel()
     out[index, 1] = ((y + cm.origin[1]) * xyr).ravel()
     out[index, 2] = (cm.height - zr).ravel()
@@ -298,7 +330,7 @@ def debug_inferred_height_map(cm: CombinedMap, xy, xyr, zr):
     height delta as its height input (gvom.py:439-440)."""
     out = np.zeros((xy * xy, 3), np.float32)
     x, y = np.meshgrid(np.arange(xy), np.arange(xy), indexing="ij")
-    index = (x + y * xy).ravel()
+    index = (x * xy + y if _PLANTED == 8 else x + y * xy).ravel()
     out[index, 0] = ((x + cm.origin[0]) * xyr).ravel()
     out[index, 1] = ((y + cm.origin[1]) * xyr).ravel()
     out[index, 2] = (cm.delta - zr).ravel()

@@ -206,8 +206,10 @@ def test_forward_backward_other_map_shape():
 def test_repeated_forward_replays_match_eager(scene):
     """A loss handle captures its launch sequence on the second forward with the same device
     points, count and weight, and replays it after that.  Every forward and backward of the
-    sequence below (new transforms, a change of weight that falls back to eager calls, and
-    back) equals a fresh handle's eager call bit for bit."""
+    sequence below (new transforms, ego moves across voxel boundaries in x, y and z, a change
+    of weight that falls back to eager calls, and back) equals a fresh handle's eager call bit
+    for bit, including the sensor-frame point gradient, which the backward rotates with the
+    replayed call's transform and whose neighbour voxels follow the replayed call's ego."""
     import gvom_native as nat
     from gvom_torch import MapLoss, se3_exp
     pts, T, ego, _ = scene
@@ -215,20 +217,27 @@ def test_repeated_forward_replays_match_eager(scene):
     rng = np.random.default_rng(11)
     dev = nat.DeviceArray(pts)
     gw = nat.DeviceArray(np.zeros_like(pts))
+    gp = nat.DeviceArray(np.zeros_like(pts))
     loss = MapLoss(XYR, ZR, XY, Z, 1.0, 1, 1, w_height=0.1)
-    plan = [(0.1, T)] + [(0.1, se3_exp(0.003 * rng.normal(size=6)) @ T) for _ in range(3)]
-    plan += [(0.2, T), (0.2, plan[1][1]), (0.1, plan[2][1]), (0.1, T)]
-    for k, (wh, Tk) in enumerate(plan):
+    ego = np.asarray(ego, np.float64)
+    # ego steps of 1.5 voxels in x, -2.5 in y and 1.5 in z move the map origin every call
+    moves = [np.array([0.6 * k, -1.0 * k, 0.3 * k]) for k in range(4)]
+    plan = [(0.1, T, ego)] + [(0.1, se3_exp(0.003 * rng.normal(size=6)) @ T, ego + moves[k]) for k in (1, 2, 3)]
+    plan += [(0.2, T, ego), (0.2, plan[1][1], ego + moves[1]), (0.1, plan[2][1], ego + moves[2]), (0.1, T, ego)]
+    for k, (wh, Tk, ek) in enumerate(plan):
         loss.w_height = wh
-        out = loss.forward(dev.ptr.value, n, Tk, ego)
-        gxi = loss.backward(1.0, None, gw.ptr.value)
+        out = loss.forward(dev.ptr.value, n, Tk, ek)
+        gxi = loss.backward(1.0, gp.ptr.value, gw.ptr.value)
         g1 = gw.to_host()
+        p1 = gp.to_host()
         fresh = MapLoss(XYR, ZR, XY, Z, 1.0, 1, 1, w_height=wh)
         dev2 = nat.DeviceArray(pts)
         gw2 = nat.DeviceArray(np.zeros_like(pts))
-        out2 = fresh.forward(dev2.ptr.value, n, Tk, ego)
-        gxi2 = fresh.backward(1.0, None, gw2.ptr.value)
+        gp2 = nat.DeviceArray(np.zeros_like(pts))
+        out2 = fresh.forward(dev2.ptr.value, n, Tk, ek)
+        gxi2 = fresh.backward(1.0, gp2.ptr.value, gw2.ptr.value)
         np.testing.assert_array_equal(out, out2)
         np.testing.assert_array_equal(gxi, gxi2)
         np.testing.assert_array_equal(g1, gw2.to_host())
-        print(f"call {k} (w_height {wh}): L {out[0]!r} equal to a fresh handle's", flush=True)
+        np.testing.assert_array_equal(p1, gp2.to_host())
+        print(f"call {k} (w_height {wh}, ego {ek}): L {out[0]!r} equal to a fresh handle's", flush=True)

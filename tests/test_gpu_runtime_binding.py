@@ -119,18 +119,40 @@ out = ml.forward(dev.ptr.value, pts.shape[0], T0, s.ego)
 gxi = ml.backward(1.0, gdev.ptr.value)
 g_direct = gdev.to_host()
 print("L", float(L), out[0], flush=True)
-assert abs(float(L) - out[0]) <= 1e-12 * abs(out[0])
-np.testing.assert_allclose(gx, gxi, rtol=1e-10, atol=1e-12 * np.abs(gxi).max())
-np.testing.assert_allclose(gp, g_direct, rtol=1e-10, atol=1e-12 * np.abs(g_direct).max())
+assert L.is_cuda and float(L) == out[0]
+np.testing.assert_array_equal(gx, gxi)
+np.testing.assert_array_equal(gp, g_direct)
 assert np.abs(gp).max() > 0
+# xi on the device and an upstream gradient of 2: on a side stream whose points come from a
+# kernel just enqueued there (the library's work must be ordered after it and before the
+# gradient reads), without host synchronisation inside map_loss
+ml2 = MapLoss(0.4, 0.2, 256, 64)
+s1 = torch.cuda.Stream()
+base = torch.tensor(pts, device="cuda:0")
+torch.cuda.synchronize()
+with torch.cuda.stream(s1):
+    p3 = (base * 1.0).requires_grad_()
+    xi3 = torch.zeros(6, dtype=torch.float64, device="cuda:0", requires_grad=True)
+    L3 = map_loss(p3, xi3, T0, s.ego, ml2)
+    (2.0 * L3).backward()
+    gp3 = p3.grad * 1.0
+    gx3 = xi3.grad * 1.0
+s1.synchronize()
+gxi2 = ml.backward(2.0, gdev.ptr.value)
+assert xi3.grad.is_cuda
+assert float(L3) == out[0]
+np.testing.assert_array_equal(gx3.cpu().numpy(), gxi2)
+np.testing.assert_array_equal(gp3.cpu().numpy(), gdev.to_host())
 print("ok", flush=True)
 """
 
 
 def test_map_loss_autograd_with_torch_cuda_tensors():
     """torch imported first (one HIP runtime for torch and the library): map_loss over a
-    float64 CUDA tensor gives the loss, the point gradient (a CUDA tensor) and the se(3)
-    gradient of direct MapLoss forward/backward calls on the same scan."""
+    float64 CUDA tensor, on torch's current stream through the stream-ordered entry points,
+    gives the loss, the point gradient (a CUDA tensor) and the se(3) gradient of direct
+    synchronous MapLoss forward/backward calls on the same scan, bit for bit; also with xi on
+    the device, an upstream gradient of 2 and a non-default stream."""
     code = CHILD_AUTOGRAD.format(pkg=os.path.join(ROOT, "g-vom_amd"), root=ROOT)
     print("child: torch imported first, map_loss on cuda:0", flush=True)
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
