@@ -141,6 +141,9 @@ struct gvom_handle {
     int ray_tiled = 1;
     int32_t *tscratch = nullptr;
     int tscratch_scans = 0;
+    int64_t tscratch_words = 0;
+    // words of the buffers whose launches the host checks against their footprint
+    int64_t slot_table_words = 0, col_count_words = 0, chunk_tot_words = 0;
     // Overlap of a device-pointer scan stage with the preceding combine (gvom_set_overlap):
     // the stage's first part (transform, keys, ray cast into the tiled scratch, sort chain)
     // runs on `front`, which waits only for ev_back; ev_back is recorded on the main stream
@@ -222,7 +225,7 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     if (p.xy_eigen_dist < 0 || p.xy_eigen_dist > 8 || p.z_eigen_dist < 0 || p.z_eigen_dist > 8)
         return fail_create(nullptr, "eigen distances must be in [0, 8]", GVOM_EINVAL);
     const int64_t XY = p.xy_size, Z = p.z_size, V = XY * XY * Z;
-    if (V * (int64_t)(p.buffer_size + 1) >= ((int64_t)1 << 31))
+    if (!gvom_plan::params_in_domain(p.xy_size, p.z_size, p.buffer_size))
         return fail_create(nullptr, "map too large: voxels * (buffer_size + 1) must be < 2^31", GVOM_EINVAL);
 
     gvom_handle *h = new gvom_handle();
@@ -272,7 +275,8 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     CKC(hipEventRecord(h->ev_back, h->stream));
     CKC(dalloc(&h->st, 1));
     CKC(hipMemset(h->st, 0, sizeof(DevState)));
-    CKC(dalloc(&h->slot_tables, V * 2 * g.B));   // two buffers per ring slot
+    h->slot_table_words = gvom_plan::alloc_slot_table_words(V, g.B);   // two buffers per ring slot
+    CKC(dalloc(&h->slot_tables, h->slot_table_words));
     h->ccap = V;
     CKC(dalloc(&h->comb_tables, 2 * V));
     CKC(dalloc(&h->comb_recs, 2 * h->ccap));
@@ -288,11 +292,13 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     CKC(dalloc(&h->neg, X));
     CKC(dalloc(&h->vis, X));
     CKC(dalloc(&h->occ_list, V));
-    CKC(dalloc(&h->col_count, X + 16));
+    h->col_count_words = gvom_plan::alloc_col_count_words(X);
+    CKC(dalloc(&h->col_count, h->col_count_words));
     CKC(dalloc(&h->occ_bits, X * ((g.Z + 63) / 64)));
     CKC(dalloc(&h->zero_page, zero_page_words()));
     CKC(hipMemset(h->zero_page, 0, sizeof(int32_t) * zero_page_words()));
-    CKC(dalloc(&h->chunk_tot, col_chunks(X) + 1));
+    h->chunk_tot_words = gvom_plan::alloc_chunk_tot_words(X);
+    CKC(dalloc(&h->chunk_tot, h->chunk_tot_words));
     CKC(dalloc(&h->sd, GVOM_MAX_BATCH));
     CKC(hipDeviceSynchronize());   // the fills above ran on the null stream
 #undef CKC
@@ -440,12 +446,15 @@ static void stage_event(gvom_handle *h, int k, hipStream_t on = nullptr) {
 // next overlapped scan stage waits for it.
 static hipError_t mark_back(gvom_handle *h) { return hipEventRecord(h->ev_back, h->stream); }
 
+// Maps whose tiled scratch keys would not fit in 31 bits use the direct cast instead.
+static bool tiled_fits(const Geo &g) { return gvom_plan::tiled_cast_fits(g.XY, g.Z, g.B); }
+
 static int ensure_tile_scratch(gvom_handle *h) {
     if (h->tscratch) return GVOM_OK;
     const Geo &g = h->g;
-    const int n = g.B < GVOM_MAX_BATCH ? g.B : GVOM_MAX_BATCH;
+    const int n = gvom_plan::tile_tables(g.B);
     const int64_t words = tile_scratch_words(g);
-    if (words * n >= ((int64_t)1 << 31)) {
+    if (!tiled_fits(g)) {
         h->err = "map too large for the tiled ray cast";
         return GVOM_EINVAL;
     }
@@ -458,6 +467,7 @@ static int ensure_tile_scratch(gvom_handle *h) {
     CKH(h, hipMemsetAsync(h->tscratch, 0, (size_t)words * n * sizeof(int32_t), h->stream));
     CKH(h, mark_back(h));
     h->tscratch_scans = n;
+    h->tscratch_words = words * n;
     return GVOM_OK;
 }
 
@@ -540,7 +550,7 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
         d.offset = offsets[s];
         d.count = offsets[s + 1] - offsets[s];
     }
-    const bool tiled = cast && h->ray_tiled != 0;
+    const bool tiled = cast && h->ray_tiled != 0 && tiled_fits(g);
     if (tiled) {
         int rc_t = ensure_tile_scratch(h);
         if (h->buf_gen != gen0) h->diff_n = -1;
@@ -605,10 +615,13 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     stage_event(h, 1, fs);
     if (cast)
         CKH(h, launch_raycast(dtype, h->world, h->valid, sd, S, NT, g, h->st, tiled ? h->tscratch : h->slot_tables,
+                              tiled ? h->tscratch_words : h->slot_table_words,
                               h->ray_block, h->ray_rpt, tiled ? 0 : h->ray_box, tiled ? 1 : 0, -1,
                               xcd ? h->rorder : nullptr, h->ray_spread, fs));
     stage_event(h, 9, fs);
-    if (tiled) CKH(h, launch_tile_convert(g, h->st, S, -1, h->tscratch, h->slot_tables, fs));
+    if (tiled)
+        CKH(h, launch_tile_convert(g, h->st, S, -1, h->tscratch, h->tscratch_words, h->slot_tables,
+                                   h->slot_table_words, fs));
     stage_event(h, 2, fs);
     // join: the segment sums read the pass counts and the partials
     CKH(h, hipStreamWaitEvent(fs, h->ev_join, 0));
@@ -731,7 +744,7 @@ static int combine_impl(gvom_handle *h) {
     CKH(h, launch_comb_tables(g, h->st, h->slot_tables, h->zero_page, h->comb_tables, h->col_count, h->occ_bits,
                               h->height, h->inferred, st));
     stage_event(h, 6);
-    CKH(h, launch_scan_columns(h->col_count, g.X, h->chunk_tot, h->st, st));
+    CKH(h, launch_scan_columns(h->col_count, h->col_count_words, g.X, h->chunk_tot, h->chunk_tot_words, h->st, st));
     CKH(h, launch_comb_records(g, h->st, h->slot_tables, h->slot_recs, h->ncap, h->comb_tables, h->comb_recs,
                                h->ccap, h->col_count, h->chunk_tot, h->occ_bits, h->occ_list, h->height, st));
     stage_event(h, 7);
@@ -1370,7 +1383,7 @@ extern "C" int gvom_graph_begin(gvom_handle *h, int32_t max_scans) {
         h->err = "disable timing before graph capture";
         return GVOM_ESTATE;
     }
-    if (h->ray_tiled) {
+    if (h->ray_tiled && tiled_fits(h->g)) {
         const int rc_t = ensure_tile_scratch(h);
         if (rc_t) return rc_t;
     }
@@ -1532,7 +1545,8 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
     int bb = h->ray_block, br = h->ray_rpt, bx = h->ray_box, bo = h->ray_xcd, bt = h->ray_tiled;
     float best = 1e30f;
     do {
-        if (ensure_tile_scratch(h) != GVOM_OK) {
+        const bool can_tile = tiled_fits(g);
+        if (can_tile && ensure_tile_scratch(h) != GVOM_OK) {
             out = GVOM_EHIP;
             break;
         }
@@ -1561,16 +1575,19 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
                 for (int icfg = 0; icfg < 6 && out == GVOM_OK; ++icfg) {
                     // icfg: 0-3 table cast (bit 0 box, bit 1 sector order); 4-5 tiled (+ order)
                     const int itile = icfg >= 4, ibox = itile ? 0 : (icfg & 1), iord = itile ? (icfg & 1) : (icfg >> 1);
+                    if (itile && !can_tile) continue;
                     float t_min = 1e30f;
                     for (int rep = 0; rep < 3; ++rep) {
                         float t = 0.f;
                         bool ok = hipEventRecord(e0, st) == hipSuccess;
                         if (ok && !itile) ok = launch_fill_i32(scratch, 0, g.V, st) == hipSuccess;
                         ok = ok && launch_raycast(dtype, h->world, h->valid, h->sd, 1, n, g, h->st,
-                                                  itile ? h->tscratch : scratch, blocks[ib], rpts[ir], ibox, itile, 0,
+                                                  itile ? h->tscratch : scratch, itile ? h->tscratch_words : g.V,
+                                                  blocks[ib], rpts[ir], ibox, itile, 0,
                                                   iord ? h->rorder : nullptr, h->ray_spread, st) == hipSuccess;
                         if (ok && itile)
-                            ok = launch_tile_convert(g, h->st, 1, 0, h->tscratch, scratch, st) == hipSuccess;
+                            ok = launch_tile_convert(g, h->st, 1, 0, h->tscratch, h->tscratch_words, scratch, g.V,
+                                                     st) == hipSuccess;
                         ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
                              hipEventElapsedTime(&t, e0, e1) == hipSuccess;
                         if (!ok) {

@@ -366,7 +366,8 @@ __global__ __launch_bounds__(256) void k_comb_rows(Geo g, const DevState *__rest
 // Column id scan: blocks of 1024 threads scan COLSCAN consecutive column counts in place
 // (exclusive, int4 loads) and write their total; k_comb_records adds the totals of the
 // earlier chunks.
-constexpr int COLSCAN = 4096;
+constexpr int COLSCAN = gvom_plan::COLSCAN;   // 1024 threads x 4 columns
+static_assert(COLSCAN == 1024 * 4, "column scan: 1024 threads x 4 columns per block");
 
 __global__ __launch_bounds__(1024) void k_col_scan(int32_t *__restrict__ cnt, int64_t X,
                                                    int32_t *__restrict__ chunk_tot, DevState *__restrict__ st) {
@@ -945,9 +946,12 @@ hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_ta
     return hipGetLastError();
 }
 
-int64_t col_chunks(int64_t X) { return (X + COLSCAN - 1) / COLSCAN; }
+int64_t col_chunks(int64_t X) { return gvom_plan::col_chunks(X); }
 
-hipError_t launch_scan_columns(int32_t *col_count, int64_t X, int32_t *chunk_tot, DevState *st, hipStream_t s) {
+hipError_t launch_scan_columns(int32_t *col_count, int64_t count_words, int64_t X, int32_t *chunk_tot,
+                               int64_t chunk_words, DevState *st, hipStream_t s) {
+    if (X < 1 || gvom_plan::col_scan_count_words(X) > count_words || gvom_plan::col_scan_chunk_words(X) > chunk_words)
+        return hipErrorInvalidValue;
     k_col_scan<<<(unsigned)col_chunks(X), 1024, 0, s>>>(col_count, X, chunk_tot, st);
     return hipGetLastError();
 }

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "gvom_internal.h"
+#include "gvom_launch_plan.h"
 
 // ---- fills --------------------------------------------------------------------------
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s);
@@ -17,14 +18,16 @@ hipError_t launch_prepare(int dtype, const void *in, void *world, uint32_t *keys
 // conversion into the slot tables (which also zeroes the scratch).
 int64_t tile_scratch_words(const Geo &g);
 hipError_t launch_tile_convert(const Geo &g, const DevState *st, int S, int force_slot, int32_t *scratch,
-                               int32_t *tables, hipStream_t s);
+                               int64_t scratch_words, int32_t *tables, int64_t table_words, hipStream_t s);
 // Groups valid rays by azimuth sector for the XCD-aware ray schedule.
 hipError_t launch_ray_order(const uint8_t *rsec, int64_t NT, DevState *st, uint32_t *order, hipStream_t s);
-// Exact-rule ray march counting passes into the slot tables (gvom.py:1174-1231).
+// Exact-rule ray march counting passes into the slot tables (gvom.py:1174-1231), or into
+// the tiled scratch; table_words = words of `slot_tables` (refused when the launch would
+// write past them or its keys would not fit in 31 bits).
 hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, const ScanDesc *sd,
                           int S, int64_t NT, const Geo &g, const DevState *st, int32_t *slot_tables,
-                          int block, int rays_per_thread, int use_box, int tiled, int force_slot,
-                          const uint32_t *order, int spread, hipStream_t s);
+                          int64_t table_words, int block, int rays_per_thread, int use_box, int tiled,
+                          int force_slot, const uint32_t *order, int spread, hipStream_t s);
 // Stable LSD radix sort of (key, val) on the low `bits` (<= 32) key bits, 8-bit passes.
 // hist holds 2 x hwords per-tile digit counts: the first pass's filled by launch_prepare
 // (the last pass clears them for the next sort), the later passes' by a histogram kernel.
@@ -65,7 +68,8 @@ hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_ta
                               int32_t *comb_tables, int32_t *col_count, uint64_t *occ_bits, double *height,
                               double *inferred, hipStream_t s);
 int64_t col_chunks(int64_t X);
-hipError_t launch_scan_columns(int32_t *col_count, int64_t X, int32_t *chunk_tot, DevState *st, hipStream_t s);
+hipError_t launch_scan_columns(int32_t *col_count, int64_t count_words, int64_t X, int32_t *chunk_tot,
+                               int64_t chunk_words, DevState *st, hipStream_t s);
 // Cell ids + compact list, then the record merge + eigenvalues + heights of occupied columns.
 hipError_t launch_comb_records(const Geo &g, DevState *st, const int32_t *slot_tables,
                                const SlotRec *slot_recs, int64_t ncap, int32_t *comb_tables,

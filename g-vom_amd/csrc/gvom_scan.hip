@@ -92,7 +92,7 @@ __device__ __forceinline__ int ray_sector(double dx, double dy) {
 }
 
 // Radix sort tiles: RS_TILE keys per block of every pass.
-constexpr int RS_TILE = 2048;
+constexpr int RS_TILE = gvom_plan::RS_TILE;
 
 // With `hist` (the radix sort's first-pass histogram, hwords words, zero on entry: the
 // sort's last pass clears it for the next scan stage), the per-tile digit counts of the
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
     // the block's box: scan of its first ray, centred on that scan's ego voxel
     int bs = 0;
     int64_t bx0 = 0, by0 = 0, bz0 = 0;
-    uint32_t bbase = 0;
+    int64_t bbase = 0;   // the box's slot buffer, in words
     (void)bbase;
     if (USE_BOX) {
         int64_t i0 = RAY_INDEX(chunk * rpt * blockDim.x);
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
         bx0 = (int64_t)floor((double)(float)(sd[bs].ego[0] / g.xy_res) - o[0]) - BOX_XY / 2;
         by0 = (int64_t)floor((double)(float)(sd[bs].ego[1] / g.xy_res) - o[1]) - BOX_XY / 2;
         bz0 = (int64_t)floor((double)(float)(sd[bs].ego[2] / g.z_res) - o[2]) - BOX_Z / 2;
-        bbase = (uint32_t)(RAY_SLOT(bs) * g.V);
+        bbase = (int64_t)RAY_SLOT(bs) * g.V;
         for (int k = threadIdx.x; k < BOX_N; k += blockDim.x) box[k] = 0;
         __syncthreads();
     }
@@ -377,11 +377,19 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
         Ray r;
         r.alive = false;
         uint32_t base = 0;
+        int64_t toff = 0;
         if (i < NT && valid[i]) {
             const int s = scan_of(sd, S, i);
             inbox_scan = USE_BOX && s == bs;
-            // TILED: the s-th scratch table of the batch; else the scan's slot table
-            base = TILED ? (uint32_t)(s * tg.vt) : (uint32_t)(RAY_SLOT(s) * g.V);
+            // keys: scan s of the batch at s * (words per table), so adjacent lanes of
+            // different scans never merge; TILED counts into the s-th scratch table, else a
+            // run head reaches the scan's slot buffer through the 64-bit offset toff
+            if (TILED) {
+                base = (uint32_t)(s * tg.vt);
+            } else {
+                base = (uint32_t)(s * g.V);
+                toff = (int64_t)RAY_SLOT(s) * g.V - (int64_t)s * g.V;
+            }
             ray_init<P>(r, sd[s], g, world, i);
         }
         for (int it = 0; it < maxit; ++it) {
@@ -405,7 +413,7 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
             const int len = run_length(key, lane);
             if (len > 0) {
                 if (USE_BOX && lbox >= 0) atomicAdd(&box[lbox], len);
-                else __hip_atomic_fetch_add(tables + key, len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                else __hip_atomic_fetch_add(tables + toff + key, len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     }
@@ -415,8 +423,8 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
             const int32_t v = box[k];
             if (v == 0) continue;
             const int64_t lz = k % BOX_Z, lc = k / BOX_Z, ly = lc % BOX_XY, lx = lc / BOX_XY;
-            const uint32_t key = bbase + (uint32_t)lin_of(bx0 + lx, by0 + ly, bz0 + lz, g);
-            __hip_atomic_fetch_add(tables + key, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int64_t w = bbase + (int64_t)lin_of(bx0 + lx, by0 + ly, bz0 + lz, g);
+            __hip_atomic_fetch_add(tables + w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 #undef RAY_SLOT
@@ -559,7 +567,8 @@ __device__ __forceinline__ bool is_head(const uint32_t *__restrict__ k, int64_t 
 }
 
 // Segment kernels: blocks of 256 threads x 8 sorted keys, independent of the sort's tile.
-constexpr int SEG_TILE = 256 * 8;
+constexpr int SEG_TILE = gvom_plan::SEG_TILE;   // 256 threads x 8 keys
+static_assert(SEG_TILE == 256 * 8, "segment kernels: 256 threads x 8 keys per block");
 
 // Per block of SEG_TILE sorted keys: [heads, valid keys, then per scan s (in-map heads,
 // halo heads, valid keys)], W = 2 + 3 * S words.
@@ -1107,11 +1116,16 @@ hipError_t launch_ray_order(const uint8_t *rsec, int64_t NT, DevState *st, uint3
 
 hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, const ScanDesc *sd,
                           int S, int64_t NT, const Geo &g, const DevState *st, int32_t *tables,
-                          int block, int rpt, int use_box, int tiled, int force_slot, const uint32_t *order,
-                          int spread, hipStream_t s) {
+                          int64_t table_words, int block, int rpt, int use_box, int tiled, int force_slot,
+                          const uint32_t *order, int spread, hipStream_t s) {
     if (NT <= 0) return hipSuccess;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
     if (tiled && use_box) return hipErrorInvalidValue;
+    if (S < 1 || S > GVOM_MAX_BATCH || !gvom_plan::raycast_keys_fit(g.V, g.XY, g.Z, S, tiled != 0))
+        return hipErrorInvalidValue;
+    const int64_t need = tiled ? (int64_t)S * gvom_plan::tile_words(g.XY, g.Z)
+                               : gvom_plan::slot_table_footprint(g.V, g.B, force_slot);
+    if (need > table_words) return hipErrorInvalidValue;   // the launch would write past `tables`
     if (rpt < 1) rpt = 1;
     int maxit = (g.XY > g.Z ? g.XY : g.Z) + 2;
     unsigned nb = nblocks(NT, block * rpt);
@@ -1184,17 +1198,20 @@ __global__ __launch_bounds__(256) void k_tile_convert(Geo g, const DevState *__r
 }
 
 hipError_t launch_tile_convert(const Geo &g, const DevState *st, int S, int force_slot, int32_t *scratch,
-                               int32_t *tables, hipStream_t s) {
+                               int64_t scratch_words, int32_t *tables, int64_t table_words, hipStream_t s) {
     const TileGeo tg = tile_geo(g);
+    if (S < 1 || (int64_t)S * tg.vt > scratch_words ||
+        gvom_plan::slot_table_footprint(g.V, g.B, force_slot) > table_words || g.Z > 256)
+        return hipErrorInvalidValue;
     k_tile_convert<<<dim3((unsigned)(tg.tx * tg.ty), (unsigned)S), 256, 0, s>>>(g, st, force_slot, scratch, tables);
     return hipGetLastError();
 }
 
-int64_t tile_scratch_words(const Geo &g) { return tile_geo(g).vt; }
+int64_t tile_scratch_words(const Geo &g) { return gvom_plan::tile_words(g.XY, g.Z); }
 
-int64_t radix_hist_entries(int64_t n) { return 256 * (int64_t)nblocks(n, RS_TILE) + 256; }
+int64_t radix_hist_entries(int64_t n) { return gvom_plan::radix_hist_words(n); }
 
-int64_t segment_block_words(int64_t n) { return ((int64_t)nblocks(n, SEG_TILE) + 1) * (2 + 3 * GVOM_MAX_BATCH); }
+int64_t segment_block_words(int64_t n) { return gvom_plan::segment_block_words(n); }
 
 hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits,
                       uint32_t *hist, int64_t hwords, uint32_t **kout, uint32_t **vout, hipStream_t s) {
@@ -1202,6 +1219,7 @@ hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, in
     *vout = v0;
     if (n <= 0) return hipSuccess;
     if (bits > 32) return hipErrorInvalidValue;
+    if (gvom_plan::radix_hist_words(n) > hwords) return hipErrorInvalidValue;   // hist too small for n keys
     int nblk = (int)nblocks(n, RS_TILE);
     uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
     for (int shift = 0, p = 0; shift < bits; shift += 8, ++p) {
@@ -1225,7 +1243,8 @@ hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, in
 
 hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start, uint32_t *seg_key,
                            uint32_t *blk, int64_t blk_words, int S, const Geo &g, DevState *st, hipStream_t s) {
-    if (segment_block_words(n) > blk_words) return hipErrorInvalidValue;   // blk too small for n keys
+    if (S < 1 || S > GVOM_MAX_BATCH || gvom_plan::segment_block_footprint(n, S) > blk_words)
+        return hipErrorInvalidValue;   // blk too small for n keys of S scans
     int nblk = n > 0 ? (int)nblocks(n, SEG_TILE) : 0;
     if (n > 0) {
         k_seg_count<<<nblk, 256, 0, s>>>(skeys, n, S, g, blk);
