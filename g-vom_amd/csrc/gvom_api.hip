@@ -43,6 +43,67 @@ int64_t bits_for(int64_t v) {  // smallest b with 2^b >= v
 }
 }  // namespace
 
+// ---- device-static scratch of gvom_process_pointcloud_device_noalloc ----------------------
+// The scan stage's point work buffers for scans of up to NOALLOC_POINTS points, carved from
+// one file-scope device array.  Every GPU the library runs on holds its own copy until the
+// process ends; all handles and calls on that GPU share it (calls that overlap on one GPU
+// race).  Host code never uses the array's name as an address: each device's address comes
+// from hipGetSymbolAddress, once per device, at handle creation.
+constexpr int64_t NOALLOC_POINTS = 131072;
+
+struct ScratchLayout {
+    int64_t world, valid, k0, v0, k1, v1, hist, hwords, blk, blk_words, seg_key, seg_start, mom, part, rsec,
+        rorder, total;
+};
+
+constexpr int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
+constexpr ScratchLayout scratch_layout(int64_t n) {
+    ScratchLayout L{};
+    int64_t at = 0;
+    L.world = at;     at = align256(at + n * 3 * (int64_t)sizeof(double));
+    L.valid = at;     at = align256(at + n);
+    L.k0 = at;        at = align256(at + n * 4);
+    L.v0 = at;        at = align256(at + n * 4);
+    L.k1 = at;        at = align256(at + n * 4);
+    L.v1 = at;        at = align256(at + n * 4);
+    L.hwords = gvom_plan::radix_hist_words(n);
+    L.hist = at;      at = align256(at + 2 * L.hwords * 4);
+    L.blk_words = gvom_plan::segment_block_words(n);
+    L.blk = at;       at = align256(at + L.blk_words * 4);
+    L.seg_key = at;   at = align256(at + n * 4);
+    L.seg_start = at; at = align256(at + (n + 1) * 4);
+    L.mom = at;       at = align256(at + n * (int64_t)sizeof(Moment));
+    L.part = at;      at = align256(at + n * (int64_t)sizeof(Moment));
+    L.rsec = at;      at = align256(at + n);
+    L.rorder = at;    at = align256(at + n * 4);
+    L.total = at;
+    return L;
+}
+
+constexpr ScratchLayout NOALLOC_LAYOUT = scratch_layout(NOALLOC_POINTS);
+static_assert(NOALLOC_LAYOUT.total < ((int64_t)32 << 20), "allocation-free scratch stays below 32 MiB");
+
+static __device__ __align__(256) uint8_t scratch[NOALLOC_LAYOUT.total];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[MAX_DEVICES] = {};
+
+// The scratch array's address on the current device (looked up once per device).
+static hipError_t device_scratch(int device, void **out) {
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_dev[device] = p;
+    }
+    *out = g_scratch_dev[device];
+    return hipSuccess;
+}
+
 struct gvom_handle {
     gvom_params p;
     Geo g;
@@ -160,6 +221,7 @@ struct gvom_handle {
     int runtime_version = 0;
     bool nested_capture = true;
     int diff_geometry = 0;   // gvom_set_diff_geometry_only
+    uint8_t *noalloc_scratch = nullptr;   // this device's copy of `scratch`
 };
 
 #define CKH(h, expr)                                                                 \
@@ -261,6 +323,11 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
         if (e != hipSuccess) return fail_create(h, std::string(#expr) + ": " + hipGetErrorString(e), GVOM_EHIP); \
     } while (0)
     CKC(hipGetDevice(&h->device));
+    {
+        void *sp = nullptr;
+        CKC(device_scratch(h->device, &sp));   // the handle's device is current here
+        h->noalloc_scratch = (uint8_t *)sp;
+    }
     CKC(hipRuntimeGetVersion(&h->runtime_version));
     h->nested_capture = h->runtime_version / 100000 >= HIP_VERSION / 100000;   // major * 100 + minor
     CKC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -401,6 +468,9 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
     return GVOM_OK;
 }
 
+static int ensure_tile_scratch(gvom_handle *h);
+static bool tiled_fits(const Geo &g);
+
 extern "C" int gvom_reserve(gvom_handle *h, int64_t max_points, int64_t max_batch_points) {
     if (!h) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
@@ -409,7 +479,9 @@ extern "C" int gvom_reserve(gvom_handle *h, int64_t max_points, int64_t max_batc
         return GVOM_EINVAL;
     }
     if (max_batch_points < max_points) max_batch_points = max_points;
-    return ensure_capacity(h, max_points, max_batch_points);
+    const int rc = ensure_capacity(h, max_points, max_batch_points);
+    if (rc || !h->ray_tiled || !tiled_fits(h->g) || h->capturing) return rc;
+    return ensure_tile_scratch(h);   // the tiled cast's scratch, so later calls need not allocate
 }
 
 static void host_origin(const Geo &g, const double ego[3], double o[3]) {
@@ -471,12 +543,81 @@ static int ensure_tile_scratch(gvom_handle *h) {
     return GVOM_OK;
 }
 
+// The scan stage's per-point work buffers: the handle's own (grown on demand), or the
+// device-static scratch of gvom_process_pointcloud_device_noalloc.
+struct WorkBufs {
+    void *pts_in = nullptr;
+    void *world = nullptr;
+    uint8_t *valid = nullptr;
+    uint32_t *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr;
+    uint32_t *hist = nullptr;
+    int64_t hwords = 0;
+    uint32_t *blk = nullptr;
+    int64_t blk_words = 0;
+    uint32_t *seg_key = nullptr;
+    int32_t *seg_start = nullptr;
+    Moment *mom = nullptr, *part = nullptr;
+    uint8_t *rsec = nullptr;
+    uint32_t *rorder = nullptr;
+    int64_t cap = 0;          // points
+    bool clear_hist = false;  // zero the first histogram before the transform kernel counts into it
+};
+
+static WorkBufs scratch_bufs(uint8_t *base) {
+    const ScratchLayout &L = NOALLOC_LAYOUT;
+    WorkBufs w;
+    w.world = base + L.world;
+    w.valid = base + L.valid;
+    w.k0 = (uint32_t *)(base + L.k0);
+    w.v0 = (uint32_t *)(base + L.v0);
+    w.k1 = (uint32_t *)(base + L.k1);
+    w.v1 = (uint32_t *)(base + L.v1);
+    w.hist = (uint32_t *)(base + L.hist);
+    w.hwords = L.hwords;
+    w.blk = (uint32_t *)(base + L.blk);
+    w.blk_words = L.blk_words;
+    w.seg_key = (uint32_t *)(base + L.seg_key);
+    w.seg_start = (int32_t *)(base + L.seg_start);
+    w.mom = (Moment *)(base + L.mom);
+    w.part = (Moment *)(base + L.part);
+    w.rsec = base + L.rsec;
+    w.rorder = (uint32_t *)(base + L.rorder);
+    w.cap = NOALLOC_POINTS;
+    w.clear_hist = true;   // another call may have left the shared histogram dirty
+    return w;
+}
+
+static WorkBufs handle_bufs(gvom_handle *h) {
+    WorkBufs w;
+    w.pts_in = h->pts_in;
+    w.world = h->world;
+    w.valid = h->valid;
+    w.k0 = h->k0;
+    w.v0 = h->v0;
+    w.k1 = h->k1;
+    w.v1 = h->v1;
+    w.hist = h->hist;
+    w.hwords = h->hwords;
+    w.blk = h->blk;
+    w.blk_words = segment_block_words(h->ptcap);
+    w.seg_key = h->seg_key;
+    w.seg_start = h->seg_start;
+    w.mom = h->mom;
+    w.part = h->part;
+    w.rsec = h->rsec;
+    w.rorder = h->rorder;
+    w.cap = h->ptcap;
+    return w;
+}
+
 // The scan stage for S scans (gvom.py:105-220 for each).
 // cast = false (gvom_diff_forward in geometry-only mode): no ray cast; the slot tables are
 // cleared instead, so the slot holds the occupied voxels and records without pass counts.
+// scratch: the scan stage runs on these work buffers and allocates nothing (null: the
+// handle's buffers, grown as needed).
 static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets, int S, int dtype,
                         const double *ego, const void *T, int t_dtype, bool on_device, bool may_overlap,
-                        bool cast = true) {
+                        bool cast = true, const WorkBufs *scratch = nullptr) {
     const Geo &g = h->g;
     if (S < 1 || S > GVOM_MAX_BATCH || S > g.B) {
         h->err = "number of scans must be in [1, min(32, buffer_size)]";
@@ -523,11 +664,26 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     }
     // A rejected call leaves a pending gvom_diff_backward valid unless its buffers moved.
     const int64_t gen0 = h->buf_gen;
-    int rc = ensure_capacity(h, max_scan, NT);
-    if (h->buf_gen != gen0) h->diff_n = -1;
-    if (rc) return rc;
-    if (!h->slot_recs || !h->pts_in || !h->world || !h->valid || !h->k0 || !h->v0 || !h->k1 || !h->v1 ||
-        !h->hist || !h->blk || !h->seg_key || !h->seg_start || !h->mom || !h->part || !h->rsec || !h->rorder || h->ptcap < NT || h->ptcap < 1) {
+    if (scratch) {
+        const int64_t need_ncap = max_scan < g.V ? (max_scan > 0 ? max_scan : 1) : g.V;
+        if (NT > scratch->cap) {
+            h->err = "too many points for the allocation-free entry point (at most " +
+                     std::to_string(scratch->cap) + ")";
+            return GVOM_EINVAL;
+        }
+        if (need_ncap > h->ncap || !h->slot_recs) {
+            h->err = "slot records too small for this scan: reserve them first (gvom_reserve)";
+            return GVOM_ESTATE;
+        }
+    } else {
+        int rc = ensure_capacity(h, max_scan, NT);
+        if (h->buf_gen != gen0) h->diff_n = -1;
+        if (rc) return rc;
+    }
+    const WorkBufs W = scratch ? *scratch : handle_bufs(h);
+    if (!h->slot_recs || !W.world || !W.valid || !W.k0 || !W.v0 || !W.k1 || !W.v1 || !W.hist || !W.blk ||
+        !W.seg_key || !W.seg_start || !W.mom || !W.part || !W.rsec || !W.rorder || W.cap < NT || W.cap < 1 ||
+        (!on_device && !W.pts_in)) {
         h->err = "internal: work buffers not allocated";
         return GVOM_ESTATE;
     }
@@ -550,7 +706,9 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
         d.offset = offsets[s];
         d.count = offsets[s + 1] - offsets[s];
     }
-    const bool tiled = cast && h->ray_tiled != 0 && tiled_fits(g);
+    // without allocation, a handle whose tiled scratch does not exist yet casts directly
+    // (same counts)
+    const bool tiled = cast && h->ray_tiled != 0 && tiled_fits(g) && (!scratch || h->tscratch);
     if (tiled) {
         int rc_t = ensure_tile_scratch(h);
         if (h->buf_gen != gen0) h->diff_n = -1;
@@ -587,8 +745,8 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
         }
     } else {
         if (!on_device && NT > 0) {
-            CKH(h, hipMemcpyAsync(h->pts_in, xyz, (size_t)NT * 3 * esz, hipMemcpyHostToDevice, fs));
-            pts = h->pts_in;
+            CKH(h, hipMemcpyAsync(W.pts_in, xyz, (size_t)NT * 3 * esz, hipMemcpyHostToDevice, fs));
+            pts = W.pts_in;
         }
         CKH(h, hipMemcpyAsync(h->sd, sdh, sizeof(ScanDesc) * S, hipMemcpyHostToDevice, fs));
     }
@@ -599,25 +757,25 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     // counts into cleared tables
     if (!tiled) CKH(h, launch_clear_slots(h->slot_tables, h->st, S, g, fs));
     if (xcd) CKH(h, launch_fill_i32(&h->st->sec_count[0], 0, 16, fs));   // sec_count and sec_cursor
-    CKH(h, launch_prepare(dtype, pts, h->world, h->k0, h->v0, h->valid, sd, S, NT, g, h->st,
-                          xcd ? h->rsec : nullptr, h->hist, h->hwords, fs));
-    if (xcd) CKH(h, launch_ray_order(h->rsec, NT, h->st, h->rorder, fs));
+    if (W.clear_hist) CKH(h, launch_fill_i32((int32_t *)W.hist, 0, W.hwords, fs));
+    CKH(h, launch_prepare(dtype, pts, W.world, W.k0, W.v0, W.valid, sd, S, NT, g, h->st,
+                          xcd ? W.rsec : nullptr, W.hist, W.hwords, fs));
+    if (xcd) CKH(h, launch_ray_order(W.rsec, NT, h->st, W.rorder, fs));
     // fork: keys, segments and chunk partials on the side stream, concurrent with the cast
     CKH(h, hipEventRecord(h->ev_fork, fs));
     CKH(h, hipStreamWaitEvent(h->side, h->ev_fork, 0));
-    uint32_t *sk = h->k0, *sv = h->v0;
-    CKH(h, radix_sort(h->k0, h->v0, h->k1, h->v1, NT, g.VB + sbits, h->hist, h->hwords, &sk, &sv, h->side));
+    uint32_t *sk = W.k0, *sv = W.v0;
+    CKH(h, radix_sort(W.k0, W.v0, W.k1, W.v1, NT, g.VB + sbits, W.hist, W.hwords, &sk, &sv, h->side));
     h->sorted_vals = sv;
-    CKH(h, launch_segments(sk, NT, h->seg_start, h->seg_key, h->blk, segment_block_words(h->ptcap), S, g, h->st,
-                           h->side));
-    CKH(h, launch_mom_part(dtype, h->world, sk, sv, NT, sd, S, g, h->st, h->part, h->side));
+    CKH(h, launch_segments(sk, NT, W.seg_start, W.seg_key, W.blk, W.blk_words, S, g, h->st, h->side));
+    CKH(h, launch_mom_part(dtype, W.world, sk, sv, NT, sd, S, g, h->st, W.part, h->side));
     CKH(h, hipEventRecord(h->ev_join, h->side));
     stage_event(h, 1, fs);
     if (cast)
-        CKH(h, launch_raycast(dtype, h->world, h->valid, sd, S, NT, g, h->st, tiled ? h->tscratch : h->slot_tables,
+        CKH(h, launch_raycast(dtype, W.world, W.valid, sd, S, NT, g, h->st, tiled ? h->tscratch : h->slot_tables,
                               tiled ? h->tscratch_words : h->slot_table_words,
                               h->ray_block, h->ray_rpt, tiled ? 0 : h->ray_box, tiled ? 1 : 0, -1,
-                              xcd ? h->rorder : nullptr, h->ray_spread, fs));
+                              xcd ? W.rorder : nullptr, h->ray_spread, fs));
     stage_event(h, 9, fs);
     if (tiled)
         CKH(h, launch_tile_convert(g, h->st, S, -1, h->tscratch, h->tscratch_words, h->slot_tables,
@@ -626,9 +784,9 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     // join: the segment sums read the pass counts and the partials
     CKH(h, hipStreamWaitEvent(fs, h->ev_join, 0));
     stage_event(h, 3, fs);
-    CKH(h, launch_mom_seg(h->seg_start, h->seg_key, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, h->part,
-                          h->mom, fs));
-    CKH(h, launch_metrics(h->seg_key, h->mom, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, S, fs));
+    CKH(h, launch_mom_seg(W.seg_start, W.seg_key, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, W.part,
+                          W.mom, fs));
+    CKH(h, launch_metrics(W.seg_key, W.mom, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, S, fs));
     if (ovl) {
         // back on the main stream, after any combine enqueued before this call
         CKH(h, hipEventRecord(h->ev_front, fs));
@@ -665,6 +823,27 @@ extern "C" int gvom_process_pointcloud_device(gvom_handle *h, const void *xyz, i
     std::lock_guard<std::mutex> lk(h->mu);
     int64_t off[2] = {0, n};
     return process_impl(h, xyz, off, 1, dtype, ego, T, t_dtype, true, true);
+}
+
+extern "C" int gvom_process_pointcloud_device_noalloc(gvom_handle *h, const void *xyz, int64_t n, int32_t dtype,
+                                                      const double ego[3], const void *T, int32_t t_dtype) {
+    if (!h || !ego || n < 0) return GVOM_EINVAL;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (h->timing) {
+        h->err = "stage timing creates events on demand: disable it for the allocation-free entry point";
+        return GVOM_ESTATE;
+    }
+    if (!h->noalloc_scratch) {
+        h->err = "internal: no device scratch";
+        return GVOM_ESTATE;
+    }
+    if (n > NOALLOC_POINTS) {
+        h->err = "too many points for the allocation-free entry point (at most " + std::to_string(NOALLOC_POINTS) + ")";
+        return GVOM_EINVAL;
+    }
+    const WorkBufs w = scratch_bufs(h->noalloc_scratch);
+    int64_t off[2] = {0, n};
+    return process_impl(h, xyz, off, 1, dtype, ego, T, t_dtype, true, true, true, &w);
 }
 
 extern "C" int gvom_process_batch(gvom_handle *h, const void *xyz, const int64_t *offsets, int32_t n_scans,

@@ -17,19 +17,18 @@ constexpr int MAX_BATCH = 32;     // scans per process batch
 constexpr int MAX_BUFFER = 30;    // ring slots
 constexpr int64_t KEY_END = (int64_t)1 << 31;   // ray keys: the sign bit marks a dead lane
 
-inline int64_t blocks(int64_t n, int64_t per) {
-    const int64_t b = (n + per - 1) / per;
-    return b > 0 ? b : 1;
+constexpr int64_t blocks(int64_t n, int64_t per) {
+    return (n + per - 1) / per > 0 ? (n + per - 1) / per : 1;
 }
 
 // ---- radix sort: one 256-digit histogram per tile (two sets: first pass, later passes)
-inline int64_t radix_hist_words(int64_t n) { return 256 * blocks(n, RS_TILE) + 256; }
+constexpr int64_t radix_hist_words(int64_t n) { return 256 * blocks(n, RS_TILE) + 256; }
 
 // ---- segment kernels: per block of SEG_TILE keys, W = 2 + 3 S counters (heads, valid keys,
 // then per scan of the batch in-map heads, halo heads, valid keys), plus one row of totals;
 // the C-ABI sizes the buffer for the largest batch
-inline int64_t segment_block_footprint(int64_t n, int S) { return (blocks(n, SEG_TILE) + 1) * (2 + 3 * (int64_t)S); }
-inline int64_t segment_block_words(int64_t n) { return segment_block_footprint(n, MAX_BATCH); }
+constexpr int64_t segment_block_footprint(int64_t n, int S) { return (blocks(n, SEG_TILE) + 1) * (2 + 3 * (int64_t)S); }
+constexpr int64_t segment_block_words(int64_t n) { return segment_block_footprint(n, MAX_BATCH); }
 
 // ---- column scan over X per-column counts: block b covers columns [b COLSCAN, (b + 1) COLSCAN)
 // with 16-byte loads and stores where all four columns exist and single words at the tail, so

@@ -30,6 +30,7 @@ SIGNATURES = {
     "gvom_reserve": (ctypes.c_int, [_P, _I64, _I64]),
     "gvom_process_pointcloud": (ctypes.c_int, [_P, _P, _I64, _I32, _P, _P, _I32]),
     "gvom_process_pointcloud_device": (ctypes.c_int, [_P, _P, _I64, _I32, _P, _P, _I32]),
+    "gvom_process_pointcloud_device_noalloc": (ctypes.c_int, [_P, _P, _I64, _I32, _P, _P, _I32]),
     "gvom_process_batch": (ctypes.c_int, [_P, _P, _P, _I32, _I32, _P, _P, _I32, _I32]),
     "gvom_combine_maps": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "gvom_combine_maps_device": (ctypes.c_int, [_P, _P]),

@@ -72,9 +72,9 @@ void gvom_destroy(gvom_handle *h);
 const char *gvom_last_error(const gvom_handle *h);
 
 /* Pre-sizes the per-scan work buffers and slot records for scans of up to
- * max_points points and batches of up to max_batch_points points.  Optional: the
- * process calls grow the buffers on demand (reference: per-call cuda.device_array,
- * gvom.py:124-174). */
+ * max_points points and batches of up to max_batch_points points, and creates the tiled
+ * ray cast's scratch.  Optional: the process calls grow the buffers on demand (reference:
+ * per-call cuda.device_array, gvom.py:124-174). */
 int gvom_reserve(gvom_handle *h, int64_t max_points, int64_t max_batch_points);
 
 /* Replaces Gvom.Process_pointcloud(pointcloud, ego_position, transform)
@@ -91,6 +91,20 @@ int gvom_process_pointcloud(gvom_handle *h, const void *xyz, int64_t n, int32_t 
 int gvom_process_pointcloud_device(gvom_handle *h, const void *xyz_device, int64_t n,
                                    int32_t dtype, const double ego[3], const void *T,
                                    int32_t t_dtype);
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid hipMalloc): as
+ * gvom_process_pointcloud_device, with the same results, but the scan stage's point work
+ * buffers come from a device-static scratch array (about 28 MiB, held by every GPU the
+ * library runs on until the process ends) and the call makes no allocation.  Limits:
+ * n <= 131072 points (larger n is rejected with GVOM_EINVAL); the slot records must hold
+ * the scan's cells already (gvom_reserve(h, n, 0) once, or an earlier process call of at
+ * least n points, else GVOM_ESTATE); stage timing must be off.  The scratch is shared by
+ * every handle and call on one GPU and keeps the last call's contents: calls that overlap
+ * on one GPU (several handles on several host threads or streams) race. */
+int gvom_process_pointcloud_device_noalloc(gvom_handle *h, const void *xyz_device, int64_t n,
+                                           int32_t dtype, const double ego[3], const void *T,
+                                           int32_t t_dtype);
 
 /* Several scans in one launch sequence, with the same result as n_scans
  * consecutive Process_pointcloud calls (gvom.py:105-220).  xyz: flat array of
