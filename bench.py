@@ -42,6 +42,42 @@ import numpy as np  # noqa: E402
 
 METRIC = "scan->map frames/s at 131k pts, 256x256x64 voxels; fwd+bwd ms; % HBM BW"
 HBM_PEAK_GBS = 8000.0
+MIN_FRAMES = 200      # every timed leg covers at least this many frames (scans, loss calls) ...
+MIN_SECONDS = 0.2     # ... and at least this long, whatever --steps asks for
+
+
+class DeviceTimer:
+    """HIP events on one stream around a timed region (device time, not host wall time)."""
+
+    def __init__(self, stream):
+        import ctypes
+        import gvom_native as nat
+        self.ct = ctypes
+        self.hip = nat._hip_runtime()
+        self.stream = ctypes.c_void_p(stream.value if hasattr(stream, "value") else stream)
+        self.e0, self.e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        self.hip.hipEventCreate(ctypes.byref(self.e0))
+        self.hip.hipEventCreate(ctypes.byref(self.e1))
+
+    def start(self):
+        self.hip.hipEventRecord(self.e0, self.stream)
+
+    def stop(self):
+        """Seconds between start() and now on the stream (waits for the stream)."""
+        self.hip.hipEventRecord(self.e1, self.stream)
+        self.hip.hipEventSynchronize(self.e1)
+        ms = self.ct.c_float()
+        self.hip.hipEventElapsedTime(self.ct.byref(ms), self.e0, self.e1)
+        return ms.value * 1e-3
+
+
+def calls_for(t_one, units_per_call, requested_units):
+    """Calls that cover max(requested, MIN_FRAMES) units and MIN_SECONDS, given one call's time."""
+    need = max(requested_units, MIN_FRAMES)
+    calls = -(-need // units_per_call)
+    if t_one > 0:
+        calls = max(calls, int(np.ceil(MIN_SECONDS / t_one)))
+    return int(calls)
 STAGES = ["clear+prepare", "raycast", "sort+segments_beyond_raycast", "moments+metrics", "combine_tables",
           "records+columns", "maps2d", "raycast_kernel", "tile_convert"]
 
@@ -92,8 +128,9 @@ def cpu_baseline(wl, seconds):
                        f"through the single-threaded C oracle, {el:.1f} s")
 
 
-def graph_rate(g, dev, scans, reps, warm=2):
-    """Frames/s of the 8-frame sequence replayed from one captured graph."""
+def graph_rate(g, dev, scans, frames, warm=2):
+    """(frames/s, frames timed) of the 8-frame sequence replayed from one captured graph,
+    over at least `frames` frames and MIN_SECONDS, timed with events on the handle's stream."""
     g.sync()
     g.graph_begin(len(scans))
     for k, s in enumerate(scans):
@@ -102,17 +139,41 @@ def graph_rate(g, dev, scans, reps, warm=2):
     gr = g.graph_end()
     egos = np.stack([s.ego for s in scans])
     Ts = np.stack([s.transform for s in scans])
+    tm = DeviceTimer(g.stream)
     for _ in range(warm):
         gr.launch(egos, Ts)
     g.sync()
-    t0 = time.perf_counter()
+    tm.start()
+    gr.launch(egos, Ts)
+    t1 = tm.stop()
+    reps = calls_for(t1, len(scans), frames)
+    g.sync()
+    tm.start()
     for _ in range(reps):
         gr.launch(egos, Ts)
-    g.sync()
-    return reps * len(scans) / (time.perf_counter() - t0)
+    t = tm.stop()
+    return reps * len(scans) / t, reps * len(scans)
 
 
-def batched_leg(reps, overlap):
+def slot_cells(g, B):
+    """Cells of every filled ring slot and of the combined map (exported state)."""
+    cells = []
+    for k in range(B):
+        try:
+            cells.append(int(len(g.slot_state(k)["hit"])))
+        except Exception:   # an empty slot
+            cells.append(0)
+    return cells, int(len(g.combined_state()["hit"]))
+
+
+def frame_bytes(N, V, X, B_used, L, C_s, sum_C_s, C, C_prev):
+    """Algorithmic HBM bytes of one scan + one combine (dense-table model, 4-byte words):
+    S1 12N + 4V + 8 C_s, S2 12N + 44 C_s, S3 4V (B_used + L) + 4V, S4 52 (sum C_s + L C_prev)
+    + 52 C, S5 36 C, S6 4V + 12 C + 20 X."""
+    return (24 * N + 4 * V + 52 * C_s) + (4 * V * (B_used + L + 2) + 52 * (sum_C_s + L * C_prev) + 100 * C + 20 * X)
+
+
+def batched_leg(requested, overlap):
     import gvom
     import gvom_native as nat
     import gvom_synth
@@ -134,14 +195,24 @@ def batched_leg(reps, overlap):
                                np.stack([s.transform for s in wl.steps[k]]))
         g.combine_maps_device()
     gr = g.graph_end()
+    tm = DeviceTimer(g.stream)
     for _ in range(2):
         gr.launch(egos, Ts)
     g.sync()
-    t0 = time.perf_counter()
+    tm.start()
+    gr.launch(egos, Ts)
+    reps = calls_for(tm.stop(), len(scans), requested)
+    g.sync()
+    tm.start()
     for _ in range(reps):
         gr.launch(egos, Ts)
-    g.sync()
-    t_b = (time.perf_counter() - t0) / reps
+    t_b = tm.stop() / reps
+    # steady-state counts of one launch (the replays repeat one sequence): 24 scans whose
+    # cells are ~3x the ring's, 8 combines over the full ring with history
+    ring, C = slot_cells(g, 8)
+    V, X = 256 * 256 * 64, 256 * 256
+    launch_bytes = 8 * (frame_bytes(0, V, X, 8, 1, 0, sum(ring), C, C) - 4 * V) + sum(
+        24 * s.points.shape[0] + 4 * V for s in scans) + 3 * 52 * sum(ring)
     p = gvom.Gvom(*wl.map_args)
     p.reserve(max(s.points.shape[0] for s in scans))
     p.set_overlap(overlap)
@@ -149,13 +220,13 @@ def batched_leg(reps, overlap):
         p.process_pointcloud_device(sdev[k], s.points.shape[0], s.ego, s.transform)
         p.combine_maps_device()
     p.sync()
-    t0 = time.perf_counter()
+    tp = DeviceTimer(p.stream)
+    tp.start()
     for _ in range(reps):
         for k, s in enumerate(scans):
             p.process_pointcloud_device(sdev[k], s.points.shape[0], s.ego, s.transform)
             p.combine_maps_device()
-    p.sync()
-    t_p = (time.perf_counter() - t0) / reps
+    t_p = tp.stop() / reps
     # the reference's calling pattern: host arrays in, host maps out, one call per scan
     hcall = gvom.Gvom(*wl.map_args)
     for s in scans[:8]:
@@ -167,24 +238,58 @@ def batched_leg(reps, overlap):
             hcall.Process_pointcloud(s.points, s.ego, s.transform)
             hcall.combine_maps()
     t_h = (time.perf_counter() - t0) / reps
+    achieved = launch_bytes / t_b / 1e9
     return {"workload": "3 sensors (128/64/16 x 1024, 10-40% dropout) x 8 steps, 256x256x64, buffer 8; "
                         "one hipGraph launch = 8 x (3-scan batch + combine_maps)",
-            "scans_per_launch": len(scans), "points_per_launch": int(n_pts),
+            "scans_per_launch": len(scans), "points_per_launch": int(n_pts), "launches_timed": reps,
+            "scans_timed": reps * len(scans),
             "scans_per_s": round(len(scans) / t_b, 1), "ms_per_launch": round(1e3 * t_b, 3),
+            "points_per_s": round(n_pts / t_b, 1),
             "plain_frames_per_s": round(len(scans) / t_p, 1),
             "speedup_vs_plain": round(t_p / t_b, 2),
             "host_api_frames_per_s": round(len(scans) / t_h, 1),
-            "speedup_vs_host_api": round(t_h / t_b, 2)}
+            "speedup_vs_host_api": round(t_h / t_b, 2),
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes": int(launch_bytes),
+                         "counts": {"ring_cells": ring, "combined_cells": C}}}
 
 
-def host_api_leg(wl, scans, reps):
+def plain_config_leg(cfg, frames, requested):
+    """Plain frames of another configuration (2: 64 x 1024 scans into 256 x 256 x 64, buffer
+    4; 3: 128 x 2048 scans into 512 x 512 x 64, buffer 8), replayed from one captured graph
+    with scan/combine overlap, inputs resident in HBM: frames/s, points/s."""
+    import gvom
+    import gvom_native as nat
+    import gvom_synth
+    wl = gvom_synth.workload(cfg, frames=frames)
+    scans = [st[0] for st in wl.steps]
+    g = gvom.Gvom(*wl.map_args)
+    g.reserve(max(s.points.shape[0] for s in scans))
+    dev = [nat.DeviceArray(s.points) for s in scans]
+    for k, s in enumerate(scans):
+        g.process_pointcloud_device(dev[k], s.points.shape[0], s.ego, s.transform)
+        g.combine_maps_device()
+    g.set_overlap(True)
+    fps, timed = graph_rate(g, dev, scans, requested)
+    npts = float(np.mean([s.points.shape[0] for s in scans]))
+    return {"workload": wl.name, "frames_per_s": round(fps, 1), "points_per_s": round(fps * npts, 1),
+            "frames_timed": timed, "mean_points_per_scan": int(npts)}
+
+
+def host_api_leg(wl, scans, requested):
     """Full host-API frames/s: host float32 points in (H2D of 12N bytes), the four 2D maps
-    out (D2H), as the reference's callers use Process_pointcloud + combine_maps."""
+    out (D2H), as the reference's callers use Process_pointcloud + combine_maps (host wall
+    time: the host's copies are part of this rate)."""
     import gvom
     g = gvom.Gvom(*wl.map_args)
     for k, s in enumerate(scans):
         g.Process_pointcloud(s.points, s.ego, s.transform)
         g.combine_maps()
+    t0 = time.perf_counter()
+    for s in scans:
+        g.Process_pointcloud(s.points, s.ego, s.transform)
+        g.combine_maps()
+    reps = calls_for((time.perf_counter() - t0) / len(scans), 1, requested)
     t0 = time.perf_counter()
     for k in range(reps):
         s = scans[k % len(scans)]
@@ -221,7 +326,11 @@ def copy_calibration(mib, reps=5):
     print(json.dumps({"copy_calibration_bytes": n, "copies": reps}), flush=True)
 
 
-def diff_leg(reps, refine=True):
+def diff_leg(requested, refine=True):
+    """Differentiable loss on one 65,536-ray scan: forward and forward + backward through the
+    stream-ordered entry points (no host synchronisation inside the timed region; device
+    events on the loss handle's stream), after the handle has captured its forward graph."""
+    import ctypes
     import gvom_native as nat
     import gvom_synth
     from gvom_torch import MapLoss
@@ -230,60 +339,95 @@ def diff_leg(reps, refine=True):
     rng = np.random.default_rng(105)
     T = gvom_synth.perturb_transform(s.transform, rng).astype(np.float64)
     pts = s.points.astype(np.float64)
+    n = pts.shape[0]
     dev = nat.DeviceArray(pts)
     gdev = nat.DeviceArray(np.zeros_like(pts))
+    out = nat.DeviceArray(np.zeros(8))       # forward results (3) and the upstream gradient dL = 1
+    one = nat.DeviceArray(np.ones(1))
+    gxi = nat.DeviceArray(np.zeros(6))
     loss = MapLoss(0.4, 0.2, 256, 64)
-    loss.forward(dev.ptr.value, pts.shape[0], T, s.ego)
-    loss.backward(1.0, gdev.ptr.value)
+    stream = loss.g.stream
+    sv = stream.value or 0
+    tm = DeviceTimer(stream)
+
+    def fwd():
+        loss.forward_async(dev.ptr.value, n, T, s.ego, out.ptr.value, sv)
+
+    def fwd_bwd():
+        fwd()
+        loss.backward_async(one.ptr.value, None, gdev.ptr.value, gxi.ptr.value, sv)
+
+    for _ in range(4):          # the second call captures the forward graph; later ones replay it
+        fwd_bwd()
+    loss.g.sync()
+    tm.start()
+    fwd()
+    reps = calls_for(tm.stop(), 1, requested)
+    tm.start()
+    for _ in range(reps):
+        fwd()
+    t_f = tm.stop() / reps
+    tm.start()
+    for _ in range(reps):
+        fwd_bwd()
+    t_fb = tm.stop() / reps
+    # the synchronous API (host round trip per call), wall time
+    loss.forward(dev.ptr.value, n, T, s.ego)
     t0 = time.perf_counter()
     for _ in range(reps):
-        loss.forward(dev.ptr.value, pts.shape[0], T, s.ego)
-    t_f = (time.perf_counter() - t0) / reps
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        loss.forward(dev.ptr.value, pts.shape[0], T, s.ego)
-        loss.backward(1.0, gdev.ptr.value)
-    t_fb = (time.perf_counter() - t0) / reps
+        loss.forward(dev.ptr.value, n, T, s.ego)
+        loss.backward(1.0, None, gdev.ptr.value)
+    t_sync = (time.perf_counter() - t0) / reps
+    # algorithmic bytes: forward = geometry-only scan stage (24N + 4V + 52 C_s) + the loss
+    # kernels (eigen 36 C_s, columns 4V + 12 X); backward 28N + 76 C_s + 8X + 24
+    V, X = 256 * 256 * 64, 256 * 256
+    C_s = int(len(loss.g.slot_state(0)["hit"]))
+    b_f = 24 * n + 4 * V + 52 * C_s + 36 * C_s + 4 * V + 12 * X
+    b_b = 28 * n + 76 * C_s + 8 * X + 24
+    ach = (b_f + b_b) / t_fb / 1e9
     timing = {"workload": "one 64x1024 scan (65,536 rays, 5% dropout), 256x256x64; L = sum lambda_min + "
-                          "0.1 * column-height variance",
-              "points": int(pts.shape[0]), "fwd_ms": round(1e3 * t_f, 3), "fwd_bwd_ms": round(1e3 * t_fb, 3),
-              "bwd_over_fwd": round(t_fb / t_f, 3)}
+                          "0.1 * column-height variance; stream-ordered calls, device events",
+              "points": int(n), "cells": C_s, "calls_timed": reps,
+              "fwd_ms": round(1e3 * t_f, 4), "fwd_bwd_ms": round(1e3 * t_fb, 4),
+              "bwd_over_fwd": round(t_fb / t_f, 3),
+              "sync_api_fwd_bwd_ms": round(1e3 * t_sync, 4),
+              "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(ach / HBM_PEAK_GBS, 4), "algorithmic_bytes": int(b_f + b_b)}}
     if not refine:
         return timing
 
-    # 20 refinement steps of a perturbed sensor extrinsic shared by 4 frames at known vehicle
-    # poses (1 m apart, ~1 deg yaw drift).  The loss is invariant to a common translation and
-    # to a rotation about z of all points, so one scan's own transform gets no gradient along
-    # those directions; across the 4 frames the extrinsic's rotation shows as misalignment
-    # between frames, its translation only through their ~1 deg relative yaw.  The rotation is
-    # refined and the translation held.
-    from gvom_torch import refine_extrinsic
-    wl4 = gvom_synth.workload(2, frames=4)
-    frames = [st[0] for st in wl4.steps]
-    E_true = np.eye(4)
+    # 20 Levenberg-Marquardt steps refining a sensor extrinsic perturbed by N(0, 0.5 deg) /
+    # N(0, 5 cm) and shared by 4 frames at known vehicle poses on a 90-degree arc of 8 m radius.
+    # The loss is invariant to a common translation of all points, and the poses differ only
+    # by yaw, so the extrinsic's vertical offset moves every frame alike and is held.
+    from gvom_torch import extrinsic_evaluator, lm_refine
+    pts4, poses4, _ = gvom_synth.refine_frames()
     E0 = gvom_synth.perturb_transform(np.eye(4, dtype=np.float32), np.random.default_rng(205)).astype(np.float64)
     loss4 = MapLoss(0.4, 0.2, 256, 64)
-    E1, Ls = refine_extrinsic([f.points.astype(np.float64) for f in frames],
-                              [f.transform.astype(np.float64) for f in frames], E0, loss4, steps=20,
-                              lr=(0.0, 0.0, 0.0, 0.002, 0.002, 0.002))
+    ev = extrinsic_evaluator(pts4, poses4, loss4, E_ref=E0)
+    t0 = time.perf_counter()
+    E1, Ls, _ = lm_refine(ev, E0, steps=20)
+    t_ref = time.perf_counter() - t0
 
     def err_e(Ea):
-        dR = Ea[:3, :3] @ E_true[:3, :3].T
-        ang = np.degrees(np.arccos(np.clip((np.trace(dR) - 1) / 2, -1, 1)))
-        return float(ang), float(np.linalg.norm(Ea[:3, 3] - E_true[:3, 3]))
+        ang = np.degrees(np.arccos(np.clip((np.trace(Ea[:3, :3]) - 1) / 2, -1, 1)))
+        return float(ang), float(np.linalg.norm(Ea[:3, 3])), [round(float(v), 4) for v in Ea[:3, 3]]
     return {**timing,
             "refine_extrinsic_4_frames_20_steps": {
-                "workload": "4 frames of 64x1024 scans at known vehicle poses, shared perturbed extrinsic; "
-                            "rotation refined (Adam, 0.002 rad), translation held",
-                "loss": [round(Ls[0], 4), round(Ls[-1], 4)],
+                "workload": "4 frames of 64x1024 scans at known vehicle poses on a 90 deg, 8 m arc, shared "
+                            "extrinsic perturbed by N(0, 0.5 deg) / N(0, 5 cm); 20 Levenberg-Marquardt steps "
+                            "(Hessian from central differences of the HIP gradient), vertical offset held",
+                "loss": [round(Ls[0], 4), round(Ls[-1], 4)], "loss_at_true_extrinsic": round(ev(np.eye(4), False)[0], 4),
                 "rot_err_deg": [round(err_e(E0)[0], 4), round(err_e(E1)[0], 4)],
-                "trans_err_m": [round(err_e(E0)[1], 4), round(err_e(E1)[1], 4)]}}
+                "trans_err_m": [round(err_e(E0)[1], 4), round(err_e(E1)[1], 4)],
+                "trans_err_xyz_m": [err_e(E0)[2], err_e(E1)[2]], "seconds": round(t_ref, 2)}}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=400,
+                    help="frames to time at least (every leg also covers >= 200 frames and >= 0.2 s)")
     ap.add_argument("--warmup", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -350,7 +494,7 @@ def main():
         rates = {}
         for b, r, f in cands:
             g.set_raycast_config(b, r, bool(f & 1), bool(f & 2), bool(f & 4), bool(f & 8))
-            rates[f"{b},{r},{f}"] = round(graph_rate(g, dev, scans, 25), 1)
+            rates[f"{b},{r},{f}"] = round(graph_rate(g, dev, scans, MIN_FRAMES)[0], 1)
         b, r, f = max(cands, key=lambda c: rates[f"{c[0]},{c[1]},{c[2]}"])
         g.set_raycast_config(b, r, bool(f & 1), bool(f & 2), bool(f & 4), bool(f & 8))
         g.set_overlap(False)
@@ -359,16 +503,20 @@ def main():
         for k in range(len(scans)):
             frame(k)
         g.sync()
+    tm = DeviceTimer(g.stream)
+    tm.start()
+    for k in range(len(scans)):
+        frame(k)
+    n_eager = calls_for(tm.stop() / len(scans), 1, args.steps)
     g.set_timing(True)
-    t0 = time.perf_counter()
-    for k in range(args.steps):
+    tm.start()
+    for k in range(n_eager):
         frame(args.warmup + k)
-    g.sync()
-    el = time.perf_counter() - t0
+    el = tm.stop()
     stages = g.stage_times()
     g.set_timing(False)
 
-    eager_fps = args.steps / el
+    eager_fps = n_eager / el
     # the stage times above come from serial frames; from here on the scan stage of frame
     # k + 1 may run while the combine of frame k runs
     overlap = not args.no_overlap
@@ -376,14 +524,14 @@ def main():
     eager_ovl_fps = None
     if overlap:
         g.sync()
-        t0 = time.perf_counter()
-        for k in range(args.steps):
+        tm.start()
+        for k in range(n_eager):
             frame(k)
-        g.sync()
-        eager_ovl_fps = args.steps / (time.perf_counter() - t0)
+        eager_ovl_fps = n_eager / tm.stop()
     fps = eager_ovl_fps or eager_fps
+    timed = n_eager
     if not args.quick:
-        fps = graph_rate(g, dev, scans, max(1, args.steps // len(scans)))
+        fps, timed = graph_rate(g, dev, scans, args.steps)
     ms = 1e3 / fps
     V = 256 * 256 * 64
     X = 256 * 256
@@ -409,7 +557,7 @@ def main():
         "value": round(fps, 2),
         "unit": "frames/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": timed,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
         "higher_is_better": True,
@@ -423,7 +571,10 @@ def main():
                    "points_per_scan": N, "grid": [256, 256, 64], "buffer": 8, "frames_cycled": len(scans),
                    "overlap": overlap},
         "points_per_s": round(fps * N, 1),
+        "timing": "HIP events on the handle's stream; every leg covers >= %d frames and >= %.1f s" % (
+            MIN_FRAMES, MIN_SECONDS),
         "plain_eager": {"frames_per_s": round(eager_fps, 2), "ms_per_frame": round(1e3 / eager_fps, 4),
+                        "frames_timed": n_eager,
                         "overlapped_frames_per_s": round(eager_ovl_fps, 2) if eager_ovl_fps else None},
         "stages_ms": {k: round(v, 5) for k, v in zip(STAGES, stages)},
         "scan": {"cells": ss["cells"], "ray_steps": ss["steps"], "combined_cells": C},
@@ -435,10 +586,11 @@ def main():
         "gen_seconds": round(gen_s, 1),
     }
     if not args.quick:
-        out["host_api"] = host_api_leg(wl, scans, max(8, args.steps // 4))
+        out["host_api"] = host_api_leg(wl, scans, args.steps)
         out["atomics"] = atomics_leg(g, ss["steps"], stages[7])
-        out["batched"] = batched_leg(max(1, args.steps // 24), not args.no_overlap)
-        out["fwd_bwd"] = diff_leg(max(5, args.steps // 8))
+        out["batched"] = batched_leg(args.steps, not args.no_overlap)
+        out["cfg3"] = plain_config_leg(3, 8, args.steps)
+        out["fwd_bwd"] = diff_leg(args.steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     else:

@@ -129,6 +129,15 @@ struct gvom_handle {
     double *height = nullptr, *inferred = nullptr, *slope_x = nullptr, *slope_y = nullptr, *rough = nullptr,
            *delta = nullptr;
     int32_t *pos = nullptr, *neg = nullptr, *vis = nullptr, *col_count = nullptr;
+    uint8_t *maps_block = nullptr;   // pos, neg, vis (int32) and rough (f64): 20 X bytes
+    // Pinned host staging of the host-array entry points (hipHostMalloc, grown on demand):
+    // two point buffers used in turn (ev_pin[k] marks the end of buffer k's last transfer)
+    // and one for the four maps.
+    void *pin_pts[2] = {nullptr, nullptr};
+    size_t pin_pts_bytes[2] = {0, 0};
+    hipEvent_t ev_pin[2] = {nullptr, nullptr};
+    int pin_next = 0;
+    void *pin_maps = nullptr;
     uint64_t *occ_bits = nullptr;   // combined map: per column, bit z of word z / 64 = occupied
     int32_t *zero_page = nullptr;   // zeros the combine reads in place of off-map words
     uint2 *occ_list = nullptr;
@@ -235,7 +244,7 @@ struct gvom_handle {
 
 static void free_all(gvom_handle *h) {
     void *ptrs[] = {h->st, h->slot_tables, h->slot_recs, h->comb_tables, h->comb_recs, h->height, h->inferred,
-                    h->slope_x, h->slope_y, h->rough, h->delta, h->pos, h->neg, h->vis, h->occ_list,
+                    h->slope_x, h->slope_y, h->maps_block, h->delta, h->occ_list,
                     h->col_count, h->chunk_tot, h->export_buf, h->pts_in, h->world, h->valid, h->k0, h->v0, h->k1, h->v1,
                     h->hist, h->blk, h->seg_key, h->seg_start, h->mom, h->part, h->sd, h->diff_rec,
                     h->diff_colh, h->diff_part, h->diff_sums, h->diff_colpt, h->diff_pmark, h->graph_sd, h->rsec, h->rorder, h->grids, h->raw, h->tscratch,
@@ -250,6 +259,11 @@ static void free_all(gvom_handle *h) {
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_join) (void)hipEventDestroy(h->ev_join);
     if (h->ev_in) (void)hipEventDestroy(h->ev_in);
+    for (int k = 0; k < 2; ++k) {
+        if (h->ev_pin[k]) (void)hipEventDestroy(h->ev_pin[k]);
+        if (h->pin_pts[k]) (void)hipHostFree(h->pin_pts[k]);
+    }
+    if (h->pin_maps) (void)hipHostFree(h->pin_maps);
     if (h->ev_out) (void)hipEventDestroy(h->ev_out);
     if (h->ev_front) (void)hipEventDestroy(h->ev_front);
     if (h->ev_back) (void)hipEventDestroy(h->ev_back);
@@ -335,11 +349,15 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     CKC(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
     CKC(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
     CKC(hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming));
+    CKC(hipEventCreateWithFlags(&h->ev_pin[0], hipEventDisableTiming));
+    CKC(hipEventCreateWithFlags(&h->ev_pin[1], hipEventDisableTiming));
     CKC(hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming));
     CKC(hipStreamCreateWithFlags(&h->front, hipStreamNonBlocking));
     CKC(hipEventCreateWithFlags(&h->ev_front, hipEventDisableTiming));
     CKC(hipEventCreateWithFlags(&h->ev_back, hipEventDisableTiming));
     CKC(hipEventRecord(h->ev_back, h->stream));
+    CKC(hipEventRecord(h->ev_pin[0], h->stream));
+    CKC(hipEventRecord(h->ev_pin[1], h->stream));
     CKC(dalloc(&h->st, 1));
     CKC(hipMemset(h->st, 0, sizeof(DevState)));
     h->slot_table_words = gvom_plan::alloc_slot_table_words(V, g.B);   // two buffers per ring slot
@@ -352,12 +370,15 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     CKC(dalloc(&h->inferred, X));
     CKC(dalloc(&h->slope_x, X));
     CKC(dalloc(&h->slope_y, X));
-    CKC(dalloc(&h->rough, X));
+    // the four output maps in one block [pos | neg | vis | rough], so a host combine copies
+    // them with one transfer
+    CKC(dalloc(&h->maps_block, 20 * X));
+    h->pos = (int32_t *)h->maps_block;
+    h->neg = h->pos + X;
+    h->vis = h->neg + X;
+    h->rough = (double *)(h->vis + X);
     CKC(dalloc(&h->delta, X));
-    CKC(dalloc(&h->pos, X));
     CKC(dalloc(&h->grids, 5 * X));
-    CKC(dalloc(&h->neg, X));
-    CKC(dalloc(&h->vis, X));
     CKC(dalloc(&h->occ_list, V));
     h->col_count_words = gvom_plan::alloc_col_count_words(X);
     CKC(dalloc(&h->col_count, h->col_count_words));
@@ -745,7 +766,22 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
         }
     } else {
         if (!on_device && NT > 0) {
-            CKH(h, hipMemcpyAsync(W.pts_in, xyz, (size_t)NT * 3 * esz, hipMemcpyHostToDevice, fs));
+            // host points: copied into a pinned staging buffer, then one asynchronous transfer;
+            // the two buffers alternate, so this call waits only for the transfer two calls back
+            const size_t bytes = (size_t)NT * 3 * esz;
+            const int k = h->pin_next;
+            CKH(h, hipEventSynchronize(h->ev_pin[k]));
+            if (h->pin_pts_bytes[k] < bytes) {
+                if (h->pin_pts[k]) CKH(h, hipHostFree(h->pin_pts[k]));
+                h->pin_pts[k] = nullptr;
+                h->pin_pts_bytes[k] = 0;
+                CKH(h, hipHostMalloc(&h->pin_pts[k], bytes, hipHostMallocDefault));
+                h->pin_pts_bytes[k] = bytes;
+            }
+            std::memcpy(h->pin_pts[k], xyz, bytes);
+            CKH(h, hipMemcpyAsync(W.pts_in, h->pin_pts[k], bytes, hipMemcpyHostToDevice, fs));
+            CKH(h, hipEventRecord(h->ev_pin[k], fs));
+            h->pin_next = 1 - k;
             pts = W.pts_in;
         }
         CKH(h, hipMemcpyAsync(h->sd, sdh, sizeof(ScanDesc) * S, hipMemcpyHostToDevice, fs));
@@ -954,11 +990,15 @@ extern "C" int gvom_combine_maps(gvom_handle *h, double origin_m[3], int32_t *po
     if (rc) return rc;
     const size_t X = (size_t)h->g.X;
     hipStream_t st = h->stream;
-    if (pos) CKH(h, hipMemcpyAsync(pos, h->pos, X * 4, hipMemcpyDeviceToHost, st));
-    if (neg) CKH(h, hipMemcpyAsync(neg, h->neg, X * 4, hipMemcpyDeviceToHost, st));
-    if (rough) CKH(h, hipMemcpyAsync(rough, h->rough, X * 8, hipMemcpyDeviceToHost, st));
-    if (vis) CKH(h, hipMemcpyAsync(vis, h->vis, X * 4, hipMemcpyDeviceToHost, st));
+    // the four maps in one transfer into pinned memory, then into the caller's arrays
+    if (!h->pin_maps) CKH(h, hipHostMalloc(&h->pin_maps, 20 * X, hipHostMallocDefault));
+    CKH(h, hipMemcpyAsync(h->pin_maps, h->maps_block, 20 * X, hipMemcpyDeviceToHost, st));
     CKH(h, hipStreamSynchronize(st));
+    const uint8_t *m = (const uint8_t *)h->pin_maps;
+    if (pos) std::memcpy(pos, m, X * 4);
+    if (neg) std::memcpy(neg, m + X * 4, X * 4);
+    if (vis) std::memcpy(vis, m + X * 8, X * 4);
+    if (rough) std::memcpy(rough, m + X * 12, X * 8);
     if (origin_m) origin_metres(h, origin_m);
     return GVOM_OK;
 }

@@ -421,6 +421,11 @@ class Gvom:
     def sync(self):
         nat.check(self._lib.gvom_sync(self._h), self._h)
 
+    @property
+    def stream(self):
+        """The handle's HIP stream (every call's work is ordered on it), as c_void_p."""
+        return ctypes.c_void_p(self._lib.gvom_stream(self._h))
+
 
 class GvomGraph:
     """A captured launch sequence; launch() replays it with new ego positions and

@@ -323,6 +323,28 @@ def workload(cfg: int, frames: int | None = None, scale_points: float = 1.0) -> 
     raise ValueError(f"unknown configuration {cfg}")
 
 
+def refine_frames(n_frames=4, radius=8.0, turn_deg=90.0, rings=64, azimuths=1024, seed=205):
+    """Frames for extrinsic refinement on the configuration-5 world: n_frames 64 x 1024 scans
+    taken with the true (identity) extrinsic at vehicle poses spread along a circular arc of
+    `radius` m turning by `turn_deg` degrees, so that the frames see the terrain from several
+    headings (a translation of the extrinsic then moves each frame's points differently).
+    Returns (sensor-frame points per frame (float64), vehicle poses (4x4 float64), ego
+    positions)."""
+    world = make_world(102, 20, 5, False)
+    raster = _raster(world, -105, 105 + radius, -105, 105 + radius)
+    pts, poses, egos = [], [], []
+    for f in range(n_frames):
+        th = math.radians(turn_deg) * f / max(1, n_frames - 1)
+        x, y = radius * math.sin(th), radius * (1.0 - math.cos(th))
+        t = np.array([x, y, 1.0 + float(world.terrain(x, y))])
+        s = _scan_for(world, raster, np.random.default_rng(np.random.SeedSequence([seed, f])), yaw_matrix(th), t,
+                      np.eye(3), np.zeros(3), rings, azimuths, -22.5, 22.5, 1.0, 100.0, 0.05, t)
+        pts.append(s.points.astype(np.float64))
+        poses.append(s.transform.astype(np.float64))
+        egos.append(t)
+    return pts, poses, egos
+
+
 def perturb_transform(T, rng, rot_deg=0.5, trans_m=0.05):
     """Configuration 5: rotate by N(0, rot_deg) per axis and translate by N(0, trans_m)."""
     r = np.deg2rad(rng.normal(0.0, rot_deg, 3))

@@ -146,37 +146,37 @@ def map_loss(points, xi, T0, ego, loss: MapLoss):
     return _FN.apply(points, xi, T0, ego, loss)
 
 
-def refine_extrinsic(scan_points, poses, E0, loss, steps=20, lr=(0.01, 0.01, 0.01, 0.002, 0.002, 0.002), ego=None):
-    """Refine a sensor extrinsic E (sensor -> vehicle) from several scans taken at known
-    vehicle poses P_i (vehicle -> world), by Adam on the map loss of their union.
+def extrinsic_evaluator(scan_points, poses, loss, ego=None, E_ref=None):
+    """evaluate(E) -> (L, dL/dxi) for a sensor extrinsic E (sensor -> vehicle) shared by scans
+    taken at known vehicle poses P_i (vehicle -> world): maps every scan with W_i = P_i E p,
+    runs the HIP forward/backward on the union (identity transform) for dL/dW, and chains it to
+    a left perturbation E <- exp(xi^) E: dL/drho = sum R_i^T g, dL/dphi = sum q x (R_i^T g)
+    with q = E p (vehicle frame).
 
-    scan_points: list of (N_i, 3) float64 arrays in the sensor frame; poses: list of 4x4;
-    E0: starting 4x4 extrinsic.  Each step maps every scan with W_i = P_i E p, runs the
-    HIP forward/backward on the union (identity transform) for dL/dW, and chains it to a
-    left perturbation E <- exp(xi) E: dL/drho = sum R_i^T g, dL/dphi = sum q x (R_i^T g)
-    with q = E p.  Returns (E, [loss per step])."""
-    E = np.asarray(E0, np.float64).copy()
+    With E_ref, the common part of the translation change, c = mean_i R_i (t_E - t_ref), is
+    taken off every point: the loss is invariant to a common translation of all points but
+    for where the points fall in the voxel grid, and that grid aliasing would otherwise pull
+    the extrinsic's translation toward a lower-loss alignment with the grid.  The translation
+    is then found from how the frames align with one another."""
     P = [np.asarray(p, np.float64) for p in poses]
     if ego is None:
         ego = np.mean([p[:3, 3] for p in P], axis=0)
-    lr = np.asarray(lr, np.float64)
-    m = np.zeros(6)
-    v = np.zeros(6)
+    M = np.mean([p[:3, :3] for p in P], axis=0)
+    t_ref = None if E_ref is None else np.asarray(E_ref, np.float64)[:3, 3].copy()
     n = int(sum(s.shape[0] for s in scan_points))
     dev = nat.DeviceArray(np.zeros((n, 3)))
     gdev = nat.DeviceArray(np.zeros((n, 3)))
     hip = nat._hip_runtime()
-    losses = []
     eye = np.eye(4)
-    for k in range(1, steps + 2):
+
+    def evaluate(E, need_grad=True):
         qs = [s @ E[:3, :3].T + E[:3, 3] for s in scan_points]                  # vehicle frame
-        W = np.concatenate([q @ p[:3, :3].T + p[:3, 3] for q, p in zip(qs, P)])
-        W = np.ascontiguousarray(W)
+        c = np.zeros(3) if t_ref is None else M @ (E[:3, 3] - t_ref)
+        W = np.ascontiguousarray(np.concatenate([q @ p[:3, :3].T + (p[:3, 3] - c) for q, p in zip(qs, P)]))
         hip.hipMemcpy(dev.ptr, W.ctypes.data_as(ctypes.c_void_p), W.nbytes, 1)
-        out = loss.forward(dev.ptr.value, n, eye, ego)
-        losses.append(float(out[0]))
-        if k == steps + 1:
-            break
+        L = float(loss.forward(dev.ptr.value, n, eye, ego)[0])
+        if not need_grad:
+            return L, None
         loss.backward(1.0, None, gdev.ptr.value)
         gw = np.empty((n, 3))
         hip.hipMemcpy(gw.ctypes.data_as(ctypes.c_void_p), gdev.ptr, gw.nbytes, 2)
@@ -187,8 +187,78 @@ def refine_extrinsic(scan_points, poses, E0, loss, steps=20, lr=(0.01, 0.01, 0.0
             grad[:3] += gl.sum(0)
             grad[3:] += np.cross(q, gl).sum(0)
             o += q.shape[0]
-        m = 0.9 * m + 0.1 * grad
-        v = 0.999 * v + 0.001 * grad * grad
-        step = lr * (m / (1 - 0.9 ** k)) / (np.sqrt(v / (1 - 0.999 ** k)) + 1e-12)
-        E = se3_exp(-step) @ E
+        if t_ref is not None:
+            # c moves with xi as dc = M (drho + dphi x t_E): every point's -c adds -(dc/dxi)^T G
+            G = M.T @ gw.sum(0)
+            grad[:3] -= G
+            grad[3:] -= np.cross(E[:3, 3], G)
+        return L, grad
+    return evaluate
+
+
+def lm_refine(evaluate, E0, steps=20, free=(1, 1, 0, 1, 1, 1), h=(5e-3, 5e-3, 5e-3, 1e-3, 1e-3, 1e-3),
+              max_step=(0.02, 0.005), mu0=1e-2, tries=4):
+    """Levenberg-Marquardt refinement of a 4x4 rigid transform E under left perturbations
+    xi = (rho, phi): E <- exp(xi^) E.
+
+    evaluate(E, need_grad=True) -> (L, dL/dxi).  Each step builds the Hessian of the free
+    coordinates from central differences of the analytic gradient (steps h), solves the damped
+    system (H + mu diag(H)) d = -g, clips the step (max_step = (metres, radians)), and accepts
+    it when the loss falls (mu /= 3), else retries with mu *= 4 (at most `tries` times; no
+    move if none falls).  Held coordinates (free = 0) never move: a loss invariant along a
+    direction gives no information there.  Returns (E, [loss per step], [accepted steps])."""
+    E = np.asarray(E0, np.float64).copy()
+    fr = np.nonzero(np.asarray(free))[0]
+    h = np.asarray(h, np.float64)
+    mu = mu0
+    L, g = evaluate(E)
+    losses, moves = [L], []
+    for _ in range(steps):
+        H = np.zeros((6, 6))
+        for j in fr:
+            e = np.zeros(6)
+            e[j] = h[j]
+            gp = evaluate(se3_exp(e) @ E)[1]
+            gm = evaluate(se3_exp(-e) @ E)[1]
+            H[:, j] = (gp - gm) / (2 * h[j])
+        Hf = H[np.ix_(fr, fr)]
+        Hf = 0.5 * (Hf + Hf.T)
+        gf = g[fr]
+        dg = np.maximum(np.abs(np.diag(Hf)), 1e-9 * max(1.0, np.abs(np.diag(Hf)).max()))
+        moved = False
+        for _t in range(tries):
+            A = Hf + mu * np.diag(dg)
+            w, V = np.linalg.eigh(A)
+            w = np.maximum(w, 1e-12 * max(1.0, np.abs(w).max()))   # keep the step a descent step
+            d = -(V @ ((V.T @ gf) / w))
+            xi = np.zeros(6)
+            xi[fr] = d
+            tn, rn = np.linalg.norm(xi[:3]), np.linalg.norm(xi[3:])
+            if tn > max_step[0]:
+                xi[:3] *= max_step[0] / tn
+            if rn > max_step[1]:
+                xi[3:] *= max_step[1] / rn
+            En = se3_exp(xi) @ E
+            Ln, gn = evaluate(En)
+            if Ln < L:
+                E, L, g = En, Ln, gn
+                mu = max(mu / 3.0, 1e-6)
+                moved = True
+                moves.append(xi)
+                break
+            mu *= 4.0
+        if not moved:
+            moves.append(np.zeros(6))
+        losses.append(L)
+    return E, losses, moves
+
+
+def refine_extrinsic(scan_points, poses, E0, loss, steps=20, free=(1, 1, 0, 1, 1, 1), ego=None, **kw):
+    """Refine a sensor extrinsic E (sensor -> vehicle) from several scans taken at known
+    vehicle poses by minimising the map loss of their union (lm_refine over
+    extrinsic_evaluator).  The loss is invariant to a common translation of all points; with
+    poses that differ only by yaw, the vertical translation of E moves every frame alike and
+    is held by default (free[2] = 0).  Returns (E, [loss per step])."""
+    E, losses, _ = lm_refine(extrinsic_evaluator(scan_points, poses, loss, ego, E_ref=E0), E0, steps=steps,
+                             free=free, **kw)
     return E, losses

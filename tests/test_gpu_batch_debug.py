@@ -58,6 +58,35 @@ def test_batch_equals_sequential_and_oracle(cfg4):
         print(f"batched step {k}: {len(step)} scans, {sum(s.points.shape[0] for s in step)} points ok", flush=True)
 
 
+def test_cfg4_all_eight_steps_against_the_oracle():
+    """Configuration 4 at full length: 3 sensors x 8 steps (24 scans of 128/64/16 x 1024 rays,
+    10-40 % dropout) through process_batch, one combine per step, buffer 8: the ring fills
+    after the third step and wraps twice (gvom.py:198-216, 242-298).  The 3 slots each step
+    writes, the combined map and the 2D maps are compared with the oracle every step."""
+    import gvom
+    wl = synth.workload(4, frames=8)
+    bat = gvom.Gvom(*wl.map_args)
+    orc = OracleGvom(*wl.map_args)
+    for k, step in enumerate(wl.steps):
+        written = []
+        for s in step:
+            orc.Process_pointcloud(s.points, s.ego, s.transform)
+            written.append(orc.last_buffer_index)
+        pts = np.concatenate([s.points for s in step])
+        off = np.cumsum([0] + [s.points.shape[0] for s in step])
+        bat.process_batch(pts, off, np.stack([s.ego for s in step]), np.stack([s.transform for s in step]))
+        for slot in written:
+            compare_scan(bat.slot_state(slot), orc.slots[slot], f"step {k} slot {slot}")
+        out_b = bat.combine_maps()
+        out_o = orc.combine_maps()
+        compare_combined(bat.combined_state(), orc.combined, f"step {k} combined")
+        compare_maps(bat.maps_state(), orc.combined, f"step {k} maps")
+        for name, a, b in zip(("origin", "pos", "neg", "vis"), (out_b[0], out_b[1], out_b[2], out_b[4]),
+                              (out_o[0], out_o[1], out_o[2], out_o[4])):
+            assert_equal(f"step {k} {name}", a, b)
+        print(f"cfg4 step {k}: slots {written}, {len(orc.combined.hit)} combined cells ok", flush=True)
+
+
 def _sort_rows(a, keycols):
     order = np.lexsort(tuple(a[:, c] for c in reversed(keycols)))
     return a[order]

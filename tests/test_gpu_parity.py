@@ -52,8 +52,9 @@ def test_cfg2_four_frames():
 
 def test_cfg3_large_map():
     """512 x 512 x 64 voxels (16.8M per table), 236k-point 128 x 2048 scans, buffer 8, the
-    sensor moving and turning: three frames, every slot, combined map and 2D map checked."""
-    wl = synth.workload(3, frames=3)
+    sensor moving and turning along a curved path: all eight frames, so the 8-slot ring fills
+    (gvom.py:198-216, 242-298); every new slot, combined map and 2D map checked."""
+    wl = synth.workload(3, frames=8)
     _run_and_compare(wl.map_args, wl.steps, "cfg3")
 
 
