@@ -71,6 +71,17 @@ class DeviceTimer:
         return ms.value * 1e-3
 
 
+def apply_ray_config(g, cfg):
+    """cfg = (block, rays per thread, flags, points per ring, rings per patch)."""
+    b, r, f, stride, pr = cfg
+    g.set_raycast_config(b, r, bool(f & 1), bool(f & 2), bool(f & 4), bool(f & 8))
+    g.set_ray_rings(stride, pr)
+
+
+def cfg_key(cfg):
+    return ",".join(str(v) for v in cfg)
+
+
 def calls_for(t_one, units_per_call, requested_units):
     """Calls that cover max(requested, MIN_FRAMES) units and MIN_SECONDS, given one call's time."""
     need = max(requested_units, MIN_FRAMES)
@@ -173,7 +184,7 @@ def frame_bytes(N, V, X, B_used, L, C_s, sum_C_s, C, C_prev):
     return (24 * N + 4 * V + 52 * C_s) + (4 * V * (B_used + L + 2) + 52 * (sum_C_s + L * C_prev) + 100 * C + 20 * X)
 
 
-def batched_leg(requested, overlap):
+def batched_leg(requested, overlap, compare=True, ray_cfg=None):
     import gvom
     import gvom_native as nat
     import gvom_synth
@@ -189,6 +200,8 @@ def batched_leg(requested, overlap):
     g = gvom.Gvom(*wl.map_args)
     g.reserve(max(s.points.shape[0] for s in scans), max(f.shape[0] for f in flat))
     g.set_overlap(overlap)
+    if ray_cfg:
+        apply_ray_config(g, ray_cfg)
     g.graph_begin(len(scans))
     for k in range(len(wl.steps)):
         g.process_batch_device(dev[k], offs[k], np.stack([s.ego for s in wl.steps[k]]),
@@ -213,9 +226,18 @@ def batched_leg(requested, overlap):
     V, X = 256 * 256 * 64, 256 * 256
     launch_bytes = 8 * (frame_bytes(0, V, X, 8, 1, 0, sum(ring), C, C) - 4 * V) + sum(
         24 * s.points.shape[0] + 4 * V for s in scans) + 3 * 52 * sum(ring)
+    achieved = launch_bytes / t_b / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes": int(launch_bytes),
+            "counts": {"ring_cells": ring, "combined_cells": C}}
+    if not compare:
+        return {"scans_per_s": round(len(scans) / t_b, 1), "ms_per_launch": round(1e3 * t_b, 3),
+                "launches_timed": reps, "roofline": roof}
     p = gvom.Gvom(*wl.map_args)
     p.reserve(max(s.points.shape[0] for s in scans))
     p.set_overlap(overlap)
+    if ray_cfg:
+        apply_ray_config(p, ray_cfg)
     for k, s in enumerate(scans[:8]):
         p.process_pointcloud_device(sdev[k], s.points.shape[0], s.ego, s.transform)
         p.combine_maps_device()
@@ -229,6 +251,8 @@ def batched_leg(requested, overlap):
     t_p = tp.stop() / reps
     # the reference's calling pattern: host arrays in, host maps out, one call per scan
     hcall = gvom.Gvom(*wl.map_args)
+    if ray_cfg:
+        apply_ray_config(hcall, ray_cfg)
     for s in scans[:8]:
         hcall.Process_pointcloud(s.points, s.ego, s.transform)
         hcall.combine_maps()
@@ -238,7 +262,6 @@ def batched_leg(requested, overlap):
             hcall.Process_pointcloud(s.points, s.ego, s.transform)
             hcall.combine_maps()
     t_h = (time.perf_counter() - t0) / reps
-    achieved = launch_bytes / t_b / 1e9
     return {"workload": "3 sensors (128/64/16 x 1024, 10-40% dropout) x 8 steps, 256x256x64, buffer 8; "
                         "one hipGraph launch = 8 x (3-scan batch + combine_maps)",
             "scans_per_launch": len(scans), "points_per_launch": int(n_pts), "launches_timed": reps,
@@ -249,12 +272,10 @@ def batched_leg(requested, overlap):
             "speedup_vs_plain": round(t_p / t_b, 2),
             "host_api_frames_per_s": round(len(scans) / t_h, 1),
             "speedup_vs_host_api": round(t_h / t_b, 2),
-            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes": int(launch_bytes),
-                         "counts": {"ring_cells": ring, "combined_cells": C}}}
+            "roofline": roof}
 
 
-def plain_config_leg(cfg, frames, requested):
+def plain_config_leg(cfg, frames, requested, ray_cfg=None):
     """Plain frames of another configuration (2: 64 x 1024 scans into 256 x 256 x 64, buffer
     4; 3: 128 x 2048 scans into 512 x 512 x 64, buffer 8), replayed from one captured graph
     with scan/combine overlap, inputs resident in HBM: frames/s, points/s."""
@@ -265,6 +286,8 @@ def plain_config_leg(cfg, frames, requested):
     scans = [st[0] for st in wl.steps]
     g = gvom.Gvom(*wl.map_args)
     g.reserve(max(s.points.shape[0] for s in scans))
+    if ray_cfg:
+        apply_ray_config(g, ray_cfg)
     dev = [nat.DeviceArray(s.points) for s in scans]
     for k, s in enumerate(scans):
         g.process_pointcloud_device(dev[k], s.points.shape[0], s.ego, s.transform)
@@ -273,7 +296,7 @@ def plain_config_leg(cfg, frames, requested):
     fps, timed = graph_rate(g, dev, scans, requested)
     npts = float(np.mean([s.points.shape[0] for s in scans]))
     return {"workload": wl.name, "frames_per_s": round(fps, 1), "points_per_s": round(fps * npts, 1),
-            "frames_timed": timed, "mean_points_per_scan": int(npts)}
+            "frames_timed": timed, "mean_points_per_scan": int(npts), "ray_config": list(ray_cfg or ())}
 
 
 def host_api_leg(wl, scans, requested):
@@ -432,10 +455,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no graph, batched or diff legs)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="headline with its graph rate, no other legs (for comparing configurations)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration)")
     ap.add_argument("--diff-only", type=int, default=0, metavar="REPS",
                     help="run only the differentiable-loss leg with REPS timed calls (for profiling)")
+    ap.add_argument("--batched-only", type=int, default=0, metavar="SCANS",
+                    help="run only the batched graph leg over at least SCANS scans (for profiling)")
     ap.add_argument("--no-overlap", action="store_true",
                     help="keep each frame's scan stage behind the previous combine (no second stream)")
     ap.add_argument("--ray-config", default="auto",
@@ -450,6 +477,9 @@ def main():
 
     if args.diff_only:
         print(json.dumps({"fwd_bwd": diff_leg(args.diff_only, refine=False)}), flush=True)
+        return
+    if args.batched_only:
+        print(json.dumps({"batched": batched_leg(args.batched_only, not args.no_overlap, compare=False)}), flush=True)
         return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -467,13 +497,17 @@ def main():
     g = gvom.Gvom(*wl.map_args)
     g.reserve(max(s.points.shape[0] for s in scans))
     dev = [nat.DeviceArray(s.points) for s in scans]
+    AZ = 1024   # the headline sensor's azimuths: its scans are rings of 1024 points
     if args.ray_config == "auto":
         rc = g.autotune(dev[0], scans[0].points.shape[0], scans[0].ego, scans[0].transform)
+        cfg = (rc[0], rc[1], rc[2], 0, 0)
         ray_cfg = {"block": rc[0], "rays_per_thread": rc[1], "flags": rc[2], "autotuned_ms": round(rc[3], 4)}
     else:
-        b, r, f = (int(v) for v in args.ray_config.split(","))
-        g.set_raycast_config(b, r, bool(f & 1), bool(f & 2), bool(f & 4), bool(f & 8))
-        ray_cfg = {"block": b, "rays_per_thread": r, "flags": f}
+        v = [int(x) for x in args.ray_config.split(",")]
+        cfg = tuple(v + [0, 0]) if len(v) == 3 else tuple(v)
+        apply_ray_config(g, cfg)
+        ray_cfg = {"block": cfg[0], "rays_per_thread": cfg[1], "flags": cfg[2], "ring_stride": cfg[3],
+                   "patch_rings": cfg[4]}
 
     def frame(k):
         s = scans[k % len(scans)]
@@ -488,18 +522,20 @@ def main():
         # GPU with the previous frame's combine, where wavefronts spread across the scan (bit
         # 3, no effect at 64 threads) pay off: the headline configuration is the best graph
         # rate among the autotuned one and the spread 128- and 256-thread tiled casts.
-        cands = list(dict.fromkeys([(ray_cfg["block"], ray_cfg["rays_per_thread"], ray_cfg["flags"]),
-                                    (256, 1, 12), (128, 1, 12)]))
+        # Also the direct cast into the z-contiguous slot tables with ring-patch ray order
+        # (the scans are rings of AZ points): 2 azimuths x 32 rings, or 4 x 16, per wavefront.
+        cands = list(dict.fromkeys([cfg, (256, 1, 12, 0, 0), (128, 1, 12, 0, 0), (256, 1, 8, AZ, 32),
+                                    (128, 1, 8, AZ, 32), (256, 1, 8, AZ, 16)]))
         g.set_overlap(True)
         rates = {}
-        for b, r, f in cands:
-            g.set_raycast_config(b, r, bool(f & 1), bool(f & 2), bool(f & 4), bool(f & 8))
-            rates[f"{b},{r},{f}"] = round(graph_rate(g, dev, scans, MIN_FRAMES)[0], 1)
-        b, r, f = max(cands, key=lambda c: rates[f"{c[0]},{c[1]},{c[2]}"])
-        g.set_raycast_config(b, r, bool(f & 1), bool(f & 2), bool(f & 4), bool(f & 8))
+        for c in cands:
+            apply_ray_config(g, c)
+            rates[cfg_key(c)] = round(graph_rate(g, dev, scans, MIN_FRAMES)[0], 1)
+        cfg = max(cands, key=lambda c: rates[cfg_key(c)])
+        apply_ray_config(g, cfg)
         g.set_overlap(False)
-        ray_cfg = {"block": b, "rays_per_thread": r, "flags": f, "autotuned": ray_cfg,
-                   "graph_frames_per_s_by_config": rates}
+        ray_cfg = {"block": cfg[0], "rays_per_thread": cfg[1], "flags": cfg[2], "ring_stride": cfg[3],
+                   "patch_rings": cfg[4], "autotuned": ray_cfg, "graph_frames_per_s_by_config": rates}
         for k in range(len(scans)):
             frame(k)
         g.sync()
@@ -551,7 +587,7 @@ def main():
     flags = ray_cfg["flags"]
     kernel_names = {"raycast": "k_raycast<float, %s, %s>" % ("true" if flags & 1 else "false",
                                                                "true" if flags & 4 else "false"),
-                    "combine_tables": "k_comb_tables"}
+                    "combine_tables": "k_comb_rows"}   # the combine's row kernel (xy_size % 4 == 0)
     out = {
         "metric": METRIC,
         "value": round(fps, 2),
@@ -585,11 +621,11 @@ def main():
                      "algorithmic_bytes": int(b_dom)},
         "gen_seconds": round(gen_s, 1),
     }
-    if not args.quick:
+    if not args.quick and not args.headline_only:
         out["host_api"] = host_api_leg(wl, scans, args.steps)
         out["atomics"] = atomics_leg(g, ss["steps"], stages[7])
-        out["batched"] = batched_leg(args.steps, not args.no_overlap)
-        out["cfg3"] = plain_config_leg(3, 8, args.steps)
+        out["batched"] = batched_leg(args.steps, not args.no_overlap, ray_cfg=cfg)
+        out["cfg3"] = plain_config_leg(3, 8, args.steps, ray_cfg=cfg[:3] + ((2048, cfg[4]) if cfg[3] else (0, 0)))
         out["fwd_bwd"] = diff_leg(args.steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)

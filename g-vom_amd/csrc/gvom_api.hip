@@ -202,6 +202,8 @@ struct gvom_handle {
     std::vector<int> set_used;   // bit k set when event k of that set was recorded
     int cur_set = -1;
     int ray_block = 128, ray_rpt = 1, ray_box = 0, ray_xcd = 0, ray_spread = 1;
+    int64_t ray_stride = 0;   // gvom_set_ray_rings: points per ring (0: linear ray order)
+    int ray_patch = 0;        // rings per patch
     uint8_t *rsec = nullptr;
     uint32_t *rorder = nullptr;
     int8_t *grids = nullptr;   // OccupancyGrid payloads staged for host copies (5 * X bytes)
@@ -811,7 +813,7 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
         CKH(h, launch_raycast(dtype, W.world, W.valid, sd, S, NT, g, h->st, tiled ? h->tscratch : h->slot_tables,
                               tiled ? h->tscratch_words : h->slot_table_words,
                               h->ray_block, h->ray_rpt, tiled ? 0 : h->ray_box, tiled ? 1 : 0, -1,
-                              xcd ? W.rorder : nullptr, h->ray_spread, fs));
+                              xcd ? W.rorder : nullptr, h->ray_spread, h->ray_stride, h->ray_patch, fs));
     stage_event(h, 9, fs);
     if (tiled)
         CKH(h, launch_tile_convert(g, h->st, S, -1, h->tscratch, h->tscratch_words, h->slot_tables,
@@ -1803,7 +1805,8 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
                         ok = ok && launch_raycast(dtype, h->world, h->valid, h->sd, 1, n, g, h->st,
                                                   itile ? h->tscratch : scratch, itile ? h->tscratch_words : g.V,
                                                   blocks[ib], rpts[ir], ibox, itile, 0,
-                                                  iord ? h->rorder : nullptr, h->ray_spread, st) == hipSuccess;
+                                                  iord ? h->rorder : nullptr, h->ray_spread, h->ray_stride,
+                                                  h->ray_patch, st) == hipSuccess;
                         if (ok && itile)
                             ok = launch_tile_convert(g, h->st, 1, 0, h->tscratch, h->tscratch_words, scratch, g.V,
                                                      st) == hipSuccess;
@@ -1874,6 +1877,19 @@ extern "C" int gvom_set_overlap(gvom_handle *h, int32_t enabled) {
         return GVOM_ESTATE;
     }
     h->overlap = enabled ? 1 : 0;
+    return GVOM_OK;
+}
+
+extern "C" int gvom_set_ray_rings(gvom_handle *h, int64_t points_per_ring, int32_t patch_rings) {
+    if (!h) return GVOM_EINVAL;
+    if (points_per_ring != 0 && !gvom_plan::ray_patch_valid(points_per_ring, patch_rings)) {
+        std::lock_guard<std::mutex> lk(h->mu);
+        h->err = "points_per_ring must be a multiple of 64 / patch_rings (patch_rings a power of two <= 64), or 0";
+        return GVOM_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->ray_stride = points_per_ring;
+    h->ray_patch = points_per_ring ? patch_rings : 0;
     return GVOM_OK;
 }
 

@@ -27,7 +27,8 @@ hipError_t launch_ray_order(const uint8_t *rsec, int64_t NT, DevState *st, uint3
 hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, const ScanDesc *sd,
                           int S, int64_t NT, const Geo &g, const DevState *st, int32_t *slot_tables,
                           int64_t table_words, int block, int rays_per_thread, int use_box, int tiled,
-                          int force_slot, const uint32_t *order, int spread, hipStream_t s);
+                          int force_slot, const uint32_t *order, int spread, int64_t ring_stride, int patch_rings,
+                          hipStream_t s);
 // Stable LSD radix sort of (key, val) on the low `bits` (<= 32) key bits, 8-bit passes.
 // hist holds 2 x hwords per-tile digit counts: the first pass's filled by launch_prepare
 // (the last pass clears them for the next sort), the later passes' by a histogram kernel.

@@ -61,6 +61,13 @@ inline bool raycast_keys_fit(int64_t V, int32_t XY, int32_t Z, int S, bool tiled
     return raycast_key_end(V, XY, Z, S, tiled) <= KEY_END;
 }
 
+// ---- ring-patch ray order: patches of 64 / pr azimuths x pr rings of `stride` points
+inline bool ray_patch_valid(int64_t stride, int pr) {
+    if (pr < 1 || pr > 64 || (pr & (pr - 1)) != 0) return false;
+    const int pa = 64 / pr;
+    return stride >= pa && stride % pa == 0 && stride < ((int64_t)1 << 40);
+}
+
 // ---- slot tables: two physical buffers per ring slot
 inline int64_t alloc_slot_table_words(int64_t V, int B) { return 2 * (int64_t)B * V; }
 // the buffers a launch with force_slot (autotuning: every scan into buffer force_slot) or into

@@ -216,7 +216,8 @@ __global__ __launch_bounds__(256) void k_ray_order(const uint8_t *__restrict__ r
 // One ray's stepping state (gvom.py:1174-1213 typing: f32 position / slope, f64 length).
 struct Ray {
     float p0, p1, p2, inc0, inc1, inc2;
-    double length, rl1, dlen, o0, o1, o2;
+    double length, rl1, dlen;
+    int32_t o0, o1, o2;   // the scan's origin (integral, |o| < 2^23)
     bool alive;
 };
 
@@ -226,7 +227,7 @@ __device__ __forceinline__ void ray_init(Ray &r, const ScanDesc &d, const Geo &g
                                          int64_t i) {
     double o[3];
     scan_origin(d, g, o);
-    r.o0 = o[0]; r.o1 = o[1]; r.o2 = o[2];
+    r.o0 = (int32_t)o[0]; r.o1 = (int32_t)o[1]; r.o2 = (int32_t)o[2];
     float pt0 = (float)(d.ego[0] / g.xy_res);
     float pt1 = (float)(d.ego[1] / g.xy_res);
     float pt2 = (float)(d.ego[2] / g.z_res);
@@ -263,24 +264,44 @@ __device__ __forceinline__ bool ray_advance(Ray &r, const Geo &g, int32_t &ix, i
     r.p0 += r.inc0;
     r.p1 += r.inc1;
     r.p2 += r.inc2;
-    const double fx = floor((double)r.p0 - r.o0);
-    const double fy = floor((double)r.p1 - r.o1);
-    const double fz = floor((double)r.p2 - r.o2);
-    if (fx < 0 || fx >= g.XY || fy < 0 || fy >= g.XY || fz < 0 || fz >= g.Z) {
+    // gvom.py:1215-1223 takes floor(pt - origin) in f64.  pt is f32 and origin an integer,
+    // both below 2^23 in magnitude, so the f64 difference is exact and equals
+    // floor(pt) - origin: one f32 floor and an integer subtraction per axis.
+    const int32_t fx = (int32_t)floorf(r.p0) - r.o0;
+    const int32_t fy = (int32_t)floorf(r.p1) - r.o1;
+    const int32_t fz = (int32_t)floorf(r.p2) - r.o2;
+    if ((uint32_t)fx >= (uint32_t)g.XY || (uint32_t)fy >= (uint32_t)g.XY || (uint32_t)fz >= (uint32_t)g.Z) {
         r.alive = false;
         return false;
     }
     r.length += r.dlen;
     r.alive = r.length < r.rl1;
-    // in-map, so each index fits in 32 bits
-    ix = (int32_t)fx;
-    iy = (int32_t)fy;
-    iz = (int32_t)fz;
+    ix = fx;
+    iy = fy;
+    iz = fz;
     return true;
 }
 
 __device__ __forceinline__ uint32_t lin_index(int32_t ix, int32_t iy, int32_t iz, const Geo &g) {
     return ((uint32_t)ix * (uint32_t)g.XY + (uint32_t)iy) * (uint32_t)g.Z + (uint32_t)iz;
+}
+
+// ray_advance for every lane at once, without branches (so a step costs no exec-mask
+// changes): the key of the voxel the step enters, base + index(voxel), or the dead-lane
+// sentinel 0xFFFFFFFF - lane when the ray was dead or leaves the map.  A dead lane's
+// position keeps moving; nothing reads it.
+template <bool TILED>
+__device__ __forceinline__ uint32_t ray_step_key(Ray &r, const Geo &g, uint32_t base, int32_t ty, int lane);
+
+// Branch-free run merge: as run_length, with the result selected instead of branched on.
+__device__ __forceinline__ int run_length_sel(uint32_t key, int lane) {
+    const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)key, 0x138, 0xF, 0xF, false);
+    const bool is_voxel = key < 0x80000000u;
+    const bool cont = (lane > 0) & (prev == key) & is_voxel;
+    const uint64_t same = __ballot(cont);
+    const uint64_t rest = (lane == 63) ? 0ull : (same >> (lane + 1));
+    const int len = 1 + __builtin_ctzll(~rest);
+    return (is_voxel & !cont) ? len : 0;
 }
 
 // Run merge across a wavefront: lanes whose keys are equal to the previous lane's
@@ -319,6 +340,49 @@ __device__ __forceinline__ uint32_t tiled_index(int32_t ix, int32_t iy, int32_t 
            ((uint32_t)(ix & 3) << 2) + (uint32_t)(iy & 3);
 }
 
+template <bool TILED>
+__device__ __forceinline__ uint32_t ray_step_key(Ray &r, const Geo &g, uint32_t base, int32_t ty, int lane) {
+    const bool was = r.alive;
+    r.p0 += r.inc0;
+    r.p1 += r.inc1;
+    r.p2 += r.inc2;
+    // floor(pt - origin) in f64 (gvom.py:1215-1223) = floor(pt) - origin, as in ray_advance
+    const int32_t fx = (int32_t)floorf(r.p0) - r.o0;
+    const int32_t fy = (int32_t)floorf(r.p1) - r.o1;
+    const int32_t fz = (int32_t)floorf(r.p2) - r.o2;
+    const bool in = ((uint32_t)fx < (uint32_t)g.XY) & ((uint32_t)fy < (uint32_t)g.XY) & ((uint32_t)fz < (uint32_t)g.Z);
+    const bool step = was & in;
+    r.length += r.dlen;   // read only while the ray is alive
+    r.alive = step & (r.length < r.rl1);
+    const uint32_t k = base + (TILED ? tiled_index(fx, fy, fz, g, ty) : lin_index(fx, fy, fz, g));
+    return step ? k : 0xFFFFFFFFu - (uint32_t)lane;
+}
+
+// Ring-patch ray order.  For an organised scan (rings of `stride` consecutive points), a
+// wavefront traces a patch of 64 / pr adjacent azimuths x pr adjacent rings instead of 64
+// azimuths of one ring: at one step its rays sit over nearly the same column at different
+// heights, so their run heads fall on few z-contiguous 64-byte segments of the slot table
+// (measured offline on the 128 x 1024 headline scan, 2 x 32 patches: 0.30M segment requests
+// against 0.76M for the tiled scratch and 2.1M for one ring per wavefront).  Within a patch
+// the lanes run ring-fastest: adjacent lanes are adjacent rings at one azimuth, whose rays
+// share voxels near the sensor, so the run merge of adjacent lanes catches them instead of
+// several lanes of one instruction updating the same word.  Bands
+// of pr complete rings are cut into patches; rays after the last complete band (and rays of a
+// scan whose size is not a multiple of stride) follow in linear order.  A bijection on
+// [0, NT): the counts do not depend on it.
+__device__ __forceinline__ int64_t patch_ray(int64_t gw, int lane, int64_t NT, int64_t stride, int pr) {
+    const int pa = 64 / pr;
+    const int64_t per_band = stride / pa;              // patches across one band of pr rings
+    const int64_t bands = NT / (stride * pr);          // complete bands
+    const int64_t full = bands * per_band;             // wavefronts in complete bands
+    if (gw < full) {
+        const int64_t band = gw / per_band, col = gw - band * per_band;
+        const int r = lane % pr, a = lane / pr;
+        return (band * pr + r) * stride + col * pa + a;
+    }
+    return bands * stride * pr + ((gw - full) << 6) + lane;
+}
+
 constexpr int BOX_XY = 32, BOX_Z = 16;
 constexpr int BOX_N = BOX_XY * BOX_XY * BOX_Z;   // 64 KiB of int32 counters
 
@@ -328,7 +392,8 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
                                                  const ScanDesc *__restrict__ sd, int S, int64_t NT,
                                                  Geo g, const DevState *__restrict__ st,
                                                  int32_t *__restrict__ tables, int maxit, int rpt, int force_slot,
-                                                 const uint32_t *__restrict__ order, int spread) {
+                                                 const uint32_t *__restrict__ order, int spread,
+                                                 int64_t ring_stride, int patch_rings) {
     extern __shared__ int32_t box[];
     // With `order` (XCD-aware schedule) block b traces chunk b / 8 of azimuth sector b % 8;
     // workgroups are dispatched to the 8 XCDs round-robin, so each XCD traces one sector.
@@ -369,9 +434,11 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
         // spread: the block's wavefronts come from across the scan (wavefront w of pass rep of
         // block b traces rays of global wavefront (rep * waves per block + w) * blocks + b), so
         // the long-ray rings do not end up together on a few CUs; lanes stay adjacent rays
-        const int64_t j = (spread && !order)
-                              ? ((((int64_t)rep * (blockDim.x >> 6) + (threadIdx.x >> 6)) * gridDim.x + chunk) << 6) + lane
-                              : (chunk * rpt + rep) * blockDim.x + threadIdx.x;
+        const int64_t wpb = blockDim.x >> 6;
+        const int64_t gw = (spread && !order) ? ((int64_t)rep * wpb + (threadIdx.x >> 6)) * gridDim.x + chunk
+                                              : (chunk * rpt + rep) * wpb + (threadIdx.x >> 6);
+        const int64_t j = (patch_rings > 0 && !order) ? patch_ray(gw, lane, NT, ring_stride, patch_rings)
+                                                      : (gw << 6) + lane;
         const int64_t i = RAY_INDEX(j);
         bool inbox_scan = false;
         Ray r;
@@ -391,6 +458,21 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
                 toff = (int64_t)RAY_SLOT(s) * g.V - (int64_t)s * g.V;
             }
             ray_init<P>(r, sd[s], g, world, i);
+        }
+        if (!USE_BOX) {
+            // two steps per round: both keys are formed before either run merge, so the two
+            // steps' arithmetic overlaps; a ray takes at most max(xy, z) + 1 steps, so the
+            // round that passes maxit has no live lane left
+            for (int it = 0; it < maxit; it += 2) {
+                if (__ballot(r.alive) == 0ull) break;
+                const uint32_t k1 = ray_step_key<TILED>(r, g, base, tg.ty, lane);
+                const uint32_t k2 = ray_step_key<TILED>(r, g, base, tg.ty, lane);
+                const int l1 = run_length_sel(k1, lane);
+                const int l2 = run_length_sel(k2, lane);
+                if (l1 > 0) __hip_atomic_fetch_add(tables + toff + k1, l1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (l2 > 0) __hip_atomic_fetch_add(tables + toff + k2, l2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            continue;
         }
         for (int it = 0; it < maxit; ++it) {
             if (__ballot(r.alive) == 0ull) break;
@@ -898,49 +980,6 @@ __device__ __forceinline__ double red32(double v) {
     return v;
 }
 
-// Ring bookkeeping after the scan stage of S scans (gvom.py:211-216), by one thread: slot
-// origins for combine (gvom.py:208), the ego the height map uses (gvom.py:110-112), the
-// written slots' buffer bits and cell counts, then the ring indices.  Everything the combine
-// reads about the ring is written here, after the scan stage, so that the stage's kernels
-// may run while a combine still reads the previous state.
-__device__ __forceinline__ void commit_process(DevState *st, const ScanDesc *sd, const Geo &g, int S) {
-    const int B = g.B;
-    int next = st->next_slot;
-    for (int s = 0; s < S; ++s) {
-        double o[3];
-        scan_origin(sd[s], g, o);
-        const int slot = (next + s) % B;
-        st->slot_origin[slot][0] = o[0];
-        st->slot_origin[slot][1] = o[1];
-        st->slot_origin[slot][2] = o[2];
-    }
-    st->latest_ego[0] = sd[S - 1].ego[0];
-    st->latest_ego[1] = sd[S - 1].ego[1];
-    st->latest_ego[2] = sd[S - 1].ego[2];
-    int mask = st->filled_mask, flip = st->flip;
-    for (int s = 0; s < S; ++s) {
-        const int slot = (next + s) % B;
-        mask |= 1 << slot;
-        flip ^= 1 << slot;   // the buffer the stage wrote now holds the slot
-        st->slot_cells[slot] = st->scan_cells[s];
-    }
-    st->filled_mask = mask;
-    st->flip = flip;
-    const int last = (next + S - 1) % B;
-    st->last_slot = last;
-    st->next_slot = (next + S) % B;
-    // the combine's source shifts: combined origin = newest slot's (gvom.py:229), each slot read
-    // at (combined voxel + origin difference) (gvom.py:1016-1022)
-    const double *co = st->slot_origin[last];
-    for (int s = 0; s < B; ++s) {
-        const double *so = st->slot_origin[s];
-        st->comb_shift[s][0] = (int32_t)(co[0] - so[0]);
-        st->comb_shift[s][1] = (int32_t)(co[1] - so[1]);
-        st->comb_shift[s][2] = (int32_t)(co[2] - so[2]);
-        st->comb_shift[s][3] = (mask >> s) & 1;
-    }
-}
-
 __global__ __launch_bounds__(256) void k_metrics(const uint32_t *__restrict__ seg_key,
                                                  const Moment *__restrict__ mom, Geo g,
                                                  DevState *__restrict__ st,
@@ -1035,8 +1074,72 @@ __global__ __launch_bounds__(256) void k_metrics(const uint32_t *__restrict__ se
     }
 }
 
-__global__ void k_commit_process(DevState *st, const ScanDesc *sd, Geo g, int S) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) commit_process(st, sd, g, S);
+// Ring bookkeeping after the scan stage of S scans (gvom.py:211-216): slot origins for
+// combine (gvom.py:208), the ego the height map uses (gvom.py:110-112), the written slots'
+// buffer bits and cell counts, the ring indices and the next combine's source shifts.
+// Everything the combine reads about the ring is written here, after the scan stage, so that
+// the stage's kernels may run while a combine still reads the previous state.  One wavefront:
+// lane s < S computes the origin of the batch's s-th slot, lane j < B reads slot j's origin
+// and writes its combine shift, so the loads are issued together, not one after another.
+__global__ __launch_bounds__(64) void k_commit_process(DevState *st, const ScanDesc *sd, Geo g, int S) {
+    if (blockIdx.x != 0) return;
+    const int lane = threadIdx.x;
+    const int B = g.B;
+    const int next = st->next_slot, mask0 = st->filled_mask, flip0 = st->flip;
+    // lane s < S: the new origin of slot (next + s) % B and the scan's cell count
+    double no[3] = {0.0, 0.0, 0.0};
+    int32_t cells = 0;
+    if (lane < S) {
+        scan_origin(sd[lane], g, no);
+        cells = st->scan_cells[lane];
+    }
+    // lane j < B: slot j's origin after this commit (new if the batch writes slot j)
+    double so[3] = {0.0, 0.0, 0.0};
+    const int wj = lane < B ? (lane - next + B) % B : 64;   // batch index that writes slot `lane`
+    if (lane < B && wj >= S) {
+        so[0] = st->slot_origin[lane][0];
+        so[1] = st->slot_origin[lane][1];
+        so[2] = st->slot_origin[lane][2];
+    }
+    const int src = wj < S ? wj : 0;
+    const double n0 = __shfl(no[0], src), n1 = __shfl(no[1], src), n2 = __shfl(no[2], src);
+    if (lane < B && wj < S) {
+        so[0] = n0;
+        so[1] = n1;
+        so[2] = n2;
+    }
+    int mask = mask0, flip = flip0;
+    for (int k = 0; k < S; ++k) {
+        const int slot = (next + k) % B;
+        mask |= 1 << slot;
+        flip ^= 1 << slot;   // the buffer the stage wrote now holds the slot
+    }
+    const int last = (next + S - 1) % B;
+    // combined origin = newest slot's (gvom.py:229); each slot is read at (combined voxel +
+    // origin difference) (gvom.py:1016-1022)
+    const double c0 = __shfl(so[0], last), c1 = __shfl(so[1], last), c2 = __shfl(so[2], last);
+    if (lane < S) {
+        const int slot = (next + lane) % B;
+        st->slot_origin[slot][0] = no[0];
+        st->slot_origin[slot][1] = no[1];
+        st->slot_origin[slot][2] = no[2];
+        st->slot_cells[slot] = cells;
+    }
+    if (lane < B) {
+        st->comb_shift[lane][0] = (int32_t)(c0 - so[0]);
+        st->comb_shift[lane][1] = (int32_t)(c1 - so[1]);
+        st->comb_shift[lane][2] = (int32_t)(c2 - so[2]);
+        st->comb_shift[lane][3] = (mask >> lane) & 1;
+    }
+    if (lane == 0) {
+        st->latest_ego[0] = sd[S - 1].ego[0];
+        st->latest_ego[1] = sd[S - 1].ego[1];
+        st->latest_ego[2] = sd[S - 1].ego[2];
+        st->filled_mask = mask;
+        st->flip = flip;
+        st->last_slot = last;
+        st->next_slot = (next + S) % B;
+    }
 }
 
 
@@ -1117,7 +1220,7 @@ hipError_t launch_ray_order(const uint8_t *rsec, int64_t NT, DevState *st, uint3
 hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, const ScanDesc *sd,
                           int S, int64_t NT, const Geo &g, const DevState *st, int32_t *tables,
                           int64_t table_words, int block, int rpt, int use_box, int tiled, int force_slot,
-                          const uint32_t *order, int spread, hipStream_t s) {
+                          const uint32_t *order, int spread, int64_t ring_stride, int patch_rings, hipStream_t s) {
     if (NT <= 0) return hipSuccess;
     if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
     if (tiled && use_box) return hipErrorInvalidValue;
@@ -1126,6 +1229,7 @@ hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, co
     const int64_t need = tiled ? (int64_t)S * gvom_plan::tile_words(g.XY, g.Z)
                                : gvom_plan::slot_table_footprint(g.V, g.B, force_slot);
     if (need > table_words) return hipErrorInvalidValue;   // the launch would write past `tables`
+    if (patch_rings != 0 && !gvom_plan::ray_patch_valid(ring_stride, patch_rings)) return hipErrorInvalidValue;
     if (rpt < 1) rpt = 1;
     int maxit = (g.XY > g.Z ? g.XY : g.Z) + 2;
     unsigned nb = nblocks(NT, block * rpt);
@@ -1133,7 +1237,8 @@ hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, co
     const size_t lds = use_box ? BOX_N * sizeof(int32_t) : 0;
 #define RAY_LAUNCH(P, BOX, TL)                                                                              \
     k_raycast<P, BOX, TL><<<nb, block, BOX ? lds : 0, s>>>((const P *)world, valid, sd, S, NT, g, st, tables, \
-                                                           maxit, rpt, force_slot, order, spread)
+                                                           maxit, rpt, force_slot, order, spread, ring_stride, \
+                                                           patch_rings)
     if (dtype == GVOM_F32) {
         if (tiled) RAY_LAUNCH(float, false, true);
         else if (use_box) RAY_LAUNCH(float, true, false);

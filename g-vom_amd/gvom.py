@@ -224,6 +224,12 @@ class Gvom:
         flags = (1 if use_box else 0) | (2 if sector_order else 0) | (4 if tiled else 0) | (8 if spread else 0)
         nat.check(self._lib.gvom_set_raycast_config(self._h, int(block), int(rays_per_thread), flags), self._h)
 
+    def set_ray_rings(self, points_per_ring, patch_rings=32):
+        """Ray order hint for organised scans (rings of points_per_ring consecutive points):
+        each wavefront of the ray cast traces 64 / patch_rings azimuths x patch_rings rings.
+        Same results for any order; 0 restores the linear order."""
+        nat.check(self._lib.gvom_set_ray_rings(self._h, int(points_per_ring), int(patch_rings)), self._h)
+
     def set_overlap(self, enabled=True):
         """Let a device-pointer process call run while the preceding combine_maps_device
         still runs (off by default).  Its points must be complete when the call is made."""

@@ -64,6 +64,7 @@ SIGNATURES = {
     "gvom_set_diff_geometry_only": (ctypes.c_int, [_P, _I32]),
     "gvom_hip_runtime_path": (ctypes.c_char_p, []),
     "gvom_set_raycast_config": (ctypes.c_int, [_P, _I32, _I32, _I32]),
+    "gvom_set_ray_rings": (ctypes.c_int, [_P, _I64, _I32]),
     "gvom_diff_backward": (ctypes.c_int, [_P, ctypes.c_double, _P, _P, _P]),
     "gvom_diff_forward_async": (ctypes.c_int, [_P, _P, _I64, _P, _P, ctypes.c_double, _P, _P]),
     "gvom_diff_backward_async": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),

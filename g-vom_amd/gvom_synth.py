@@ -323,26 +323,30 @@ def workload(cfg: int, frames: int | None = None, scale_points: float = 1.0) -> 
     raise ValueError(f"unknown configuration {cfg}")
 
 
-def refine_frames(n_frames=4, radius=8.0, turn_deg=90.0, rings=64, azimuths=1024, seed=205):
-    """Frames for extrinsic refinement on the configuration-5 world: n_frames 64 x 1024 scans
-    taken with the true (identity) extrinsic at vehicle poses spread along a circular arc of
-    `radius` m turning by `turn_deg` degrees, so that the frames see the terrain from several
-    headings (a translation of the extrinsic then moves each frame's points differently).
+REFINE_POSES = ((0.0, 0.0, 0.0), (6.0, 3.0, 45.0), (12.0, 1.0, -30.0), (15.0, -5.0, -80.0))
+
+
+def refine_frames(poses=REFINE_POSES, rings=64, azimuths=1024, seed=205):
+    """Frames for extrinsic refinement on the configuration-5 world: 64 x 1024 scans taken
+    with the true (identity) extrinsic at vehicle poses (x m, y m, heading deg) along an
+    S-shaped path.  The headings differ, so a translation of the extrinsic moves each frame's
+    points differently; the poses lie on no common circle, so no yaw of the extrinsic combined
+    with a translation moves all frames as one rigid body (on a circular arc it would, and
+    the loss could not tell that combination from the truth).
     Returns (sensor-frame points per frame (float64), vehicle poses (4x4 float64), ego
     positions)."""
     world = make_world(102, 20, 5, False)
-    raster = _raster(world, -105, 105 + radius, -105, 105 + radius)
-    pts, poses, egos = [], [], []
-    for f in range(n_frames):
-        th = math.radians(turn_deg) * f / max(1, n_frames - 1)
-        x, y = radius * math.sin(th), radius * (1.0 - math.cos(th))
+    raster = _raster(world, -105, 125, -110, 110)
+    pts, poses_out, egos = [], [], []
+    for f, (x, y, hd) in enumerate(poses):
+        th = math.radians(hd)
         t = np.array([x, y, 1.0 + float(world.terrain(x, y))])
         s = _scan_for(world, raster, np.random.default_rng(np.random.SeedSequence([seed, f])), yaw_matrix(th), t,
                       np.eye(3), np.zeros(3), rings, azimuths, -22.5, 22.5, 1.0, 100.0, 0.05, t)
         pts.append(s.points.astype(np.float64))
-        poses.append(s.transform.astype(np.float64))
+        poses_out.append(s.transform.astype(np.float64))
         egos.append(t)
-    return pts, poses, egos
+    return pts, poses_out, egos
 
 
 def perturb_transform(T, rng, rot_deg=0.5, trans_m=0.05):

@@ -197,20 +197,24 @@ def extrinsic_evaluator(scan_points, poses, loss, ego=None, E_ref=None):
 
 
 def lm_refine(evaluate, E0, steps=20, free=(1, 1, 0, 1, 1, 1), h=(5e-3, 5e-3, 5e-3, 1e-3, 1e-3, 1e-3),
-              max_step=(0.02, 0.005), mu0=1e-2, tries=4):
-    """Levenberg-Marquardt refinement of a 4x4 rigid transform E under left perturbations
-    xi = (rho, phi): E <- exp(xi^) E.
+              scale=(0.02, 0.005), radius0=1.0, radius_max=4.0):
+    """Trust-region Levenberg-Marquardt refinement of a 4x4 rigid transform E under left
+    perturbations xi = (rho, phi): E <- exp(xi^) E.
 
-    evaluate(E, need_grad=True) -> (L, dL/dxi).  Each step builds the Hessian of the free
-    coordinates from central differences of the analytic gradient (steps h), solves the damped
-    system (H + mu diag(H)) d = -g, clips the step (max_step = (metres, radians)), and accepts
-    it when the loss falls (mu /= 3), else retries with mu *= 4 (at most `tries` times; no
-    move if none falls).  Held coordinates (free = 0) never move: a loss invariant along a
-    direction gives no information there.  Returns (E, [loss per step], [accepted steps])."""
+    evaluate(E, need_grad=True) -> (L, dL/dxi).  Each step builds the Hessian H of the free
+    coordinates from central differences of the analytic gradient (steps h) and takes the
+    minimiser of the quadratic model g.d + d.H.d / 2 within a trust region |d / s| <= radius
+    (s = `scale`: metres for rho, radians for phi), found on the Levenberg-Marquardt path
+    (H + mu diag(1/s^2)) d = -g.  The step is accepted when the loss falls; the radius halves
+    when the loss fell by less than a quarter of the model's prediction and doubles (up to
+    radius_max) when the model predicted well at the boundary.  Held coordinates (free = 0)
+    never move: a loss invariant along a direction gives no information there.  Returns
+    (E, [loss after each step, starting with the initial loss], [accepted steps])."""
     E = np.asarray(E0, np.float64).copy()
     fr = np.nonzero(np.asarray(free))[0]
     h = np.asarray(h, np.float64)
-    mu = mu0
+    sc = np.array([scale[0]] * 3 + [scale[1]] * 3, np.float64)[fr]
+    radius = radius0
     L, g = evaluate(E)
     losses, moves = [L], []
     for _ in range(steps):
@@ -223,32 +227,43 @@ def lm_refine(evaluate, E0, steps=20, free=(1, 1, 0, 1, 1, 1), h=(5e-3, 5e-3, 5e
             H[:, j] = (gp - gm) / (2 * h[j])
         Hf = H[np.ix_(fr, fr)]
         Hf = 0.5 * (Hf + Hf.T)
-        gf = g[fr]
-        dg = np.maximum(np.abs(np.diag(Hf)), 1e-9 * max(1.0, np.abs(np.diag(Hf)).max()))
-        moved = False
-        for _t in range(tries):
-            A = Hf + mu * np.diag(dg)
-            w, V = np.linalg.eigh(A)
-            w = np.maximum(w, 1e-12 * max(1.0, np.abs(w).max()))   # keep the step a descent step
-            d = -(V @ ((V.T @ gf) / w))
-            xi = np.zeros(6)
-            xi[fr] = d
-            tn, rn = np.linalg.norm(xi[:3]), np.linalg.norm(xi[3:])
-            if tn > max_step[0]:
-                xi[:3] *= max_step[0] / tn
-            if rn > max_step[1]:
-                xi[3:] *= max_step[1] / rn
-            En = se3_exp(xi) @ E
-            Ln, gn = evaluate(En)
-            if Ln < L:
-                E, L, g = En, Ln, gn
-                mu = max(mu / 3.0, 1e-6)
-                moved = True
-                moves.append(xi)
-                break
-            mu *= 4.0
-        if not moved:
+        # scaled coordinates u = d / s
+        Hs = Hf * np.outer(sc, sc)
+        gs = g[fr] * sc
+        w, V = np.linalg.eigh(Hs)
+        b = V.T @ gs
+
+        def step(mu):
+            return -(V @ (b / (w + mu)))
+        lo = max(0.0, -w.min()) + 1e-12 * max(1.0, np.abs(w).max())
+        u = step(lo)
+        if np.linalg.norm(u) > radius:
+            hi = lo + 1.0
+            while np.linalg.norm(step(hi)) > radius:
+                hi = lo + 2.0 * (hi - lo)
+            for _b in range(60):
+                mid = 0.5 * (lo + hi)
+                if np.linalg.norm(step(mid)) > radius:
+                    lo = mid
+                else:
+                    hi = mid
+            u = step(hi)
+        d = u * sc
+        pred = -(g[fr] @ d + 0.5 * d @ Hf @ d)
+        xi = np.zeros(6)
+        xi[fr] = d
+        En = se3_exp(xi) @ E
+        Ln, gn = evaluate(En)
+        rho = (L - Ln) / pred if pred > 0 else -1.0
+        if Ln < L:
+            E, L, g = En, Ln, gn
+            moves.append(xi)
+        else:
             moves.append(np.zeros(6))
+        if rho < 0.25:
+            radius *= 0.5
+        elif rho > 0.75 and np.linalg.norm(u) > 0.99 * radius:
+            radius = min(2.0 * radius, radius_max)
         losses.append(L)
     return E, losses, moves
 

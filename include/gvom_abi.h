@@ -322,6 +322,16 @@ int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int32_t dtype, 
  * the first process call or gvom_graph_begin. */
 int gvom_set_raycast_config(gvom_handle *h, int32_t block, int32_t rays_per_thread, int32_t flags);
 
+/* Ray order hint (no reference counterpart): the scans are organised in rings of
+ * points_per_ring consecutive points (a spinning lidar's PointCloud2 rows, e.g. 1024 for a
+ * 1024-azimuth sensor).  Each wavefront of the ray cast then traces a patch of
+ * 64 / patch_rings adjacent azimuths x patch_rings adjacent rings, whose steps fall on few
+ * z-contiguous 64-byte segments of the slot table, instead of 64 azimuths of one ring.  The
+ * counts are the same for any order; the hint only changes the cast's speed, and a wrong
+ * stride only makes it slower.  points_per_ring must be a multiple of 64 / patch_rings
+ * (patch_rings a power of two up to 64); 0 restores the linear order (the default). */
+int gvom_set_ray_rings(gvom_handle *h, int64_t points_per_ring, int32_t patch_rings);
+
 /* Differentiable form (no reference counterpart).  Processes one scan like
  * gvom_process_pointcloud into the next ring slot (without the ray cast when
  * gvom_set_diff_geometry_only is on; points: device array of n x 3

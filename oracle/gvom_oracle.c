@@ -50,7 +50,9 @@ typedef struct {
  *   2 = combine shift with the sign of d reversed (gvom.py:1017-1026, 1044-1053),
  *   3 = inferred height one voxel too high (gvom.py:587),
  *   4 = robot-radius test on the cell centre instead of its corner (gvom.py:566-567),
- *   5 = transform without the translation column (gvom.py:1128-1134). */
+ *   5 = transform without the translation column (gvom.py:1128-1134),
+ *   9 = minimum-distance filter measured from the ego instead of the world origin
+ *       (gvom.py:1145-1149). */
 static int g_planted = 0;
 void orc_set_planted_mistake(int m) { g_planted = m; }
 
@@ -127,8 +129,12 @@ static int in_domain(const orc_params *p, double x, double y, double z) {
 }
 
 /* Minimum-distance filter on world coordinates, in the point dtype (gvom.py:1145-1149). */
-static int passes_min_distance(const orc_params *p, const void *pts, int dt, int64_t i) {
+static int passes_min_distance(const orc_params *p, const void *pts, int dt, int64_t i, const double ego[3]) {
     double md2 = p->min_distance * p->min_distance;
+    if (g_planted == 9) {
+        double e0 = ld(pts, dt, 3 * i) - ego[0], e1 = ld(pts, dt, 3 * i + 1) - ego[1], e2 = ld(pts, dt, 3 * i + 2) - ego[2];
+        return !(e0 * e0 + e1 * e1 + e2 * e2 < md2);
+    }
     if (dt == ORC_F32) {
         const float *q = (const float *)pts + 3 * i;
         float d2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
@@ -207,7 +213,7 @@ int64_t orc_process(const orc_params *p, const void *pts, int64_t n, int32_t dt,
     for (int64_t i = 0; i < n; ++i) {
         double x = ld(pts_world, dt, 3 * i), y = ld(pts_world, dt, 3 * i + 1),
                z = ld(pts_world, dt, 3 * i + 2);
-        valid[i] = (uint8_t)(in_domain(p, x, y, z) && passes_min_distance(p, pts_world, dt, i));
+        valid[i] = (uint8_t)(in_domain(p, x, y, z) && passes_min_distance(p, pts_world, dt, i, ego));
     }
 
     memset(scratch_hit, 0, (size_t)V * 4);

@@ -95,12 +95,12 @@ int main() {
                          nullptr) == hipErrorInvalidValue, "radix histogram one word short");
         const int64_t tw = P::tile_words(g.XY, g.Z);
         CHECK(launch_raycast(GVOM_F32, nullptr, nullptr, nullptr, 2, n, g, nullptr, nullptr, 2 * tw - 1, 256, 1, 0, 1,
-                             -1, nullptr, 1, nullptr) == hipErrorInvalidValue, "tiled cast into a short scratch");
+                             -1, nullptr, 1, 0, 0, nullptr) == hipErrorInvalidValue, "tiled cast into a short scratch");
         CHECK(launch_raycast(GVOM_F32, nullptr, nullptr, nullptr, 2, n, g, nullptr, nullptr,
-                             P::alloc_slot_table_words(g.V, g.B) - 1, 256, 1, 0, 0, -1, nullptr, 1, nullptr) ==
+                             P::alloc_slot_table_words(g.V, g.B) - 1, 256, 1, 0, 0, -1, nullptr, 1, 0, 0, nullptr) ==
                   hipErrorInvalidValue, "direct cast into short slot tables");
         CHECK(launch_raycast(GVOM_F32, nullptr, nullptr, nullptr, 1, n, g, nullptr, nullptr, g.V - 1, 256, 1, 0, 0, 0,
-                             nullptr, 1, nullptr) == hipErrorInvalidValue, "autotune cast into a short table");
+                             nullptr, 1, 0, 0, nullptr) == hipErrorInvalidValue, "autotune cast into a short table");
         CHECK(launch_tile_convert(g, nullptr, 3, -1, nullptr, 3 * tw - 1, nullptr, P::alloc_slot_table_words(g.V, g.B),
                                   nullptr) == hipErrorInvalidValue, "conversion from a short scratch");
         CHECK(launch_tile_convert(g, nullptr, 3, -1, nullptr, 3 * tw, nullptr, P::alloc_slot_table_words(g.V, g.B) - 1,
@@ -122,7 +122,7 @@ int main() {
         CHECK(P::raycast_keys_fit(h.V, h.XY, h.Z, 8, false) && !P::raycast_keys_fit(h.V, h.XY, h.Z, 9, false),
               "2048 x 2048 x 64 direct keys");
         CHECK(launch_raycast(GVOM_F32, nullptr, nullptr, nullptr, 9, n, h, nullptr, nullptr,
-                             P::alloc_slot_table_words(h.V, 9), 256, 1, 0, 0, -1, nullptr, 1, nullptr) ==
+                             P::alloc_slot_table_words(h.V, 9), 256, 1, 0, 0, -1, nullptr, 1, 0, 0, nullptr) ==
                   hipErrorInvalidValue, "keys past 2^31 refused");
     }
     printf("%lld shapes\n", cases);

@@ -201,6 +201,55 @@ def test_ray_cast_variants_equal(cfg4, flags):
         print(f"ray variant {flags}: step {k} slots and combined map identical", flush=True)
 
 
+@pytest.mark.parametrize("rings", [(1024, 32, 8), (1024, 16, 12), (1000, 8, 8), (1024, 64, 0), (960, 2, 8)])
+def test_ring_patch_ray_order_equal(cfg4, rings):
+    """The ring-patch ray order (gvom_set_ray_rings: patches of 64 / pr azimuths x pr rings)
+    leaves slot tables, records, the combined map and the 2D maps bit-identical to the linear
+    order, for single scans and ragged 3-scan batches (whose rings are 1024 points before
+    dropout, so the patches only approximate rings), with the direct and the tiled cast, and
+    for strides that do not divide the scans."""
+    import gvom
+    stride, pr, flags = rings
+    wl = cfg4
+    a = gvom.Gvom(*wl.map_args)
+    a.set_raycast_config(256, 1, False, False, False, False)
+    b = gvom.Gvom(*wl.map_args)
+    b.set_raycast_config(256, 1, bool(flags & 1), bool(flags & 2), bool(flags & 4), bool(flags & 8))
+    b.set_ray_rings(stride, pr)
+    for k, step in enumerate(wl.steps):
+        if k == 0:
+            for s in step:
+                a.Process_pointcloud(s.points, s.ego, s.transform)
+                b.Process_pointcloud(s.points, s.ego, s.transform)
+        else:
+            pts = np.concatenate([s.points for s in step])
+            off = np.cumsum([0] + [s.points.shape[0] for s in step])
+            egos = np.stack([s.ego for s in step])
+            Ts = np.stack([s.transform for s in step])
+            a.process_batch(pts, off, egos, Ts)
+            b.process_batch(pts, off, egos, Ts)
+        for slot in range(wl.map_args[4]):
+            sa, sb = a.slot_state(slot), b.slot_state(slot)
+            assert (sa is None) == (sb is None)
+            if sa is not None:
+                _state_equal(sa, sb, f"rings {rings} step {k} slot {slot}")
+        a.combine_maps()
+        b.combine_maps()
+        _state_equal(a.combined_state(), b.combined_state(), f"rings {rings} step {k} combined")
+        _state_equal(a.maps_state(), b.maps_state(), f"rings {rings} step {k} maps")
+        print(f"ring patches {rings}: step {k} slots and combined map identical", flush=True)
+
+
+def test_ring_patch_hint_rejected():
+    import gvom
+    import gvom_native as nat
+    g = gvom.Gvom(*synth.gvom_args(0.4, 0.2, 64, 32, 2))
+    for stride, pr in ((1023, 32), (1024, 3), (1024, 128), (1, 32)):
+        with pytest.raises(nat.GvomError):
+            g.set_ray_rings(stride, pr)
+    g.set_ray_rings(0, 0)
+
+
 @pytest.mark.parametrize("variant", [(128, 1, 4), (256, 1, 0)])
 def test_overlap_matches_serial(variant):
     """With scan/combine overlap on, device-pointer frames issued back to back (eager, then

@@ -1,4 +1,4 @@
-"""Closed-form pins of five oracle functions -- CPU only.
+"""Closed-form pins of six oracle functions -- CPU only.
 
 Each test fixes the oracle's answer from what the mathematics of the reference's rule
 forces, not from a second restatement of the same formula, and each runs again with a
@@ -18,7 +18,9 @@ planted mistake in the oracle (oracle.gvom_oracle.planted_mistake) that must mak
     radius of k cells, the cells given the default height are the lattice points of a disc
     (Gauss circle counts 5, 13, 29, 49, 81, 113);
   - debug exporters (gvom.py:455-503): rows hand-computed from a three-cell map whose
-    hits, totals and eigenvalues are worked out by hand below.
+    hits, totals and eigenvalues are worked out by hand below;
+  - the minimum-distance filter (gvom.py:1145-1149): measured from the world origin after the
+    transform, not from the ego, with |p| = min_distance kept.
 
 All coordinates are dyadic, so every float in these scenes is exact and the expected
 values are exact.
@@ -279,6 +281,42 @@ def test_robot_radius_cells_are_the_lattice_disc(k):
 def test_robot_radius_pin_catches_the_cell_centre():
     cm, _, _, _ = _radius_case(1, 4)
     assert (cm.height > -1000.0).sum() != GAUSS_CIRCLE[1]
+
+
+# -- minimum-distance filter -----------------------------------------------------------
+def _min_distance_case(planted):
+    """min_distance 2 m, ego at (6, 0, 0.5): points within 2 m of the world origin are dropped
+    (no hit, no ray) although they are in the map and far from the ego; points within 2 m of
+    the ego are kept (gvom.py:1145-1149 measures from the world origin after the transform)."""
+    xy, z = 32, 16
+    o = OracleGvom(*_params(xy, z, 1, min_distance=2.0))
+    ego = np.array([6.0, 0.0, 0.5])
+    near_origin = np.array([[0.5625, 0.5625, 0.3125], [-1.0625, 0.8125, 0.3125], [1.8125, 0.0625, 0.0625]])
+    on_circle = np.array([[2.0, 0.0, 0.0]])                # |p| = 2 exactly: d2 < 4 is false, kept
+    near_ego = ego + np.array([[0.3125, 0.3125, 0.0625], [-0.5625, 0.8125, -0.1875]])
+    pts = np.concatenate([near_origin, on_circle, near_ego])
+    with planted_mistake(planted):
+        o.Process_pointcloud(pts, ego)
+    sl = o.slots[0]
+    occ = canonical_scan(sl.index_map, sl.hit, sl.total)["occ"]
+    org = sl.origin
+
+    def vox(p):
+        f = np.floor(np.array([p[0] / XYR, p[1] / XYR, p[2] / ZR]) - org).astype(int)
+        return f[0] + f[1] * xy + f[2] * xy * xy
+    kept = [bool(occ[vox(p)]) for p in pts]
+    return kept, int(occ.sum()), int(sl.valid.sum())
+
+
+def test_min_distance_is_measured_from_the_world_origin():
+    kept, cells, valid = _min_distance_case(0)
+    assert kept == [False, False, False, True, True, True]
+    assert valid == 3 and cells == 3
+
+
+def test_min_distance_pin_catches_a_filter_around_the_ego():
+    kept, _, _ = _min_distance_case(9)
+    assert kept != [False, False, False, True, True, True]
 
 
 # -- debug exporters ------------------------------------------------------------------

@@ -431,3 +431,34 @@ def test_positive_obstacle_density_window():
     dens = sum(hits[i] for i in sel) / sum(tots[i] for i in sel)
     assert o.combined.height[col] == h
     assert o.combined.pos[col] == int(dens * 100)
+
+
+@pytest.mark.parametrize("zs,expect", [((4, 1, 3, 2), 100),    # delta = (4 - 1) z_res = 1.5 > 0.5
+                                       ((3, 2, 3, 2), 0)])     # delta = 0.5 exactly: not > 0.5
+def test_negative_obstacle_threshold(zs, expect):
+    """__make_negative_obstacle_map (gvom.py:506-512) on a hand-built slot: an unobserved
+    column with a free voxel (an inferred height) whose four neighbours are observed at voxel
+    heights zs (+x, -x, +y, -y); the guessed height difference is max - min of the neighbours'
+    heights (gvom.py:593-714, the inferred height lies below them), 100 where it exceeds the
+    threshold."""
+    params = (1.0, 0.5, 12, 16, 1, 0.1, 0.5, 0.5, 0.3, 2.0, 0.1, 1.0, 0, 0)
+    o = OracleGvom(*params)
+    o.Process_pointcloud(np.array([[100.0, 100.0, 100.0]], np.float32), (6.5, 6.5, 4.25))   # sets up a slot
+    sl = o.slots[0]
+    XYm = 12
+    idx = np.full(o.voxel_count, -1, np.int32)
+    c = (5, 5)
+    idx[c[0] + c[1] * XYm + 2 * XYm * XYm] = -4          # free, 3 passes: inferred height at z = 2
+    for i, ((dx, dy), z) in enumerate(zip(((1, 0), (-1, 0), (0, 1), (0, -1)), zs)):
+        idx[(c[0] + dx) + (c[1] + dy) * XYm + z * XYm * XYm] = i
+    sl.index_map = idx
+    sl.hit = np.full(4, 3, np.int32)
+    sl.total = np.full(4, 3, np.int32)
+    sl.min_h = np.full(4, 0.25, np.float32)
+    sl.metrics = np.zeros((4, 10))
+    sl.metrics[:, 9] = 1.0
+    out = o.combine_maps()
+    oz = sl.origin[2]
+    assert o.combined.height[c] == -1000.0 and o.combined.inferred[c] == (2 + oz) * 0.5
+    assert o.combined.delta[c] == (max(zs) - min(zs)) * 0.5
+    assert out[2][c] == expect
