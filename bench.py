@@ -349,10 +349,12 @@ def copy_calibration(mib, reps=5):
     print(json.dumps({"copy_calibration_bytes": n, "copies": reps}), flush=True)
 
 
-def diff_leg(requested, refine=True):
+def diff_leg(requested, refine=True, sync_only=False):
     """Differentiable loss on one 65,536-ray scan: forward and forward + backward through the
     stream-ordered entry points (no host synchronisation inside the timed region; device
-    events on the loss handle's stream), after the handle has captured its forward graph."""
+    events on the loss handle's stream), after the handle has captured its forward graph.
+    sync_only: time only the synchronous entry points (the profiling mode: a profiled run of the
+    stream-ordered calls ended in a host segfault inside the HIP runtime, see the design notes)."""
     import ctypes
     import gvom_native as nat
     import gvom_synth
@@ -369,6 +371,20 @@ def diff_leg(requested, refine=True):
     one = nat.DeviceArray(np.ones(1))
     gxi = nat.DeviceArray(np.zeros(6))
     loss = MapLoss(0.4, 0.2, 256, 64)
+    if sync_only:
+        loss.forward(dev.ptr.value, n, T, s.ego)
+        loss.backward(1.0, None, gdev.ptr.value)
+        t0 = time.perf_counter()
+        for _ in range(requested):
+            loss.forward(dev.ptr.value, n, T, s.ego)
+        t_f = (time.perf_counter() - t0) / requested
+        t0 = time.perf_counter()
+        for _ in range(requested):
+            loss.forward(dev.ptr.value, n, T, s.ego)
+            loss.backward(1.0, None, gdev.ptr.value)
+        t_fb = (time.perf_counter() - t0) / requested
+        return {"api": "synchronous", "calls_timed": requested, "fwd_ms": round(1e3 * t_f, 4),
+                "fwd_bwd_ms": round(1e3 * t_fb, 4)}
     stream = loss.g.stream
     sv = stream.value or 0
     tm = DeviceTimer(stream)
@@ -419,10 +435,10 @@ def diff_leg(requested, refine=True):
     if not refine:
         return timing
 
-    # 20 Levenberg-Marquardt steps refining a sensor extrinsic perturbed by N(0, 0.5 deg) /
-    # N(0, 5 cm) and shared by 4 frames at known vehicle poses on a 90-degree arc of 8 m radius.
-    # The loss is invariant to a common translation of all points, and the poses differ only
-    # by yaw, so the extrinsic's vertical offset moves every frame alike and is held.
+    # 20 trust-region Levenberg-Marquardt steps refining a sensor extrinsic perturbed by
+    # N(0, 0.5 deg) / N(0, 5 cm) and shared by 4 frames at known vehicle poses along an S-shaped
+    # path.  The loss is invariant to a common translation of all points, and the poses differ
+    # only by yaw, so the extrinsic's vertical offset moves every frame alike and is held.
     from gvom_torch import extrinsic_evaluator, lm_refine
     pts4, poses4, _ = gvom_synth.refine_frames()
     E0 = gvom_synth.perturb_transform(np.eye(4, dtype=np.float32), np.random.default_rng(205)).astype(np.float64)
@@ -437,9 +453,10 @@ def diff_leg(requested, refine=True):
         return float(ang), float(np.linalg.norm(Ea[:3, 3])), [round(float(v), 4) for v in Ea[:3, 3]]
     return {**timing,
             "refine_extrinsic_4_frames_20_steps": {
-                "workload": "4 frames of 64x1024 scans at known vehicle poses on a 90 deg, 8 m arc, shared "
-                            "extrinsic perturbed by N(0, 0.5 deg) / N(0, 5 cm); 20 Levenberg-Marquardt steps "
-                            "(Hessian from central differences of the HIP gradient), vertical offset held",
+                "workload": "4 frames of 64x1024 scans at known vehicle poses along an S-shaped path (headings "
+                            "0/45/-30/-80 deg), shared extrinsic perturbed by N(0, 0.5 deg) / N(0, 5 cm); 20 "
+                            "trust-region Levenberg-Marquardt steps (Hessian from central differences of the HIP "
+                            "gradient), vertical offset held",
                 "loss": [round(Ls[0], 4), round(Ls[-1], 4)], "loss_at_true_extrinsic": round(ev(np.eye(4), False)[0], 4),
                 "rot_err_deg": [round(err_e(E0)[0], 4), round(err_e(E1)[0], 4)],
                 "trans_err_m": [round(err_e(E0)[1], 4), round(err_e(E1)[1], 4)],
@@ -476,10 +493,15 @@ def main():
         sys.exit(2)
 
     if args.diff_only:
-        print(json.dumps({"fwd_bwd": diff_leg(args.diff_only, refine=False)}), flush=True)
+        print(json.dumps({"fwd_bwd": diff_leg(args.diff_only, refine=False, sync_only=True)}), flush=True)
         return
     if args.batched_only:
-        print(json.dumps({"batched": batched_leg(args.batched_only, not args.no_overlap, compare=False)}), flush=True)
+        rcfg = None
+        if args.ray_config != "auto":
+            v = [int(x) for x in args.ray_config.split(",")]
+            rcfg = tuple(v + [0, 0]) if len(v) == 3 else tuple(v)
+        print(json.dumps({"batched": batched_leg(args.batched_only, not args.no_overlap, compare=False,
+                                                 ray_cfg=rcfg)}), flush=True)
         return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -498,6 +520,7 @@ def main():
     g.reserve(max(s.points.shape[0] for s in scans))
     dev = [nat.DeviceArray(s.points) for s in scans]
     AZ = 1024   # the headline sensor's azimuths: its scans are rings of 1024 points
+    cfg_linear = None
     if args.ray_config == "auto":
         rc = g.autotune(dev[0], scans[0].points.shape[0], scans[0].ego, scans[0].transform)
         cfg = (rc[0], rc[1], rc[2], 0, 0)
@@ -522,16 +545,19 @@ def main():
         # GPU with the previous frame's combine, where wavefronts spread across the scan (bit
         # 3, no effect at 64 threads) pay off: the headline configuration is the best graph
         # rate among the autotuned one and the spread 128- and 256-thread tiled casts.
-        # Also the direct cast into the z-contiguous slot tables with ring-patch ray order
-        # (the scans are rings of AZ points): 2 azimuths x 32 rings, or 4 x 16, per wavefront.
-        cands = list(dict.fromkeys([cfg, (256, 1, 12, 0, 0), (128, 1, 12, 0, 0), (256, 1, 8, AZ, 32),
-                                    (128, 1, 8, AZ, 32), (256, 1, 8, AZ, 16)]))
+        # Also the direct cast into the z-contiguous slot tables with ring-patch ray order (the
+        # scans are rings of AZ points): one azimuth x 64 rings per wavefront, spread or not.
+        cands = list(dict.fromkeys([cfg, (256, 1, 12, 0, 0), (128, 1, 12, 0, 0), (256, 1, 0, AZ, 64),
+                                    (128, 1, 0, AZ, 64), (256, 1, 8, AZ, 64)]))
         g.set_overlap(True)
         rates = {}
         for c in cands:
             apply_ray_config(g, c)
             rates[cfg_key(c)] = round(graph_rate(g, dev, scans, MIN_FRAMES)[0], 1)
         cfg = max(cands, key=lambda c: rates[cfg_key(c)])
+        # scans whose dropped returns are removed are not organised in rings: the batched and
+        # configuration-3 legs (10-40 % dropout) take the best linear-order configuration
+        cfg_linear = max((c for c in cands if c[3] == 0), key=lambda c: rates[cfg_key(c)])
         apply_ray_config(g, cfg)
         g.set_overlap(False)
         ray_cfg = {"block": cfg[0], "rays_per_thread": cfg[1], "flags": cfg[2], "ring_stride": cfg[3],
@@ -624,8 +650,8 @@ def main():
     if not args.quick and not args.headline_only:
         out["host_api"] = host_api_leg(wl, scans, args.steps)
         out["atomics"] = atomics_leg(g, ss["steps"], stages[7])
-        out["batched"] = batched_leg(args.steps, not args.no_overlap, ray_cfg=cfg)
-        out["cfg3"] = plain_config_leg(3, 8, args.steps, ray_cfg=cfg[:3] + ((2048, cfg[4]) if cfg[3] else (0, 0)))
+        out["batched"] = batched_leg(args.steps, not args.no_overlap, ray_cfg=cfg_linear)
+        out["cfg3"] = plain_config_leg(3, 8, args.steps, ray_cfg=cfg_linear)
         out["fwd_bwd"] = diff_leg(args.steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)

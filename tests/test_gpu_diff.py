@@ -244,11 +244,12 @@ def test_repeated_forward_replays_match_eager(scene):
 
 
 def test_extrinsic_refinement_reduces_loss_and_rotation_error():
-    """20 Levenberg-Marquardt steps on the HIP loss and gradient: a sensor extrinsic perturbed
-    by N(0, 0.5 deg) / N(0, 5 cm) (seeded) and shared by 4 frames at known poses on a 90-degree
-    arc; the loss and the rotation error must both fall (the rotation error to under 70 % of
-    its start).  The translation error is printed: the loss's sensitivity to a few centimetres
-    of extrinsic offset is of the order of its discretisation noise."""
+    """20 trust-region Levenberg-Marquardt steps on the HIP loss and gradient: a sensor
+    extrinsic perturbed by N(0, 0.5 deg) / N(0, 5 cm) (seeded) and shared by 4 frames at known
+    poses along an S-shaped path (no common circle, so no yaw-translation combination moves the
+    frames as one rigid body); the loss falls at every step, the rotation error to under a
+    quarter of its start and the horizontal translation error to under half (the vertical offset
+    is held: the loss is invariant to it)."""
     from gvom_torch import MapLoss, extrinsic_evaluator, lm_refine
     pts, poses, _ = synth.refine_frames()
     E0 = synth.perturb_transform(np.eye(4, dtype=np.float32), np.random.default_rng(205)).astype(np.float64)
@@ -258,7 +259,8 @@ def test_extrinsic_refinement_reduces_loss_and_rotation_error():
     def rot_err(E):
         return float(np.degrees(np.arccos(np.clip((np.trace(E[:3, :3]) - 1) / 2, -1, 1))))
     print(f"loss {Ls[0]:.3f} -> {Ls[-1]:.3f} (true extrinsic {ev(np.eye(4), False)[0]:.3f}); rotation error "
-          f"{rot_err(E0):.3f} -> {rot_err(E1):.3f} deg; translation {E0[:3, 3]} -> {E1[:3, 3]} m", flush=True)
+          f"{rot_err(E0):.4f} -> {rot_err(E1):.4f} deg; translation {E0[:3, 3]} -> {E1[:3, 3]} m", flush=True)
     assert all(b <= a for a, b in zip(Ls, Ls[1:]))
-    assert Ls[-1] < 0.8 * Ls[0]
-    assert rot_err(E1) < 0.7 * rot_err(E0)
+    assert Ls[-1] < 0.5 * Ls[0]
+    assert rot_err(E1) < 0.25 * rot_err(E0)
+    assert np.linalg.norm(E1[:2, 3]) < 0.5 * np.linalg.norm(E0[:2, 3])
