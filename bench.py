@@ -89,7 +89,7 @@ def calls_for(t_one, units_per_call, requested_units):
     if t_one > 0:
         calls = max(calls, int(np.ceil(MIN_SECONDS / t_one)))
     return int(calls)
-STAGES = ["clear+prepare", "raycast", "sort+segments_beyond_raycast", "moments+metrics", "combine_tables",
+STAGES = ["prepare", "clear+raycast", "sort+segments_beyond_raycast", "moments+metrics", "combine_tables",
           "records+columns", "maps2d", "raycast_kernel", "tile_convert"]
 
 

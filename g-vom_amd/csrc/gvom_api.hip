@@ -791,9 +791,6 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     h->diff_n = -1;   // every check passed: the work buffers a pending gvom_diff_backward reads are reused below
     stage_event(h, 0, fs);
     const bool xcd = cast && h->ray_xcd != 0;
-    // the tiled path's conversion writes every word of the slot tables; the direct cast
-    // counts into cleared tables
-    if (!tiled) CKH(h, launch_clear_slots(h->slot_tables, h->st, S, g, fs));
     if (xcd) CKH(h, launch_fill_i32(&h->st->sec_count[0], 0, 16, fs));   // sec_count and sec_cursor
     if (W.clear_hist) CKH(h, launch_fill_i32((int32_t *)W.hist, 0, W.hwords, fs));
     CKH(h, launch_prepare(dtype, pts, W.world, W.k0, W.v0, W.valid, sd, S, NT, g, h->st,
@@ -808,6 +805,9 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     CKH(h, launch_segments(sk, NT, W.seg_start, W.seg_key, W.blk, W.blk_words, S, g, h->st, h->side));
     CKH(h, launch_mom_part(dtype, W.world, sk, sv, NT, sd, S, g, h->st, W.part, h->side));
     CKH(h, hipEventRecord(h->ev_join, h->side));
+    // the tiled path's conversion writes every word of the slot tables; the direct cast counts
+    // into cleared tables, cleared here, beside the sort chain (nothing before the cast reads them)
+    if (!tiled) CKH(h, launch_clear_slots(h->slot_tables, h->st, S, g, fs));
     stage_event(h, 1, fs);
     if (cast)
         CKH(h, launch_raycast(dtype, W.world, W.valid, sd, S, NT, g, h->st, tiled ? h->tscratch : h->slot_tables,

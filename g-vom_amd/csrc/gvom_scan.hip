@@ -1150,8 +1150,6 @@ __global__ __launch_bounds__(256) void k_clear_slots(int32_t *__restrict__ table
     // grid (ceil(V / 4096), S): block (b, s) clears words [4096 b, 4096 (b + 1)) of the table
     // of the batch's s-th slot; a wavefront stores 1 KiB contiguous per round
     const int s = blockIdx.y;
-    if (blockIdx.x == 0 && s == 0 && threadIdx.x < 16)
-        (&const_cast<DevState *>(st)->sec_count[0])[threadIdx.x] = 0;   // sec_count and sec_cursor
     const int buf = __builtin_amdgcn_readfirstlane(new_buf(st->flip, (st->next_slot + s) % B));
     int32_t *t = tables + (int64_t)buf * V;
     const int32_t base = (int32_t)blockIdx.x * 4096;

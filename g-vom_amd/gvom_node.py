@@ -113,6 +113,14 @@ class GvomNode:
             print("no odom")
             return False
         tf_matrix = transform_from_translation_rotation(translation, rotation)
+        # An organised cloud (height = rings, width = azimuths) keeps its rows on the device,
+        # non-finite returns included (the scan stage drops them), so the ray cast can trace
+        # ring patches; the order does not change the counts.
+        height, width = int(getattr(cloud, "height", 1)), int(getattr(cloud, "width", 0))
+        stride = width if height >= 64 and width > 0 else 0
+        if stride != getattr(self, "_ray_stride", None):
+            self.voxel_mapper.set_ray_rings(stride, 64)
+            self._ray_stride = stride
         self.voxel_mapper.process_pointcloud2(cloud, self.odom_data, tf_matrix, device_data=device_data)
         return True
 
