@@ -68,7 +68,7 @@ constexpr ScratchLayout scratch_layout(int64_t n) {
     L.k1 = at;        at = align256(at + n * 4);
     L.v1 = at;        at = align256(at + n * 4);
     L.hwords = gvom_plan::radix_hist_words(n);
-    L.hist = at;      at = align256(at + 2 * L.hwords * 4);
+    L.hist = at;      at = align256(at + gvom_plan::RADIX_HIST_SETS * L.hwords * 4);
     L.blk_words = gvom_plan::segment_block_words(n);
     L.blk = at;       at = align256(at + L.blk_words * 4);
     L.seg_key = at;   at = align256(at + n * 4);
@@ -474,10 +474,10 @@ static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) 
         CKH(h, dalloc(&h->v0, pc));
         CKH(h, dalloc(&h->k1, pc));
         CKH(h, dalloc(&h->v1, pc));
-        // the sort's two histograms; the first must start at zero (see radix_sort)
+        // the sort's histogram sets; the first must start at zero (see radix_sort)
         h->hwords = radix_hist_entries(pc);
-        CKH(h, dalloc(&h->hist, 2 * h->hwords));
-        CKH(h, hipMemsetAsync(h->hist, 0, sizeof(uint32_t) * 2 * h->hwords, h->stream));
+        CKH(h, dalloc(&h->hist, gvom_plan::RADIX_HIST_SETS * h->hwords));
+        CKH(h, hipMemsetAsync(h->hist, 0, sizeof(uint32_t) * gvom_plan::RADIX_HIST_SETS * h->hwords, h->stream));
         CKH(h, hipStreamSynchronize(h->stream));
         CKH(h, dalloc(&h->blk, segment_block_words(pc)));
         CKH(h, dalloc(&h->seg_key, pc));
@@ -800,7 +800,7 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     CKH(h, hipEventRecord(h->ev_fork, fs));
     CKH(h, hipStreamWaitEvent(h->side, h->ev_fork, 0));
     uint32_t *sk = W.k0, *sv = W.v0;
-    CKH(h, radix_sort(W.k0, W.v0, W.k1, W.v1, NT, g.VB + sbits, W.hist, W.hwords, &sk, &sv, h->side));
+    CKH(h, radix_sort(W.k0, W.v0, W.k1, W.v1, NT, g.VB + sbits, W.hist, W.hwords, h->st, &sk, &sv, h->side));
     h->sorted_vals = sv;
     CKH(h, launch_segments(sk, NT, W.seg_start, W.seg_key, W.blk, W.blk_words, S, g, h->st, h->side));
     CKH(h, launch_mom_part(dtype, W.world, sk, sv, NT, sd, S, g, h->st, W.part, h->side));
@@ -1119,6 +1119,10 @@ extern "C" int gvom_sync(gvom_handle *h) {
 static int read_state(gvom_handle *h, DevState *out) {
     CKH(h, hipMemcpyAsync(out, h->st, sizeof(DevState), hipMemcpyDeviceToHost, h->stream));
     CKH(h, hipStreamSynchronize(h->stream));
+    if (out->sort_fault) {
+        h->err = "internal: a radix-sort tile gave up waiting for the counts of an earlier tile";
+        return GVOM_ESTATE;
+    }
     return GVOM_OK;
 }
 

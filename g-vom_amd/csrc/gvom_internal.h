@@ -68,6 +68,9 @@ struct DevState {
     // whether the slot holds a scan; written with the ring indices at the end of every scan
     // stage for the combine that follows
     alignas(16) int32_t comb_shift[GVOM_MAX_BUFFER][4];
+    // set when a radix-sort tile gave up waiting for an earlier tile's counts (a defect; the
+    // state getters then report an error)
+    int32_t sort_fault;
 };
 
 // Physical buffer of ring slot `slot`: the one holding its scan, and the one the scan stage in

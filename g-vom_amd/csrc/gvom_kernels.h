@@ -29,12 +29,15 @@ hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, co
                           int64_t table_words, int block, int rays_per_thread, int use_box, int tiled,
                           int force_slot, const uint32_t *order, int spread, int64_t ring_stride, int patch_rings,
                           hipStream_t s);
-// Stable LSD radix sort of (key, val) on the low `bits` (<= 32) key bits, 8-bit passes.
-// hist holds 2 x hwords per-tile digit counts: the first pass's filled by launch_prepare
-// (the last pass clears them for the next sort), the later passes' by a histogram kernel.
-// On return *kout/*vout point at whichever buffer pair holds the result.
+// Stable LSD radix sort of (key, val) on the low `bits` (<= 32) key bits, 8-bit passes, one
+// kernel per pass.  hist holds RADIX_HIST_SETS x hwords words: the first pass's per-tile
+// counts, filled by launch_prepare (the last pass clears them for the next sort), and the
+// later passes' published tile counts and digit totals, zeroed by launch_prepare.  A tile
+// that gives up waiting sets st->sort_fault.  On return *kout/*vout point at whichever
+// buffer pair holds the result.
 hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits,
-                      uint32_t *hist, int64_t hwords, uint32_t **kout, uint32_t **vout, hipStream_t s);
+                      uint32_t *hist, int64_t hwords, DevState *st, uint32_t **kout, uint32_t **vout,
+                      hipStream_t s);
 int64_t radix_hist_entries(int64_t n);
 // words of the segment kernels' per-block counts (launch_segments' blk) for n sorted keys
 int64_t segment_block_words(int64_t n);

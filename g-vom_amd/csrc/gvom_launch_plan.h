@@ -21,8 +21,12 @@ constexpr int64_t blocks(int64_t n, int64_t per) {
     return (n + per - 1) / per > 0 ? (n + per - 1) / per : 1;
 }
 
-// ---- radix sort: one 256-digit histogram per tile (two sets: first pass, later passes)
+// ---- radix sort: RADIX_HIST_SETS sets of 256 words per tile + 256: the first pass's per-tile
+// histogram, then per later pass (at most three: keys have <= 32 bits) the tiles' published
+// digit counts and the pass's digit totals
+constexpr int RADIX_HIST_SETS = 4;
 constexpr int64_t radix_hist_words(int64_t n) { return 256 * blocks(n, RS_TILE) + 256; }
+constexpr int64_t radix_hist_alloc_words(int64_t n) { return RADIX_HIST_SETS * radix_hist_words(n); }
 
 // ---- segment kernels: per block of SEG_TILE keys, W = 2 + 3 S counters (heads, valid keys,
 // then per scan of the batch in-map heads, halo heads, valid keys), plus one row of totals;

@@ -94,9 +94,11 @@ __device__ __forceinline__ int ray_sector(double dx, double dy) {
 // Radix sort tiles: RS_TILE keys per block of every pass.
 constexpr int RS_TILE = gvom_plan::RS_TILE;
 
-// With `hist` (the radix sort's first-pass histogram, hwords words, zero on entry: the
-// sort's last pass clears it for the next scan stage), the per-tile digit counts of the
-// first pass are accumulated here, so the sort starts with its first scatter.
+// With `hist` (the radix sort's RADIX_HIST_SETS x hwords words; the first set, the first
+// pass's per-tile histogram, is zero on entry: the sort's last pass clears it for the next
+// scan stage), the per-tile digit counts of the first pass are accumulated here, so the sort
+// starts with its first scatter, and the later passes' sets (published tile counts and digit
+// totals) are zeroed.
 template <typename P>
 __global__ __launch_bounds__(256) void k_prepare(const P *__restrict__ in, P *__restrict__ world,
                                                  uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
@@ -148,6 +150,13 @@ __global__ __launch_bounds__(256) void k_prepare(const P *__restrict__ in, P *__
         }
         rsec[i] = (uint8_t)sec;
     }
+    }
+    if (hist) {
+        // the words of sets 1 .. RADIX_HIST_SETS - 1 that a sort of NT keys uses
+        const int64_t rw = gvom_plan::radix_hist_words(NT);
+        const int64_t nz = (int64_t)(gvom_plan::RADIX_HIST_SETS - 1) * rw;
+        for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nz; k += (int64_t)gridDim.x * blockDim.x)
+            hist[(1 + k / rw) * hwords + k % rw] = 0;
     }
     if (rsec || hist) __syncthreads();
     if (rsec && threadIdx.x < 8 && scount[threadIdx.x]) atomicAdd(&st->sec_count[threadIdx.x], scount[threadIdx.x]);
@@ -524,34 +533,108 @@ constexpr int RS_THREADS = 1024;
 constexpr int RS_WAVES = RS_THREADS / 64;
 constexpr int RS_ROWS = RS_TILE / RS_THREADS;   // 64-key rows per wavefront
 
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t *__restrict__ keys, int64_t n, int shift,
-                                                        uint32_t *__restrict__ hist) {
-    __shared__ uint32_t h[256];
-    if (threadIdx.x < 256) h[threadIdx.x] = 0;
-    __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+// Published tile counts of the lookback passes: count | RS_READY once the tile has ranked.
+constexpr uint32_t RS_READY = 0x80000000u;
+// Polls of an unpublished count before a tile gives up (about 30 ms with s_sleep 1): every
+// tile publishes before it waits, and tiles wait only on lower ones, so only a defect can
+// reach the bound; the tile then uses what it has (positions stay in [0, n)) and flags it.
+constexpr int RS_SPIN_LIMIT = 1 << 20;
+
+// Ranks the tile's keys by the digit at `shift`, stably: rows in key order, waves in order.
+// wcnt[w][d] ends as the count of digit d in wavefront w; lo = rank within the wavefront.
+// With nh, the next digit of every key is counted into nh (the next pass's totals).
+__device__ __forceinline__ void rs_rank_tile(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                             int64_t n, int shift, int64_t base, int lane, int w,
+                                             uint32_t (*wcnt)[256], uint32_t *nh, uint32_t (&kk)[RS_ROWS],
+                                             uint32_t (&vv)[RS_ROWS], uint32_t (&lo)[RS_ROWS]) {
+    const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
-    for (int k = 0; k < RS_ROWS; ++k) {
-        const int64_t i = base + k * RS_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
+    for (int r = 0; r < RS_ROWS; ++r) {
+        int64_t i = base + r * 64 + lane;
+        bool in = i < n;
+        uint32_t key = in ? kin[i] : 0u;
+        kk[r] = key;
+        vv[r] = in ? vin[i] : 0u;
+        uint32_t d = (key >> shift) & 0xFFu;
+        if (nh && in) atomicAdd(&nh[(key >> (shift + 8)) & 0xFFu], 1u);
+        uint64_t m = __ballot(in);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        uint32_t rank = __popcll(m & lt), cnt = __popcll(m);
+        uint32_t prior = wcnt[w][d];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        if (in && rank == cnt - 1) wcnt[w][d] = prior + cnt;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        lo[r] = prior + rank;
     }
-    __syncthreads();
-    if (threadIdx.x < 256) hist[(int64_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];   // block-major
 }
 
-// hist: this pass's per-tile digit counts (block-major).  hclear (last pass only) is the
-// first pass's histogram, cleared for the next sort.
+// Turns wcnt[.][d] into exclusive wave offsets (thread d < 256) and returns the tile's count
+// of digit d.
+__device__ __forceinline__ uint32_t rs_wave_offsets(uint32_t (*wcnt)[256], int d) {
+    uint32_t a = 0;
+#pragma unroll
+    for (int ww = 0; ww < RS_WAVES; ++ww) {
+        uint32_t c = wcnt[ww][d];
+        wcnt[ww][d] = a;
+        a += c;
+    }
+    return a;
+}
+
+// Exclusive prefix of the digits' totals inside the wavefront (thread d < 256 holds digit
+// d's); the wavefront's sum goes to wtot[w].  After a barrier, rs_digit_add adds the sums of
+// the wavefronts before.
+__device__ __forceinline__ uint32_t rs_digit_scan(uint32_t total, int lane, int w, uint32_t *wtot) {
+    uint32_t v = total;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t u = __shfl_up(v, off);
+        if (lane >= off) v += u;
+    }
+    if (lane == 63) wtot[w] = v;
+    return v - total;
+}
+
+__device__ __forceinline__ uint32_t rs_digit_add(const uint32_t *wtot, int w) {
+    uint32_t wo = 0;
+    for (int ww = 0; ww < w; ++ww) wo += wtot[ww];
+    return wo;
+}
+
+__device__ __forceinline__ void rs_scatter_tile(uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, int64_t n,
+                                                int shift, int64_t base, int lane, int w, uint32_t (*wcnt)[256],
+                                                const uint32_t *gbase, const uint32_t (&kk)[RS_ROWS],
+                                                const uint32_t (&vv)[RS_ROWS], const uint32_t (&lo)[RS_ROWS]) {
+#pragma unroll
+    for (int r = 0; r < RS_ROWS; ++r) {
+        int64_t i = base + r * 64 + lane;
+        if (i < n) {
+            uint32_t d = (kk[r] >> shift) & 0xFFu;
+            uint32_t pos = gbase[d] + wcnt[w][d] + lo[r];
+            kout[pos] = kk[r];
+            vout[pos] = vv[r];
+        }
+    }
+}
+
+// First pass.  hist: its per-tile digit counts (block-major, from k_prepare).  tot_next
+// (zero on entry, or null in a one-pass sort): the next digit's totals, accumulated here.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *__restrict__ kin,
                                                            const uint32_t *__restrict__ vin,
                                                            uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                            int64_t n, int shift, const uint32_t *__restrict__ hist,
-                                                           int nblk, uint32_t *__restrict__ hclear, int64_t hwords) {
+                                                           int nblk, uint32_t *__restrict__ tot_next) {
     __shared__ uint32_t wcnt[RS_WAVES][256];
     __shared__ uint32_t ptot[4][256], pbef[4][256];
-    __shared__ uint32_t gbase[256];
+    __shared__ uint32_t gbase[256], nh[256];
     __shared__ uint32_t wtot[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int k = threadIdx.x; k < RS_WAVES * 256; k += RS_THREADS) (&wcnt[0][0])[k] = 0;
+    if (threadIdx.x < 256) nh[threadIdx.x] = 0;
     {
         // global offset of digit d for this block: keys of smaller digits in all blocks plus
         // keys of digit d in earlier blocks.  Four threads per digit each add every fourth
@@ -571,74 +654,103 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *__res
     __syncthreads();
     if (threadIdx.x < 256) {
         const int d = threadIdx.x;
-        const uint32_t total = ptot[0][d] + ptot[1][d] + ptot[2][d] + ptot[3][d];
-        const uint32_t before = pbef[0][d] + pbef[1][d] + pbef[2][d] + pbef[3][d];
-        uint32_t v = total;
-        for (int off = 1; off < 64; off <<= 1) {
-            uint32_t u = __shfl_up(v, off);
-            if (lane >= off) v += u;
-        }
-        if (lane == 63) wtot[w] = v;
-        ptot[0][d] = v - total + before;   // digit-local part, wave offsets added below
+        ptot[0][d] = rs_digit_scan(ptot[0][d] + ptot[1][d] + ptot[2][d] + ptot[3][d], lane, w, wtot) +
+                     pbef[0][d] + pbef[1][d] + pbef[2][d] + pbef[3][d];
     }
     __syncthreads();
-    if (threadIdx.x < 256) {
-        uint32_t wo = 0;
-        for (int ww = 0; ww < w; ++ww) wo += wtot[ww];
-        gbase[threadIdx.x] = wo + ptot[0][threadIdx.x];
-    }
-    // rank every key inside its wavefront's rows (stable: rows in key order, waves in order)
+    if (threadIdx.x < 256) gbase[threadIdx.x] = rs_digit_add(wtot, w) + ptot[0][threadIdx.x];
     const int64_t base = (int64_t)blockIdx.x * RS_TILE + w * (RS_ROWS * 64);
-    const uint64_t lt = (1ull << lane) - 1ull;
     uint32_t kk[RS_ROWS], vv[RS_ROWS], lo[RS_ROWS];
+    rs_rank_tile(kin, vin, n, shift, base, lane, w, wcnt, tot_next ? nh : nullptr, kk, vv, lo);
+    __syncthreads();
+    if (threadIdx.x < 256) {
+        rs_wave_offsets(wcnt, threadIdx.x);
+        if (tot_next && nh[threadIdx.x]) atomicAdd(&tot_next[threadIdx.x], nh[threadIdx.x]);
+    }
+    __syncthreads();
+    rs_scatter_tile(kout, vout, n, shift, base, lane, w, wcnt, gbase, kk, vv, lo);
+}
+
+// Later passes, without a histogram kernel: each tile ranks its keys, publishes its digit
+// counts (count | RS_READY in status[tile][d], zero on entry), then adds the counts of the
+// tiles before it, waiting for those not yet published; `tot` (this digit's totals,
+// accumulated by the previous pass) gives the digits' starts.  A tile waits only on lower
+// tiles, which were dispatched before it or are dispatched as other work retires, so every
+// wait ends (and is bounded regardless).  tot_next as in k_rs_scatter; hclear (last pass) is
+// the first pass's histogram, cleared for the next sort.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter_lb(const uint32_t *__restrict__ kin,
+                                                              const uint32_t *__restrict__ vin,
+                                                              uint32_t *__restrict__ kout,
+                                                              uint32_t *__restrict__ vout, int64_t n, int shift,
+                                                              uint32_t *__restrict__ status,
+                                                              const uint32_t *__restrict__ tot,
+                                                              uint32_t *__restrict__ tot_next, int nblk,
+                                                              uint32_t *__restrict__ hclear, int64_t hwords,
+                                                              DevState *__restrict__ st) {
+    __shared__ uint32_t wcnt[RS_WAVES][256];
+    __shared__ uint32_t pbef[4][256];
+    __shared__ uint32_t gbase[256], nh[256];
+    __shared__ uint32_t wtot[4];
+    __shared__ int fault;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int me = (int)blockIdx.x;
+    for (int k = threadIdx.x; k < RS_WAVES * 256; k += RS_THREADS) (&wcnt[0][0])[k] = 0;
+    if (threadIdx.x < 256) nh[threadIdx.x] = 0;
+    if (threadIdx.x == 0) fault = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)me * RS_TILE + w * (RS_ROWS * 64);
+    uint32_t kk[RS_ROWS], vv[RS_ROWS], lo[RS_ROWS];
+    rs_rank_tile(kin, vin, n, shift, base, lane, w, wcnt, tot_next ? nh : nullptr, kk, vv, lo);
+    __syncthreads();
+    if (threadIdx.x < 256) {
+        const uint32_t c = rs_wave_offsets(wcnt, threadIdx.x);
+        __hip_atomic_store(&status[(int64_t)me * 256 + threadIdx.x], c | RS_READY, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        if (tot_next && nh[threadIdx.x]) atomicAdd(&tot_next[threadIdx.x], nh[threadIdx.x]);
+    }
+    {
+        // counts of digit d in the tiles before this one; four threads per digit, eight loads
+        // in flight per thread before any is waited on
+        const int d = threadIdx.x & 255, q = threadIdx.x >> 8;
+        uint32_t before = 0;
+        for (int b0 = q; b0 < me; b0 += 32) {
+            uint32_t v[8];
 #pragma unroll
-    for (int r = 0; r < RS_ROWS; ++r) {
-        int64_t i = base + r * 64 + lane;
-        bool in = i < n;
-        uint32_t key = in ? kin[i] : 0u;
-        kk[r] = key;
-        vv[r] = in ? vin[i] : 0u;
-        uint32_t d = (key >> shift) & 0xFFu;
-        uint64_t m = __ballot(in);
+            for (int u = 0; u < 8; ++u) {
+                const int b = b0 + 4 * u;
+                v[u] = b < me ? __hip_atomic_load(&status[(int64_t)b * 256 + d], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : RS_READY;
+            }
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bb = __ballot(bit);
-            m &= bit ? bb : ~bb;
+            for (int u = 0; u < 8; ++u) {
+                const int b = b0 + 4 * u;
+                int spins = 0;
+                while (!(v[u] & RS_READY) && spins < RS_SPIN_LIMIT) {
+                    __builtin_amdgcn_s_sleep(1);
+                    v[u] = __hip_atomic_load(&status[(int64_t)b * 256 + d], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+                    ++spins;
+                }
+                if (v[u] & RS_READY) before += v[u] & ~RS_READY;
+                else fault = 1;
+            }
         }
-        uint32_t rank = __popcll(m & lt), cnt = __popcll(m);
-        uint32_t prior = wcnt[w][d];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        if (in && rank == cnt - 1) wcnt[w][d] = prior + cnt;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        lo[r] = prior + rank;
+        pbef[q][d] = before;
     }
     __syncthreads();
     if (threadIdx.x < 256) {
         const int d = threadIdx.x;
-        uint32_t a = 0;
-#pragma unroll
-        for (int ww = 0; ww < RS_WAVES; ++ww) {
-            uint32_t c = wcnt[ww][d];
-            wcnt[ww][d] = a;
-            a += c;
-        }
+        pbef[0][d] += rs_digit_scan(tot[d], lane, w, wtot) + pbef[1][d] + pbef[2][d] + pbef[3][d];
     }
     __syncthreads();
-#pragma unroll
-    for (int r = 0; r < RS_ROWS; ++r) {
-        int64_t i = base + r * 64 + lane;
-        if (i < n) {
-            uint32_t d = (kk[r] >> shift) & 0xFFu;
-            uint32_t pos = gbase[d] + wcnt[w][d] + lo[r];
-            kout[pos] = kk[r];
-            vout[pos] = vv[r];
-        }
-    }
+    if (threadIdx.x < 256) gbase[threadIdx.x] = rs_digit_add(wtot, w) + pbef[0][threadIdx.x];
+    __syncthreads();
+    rs_scatter_tile(kout, vout, n, shift, base, lane, w, wcnt, gbase, kk, vv, lo);
     if (hclear)
-        for (int64_t k = (int64_t)blockIdx.x * RS_THREADS + threadIdx.x; k < hwords;
-             k += (int64_t)gridDim.x * RS_THREADS)
+        for (int64_t k = (int64_t)me * RS_THREADS + threadIdx.x; k < hwords; k += (int64_t)gridDim.x * RS_THREADS)
             hclear[k] = 0;
+    if (threadIdx.x == 0 && fault) st->sort_fault = 1;
 }
 
 // ------------------------------------------------------------------------------------
@@ -1317,24 +1429,29 @@ int64_t radix_hist_entries(int64_t n) { return gvom_plan::radix_hist_words(n); }
 int64_t segment_block_words(int64_t n) { return gvom_plan::segment_block_words(n); }
 
 hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits,
-                      uint32_t *hist, int64_t hwords, uint32_t **kout, uint32_t **vout, hipStream_t s) {
+                      uint32_t *hist, int64_t hwords, DevState *st, uint32_t **kout, uint32_t **vout, hipStream_t s) {
     *kout = k0;
     *vout = v0;
     if (n <= 0) return hipSuccess;
     if (bits > 32) return hipErrorInvalidValue;
     if (gvom_plan::radix_hist_words(n) > hwords) return hipErrorInvalidValue;   // hist too small for n keys
-    int nblk = (int)nblocks(n, RS_TILE);
+    const int nblk = (int)nblocks(n, RS_TILE);
+    const int passes = (bits + 7) / 8;
+    // set p >= 1 of hist: pass p's published tile counts (256 per tile), then its digit totals
+    auto status = [&](int p) { return hist + (int64_t)p * hwords; };
+    auto totals = [&](int p) { return hist + (int64_t)p * hwords + (int64_t)256 * nblk; };
     uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
-    for (int shift = 0, p = 0; shift < bits; shift += 8, ++p) {
-        // pass 0 reads the first histogram (from launch_prepare), later passes the second
-        uint32_t *hp = p == 0 ? hist : hist + hwords;
-        const bool last = shift + 8 >= bits;
-        if (p > 0) k_rs_hist<<<nblk, RS_THREADS, 0, s>>>(ka, n, shift, hp);
-        k_rs_scatter<<<nblk, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, hp, nblk, last && p > 0 ? hist : nullptr,
-                                                  hwords);
-        if (last && p == 0) {
-            hipError_t e = launch_fill_i32((int32_t *)hist, 0, hwords, s);
-            if (e != hipSuccess) return e;
+    for (int p = 0; p < passes; ++p) {
+        uint32_t *tn = p + 1 < passes ? totals(p + 1) : nullptr;
+        if (p == 0) {
+            k_rs_scatter<<<nblk, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, 0, hist, nblk, tn);
+            if (passes == 1) {
+                hipError_t e = launch_fill_i32((int32_t *)hist, 0, hwords, s);
+                if (e != hipSuccess) return e;
+            }
+        } else {
+            k_rs_scatter_lb<<<nblk, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, 8 * p, status(p), totals(p), tn, nblk,
+                                                         p + 1 == passes ? hist : nullptr, hwords, st);
         }
         uint32_t *tk = ka; ka = kb; kb = tk;
         uint32_t *tv = va; va = vb; vb = tv;

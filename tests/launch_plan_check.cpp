@@ -71,6 +71,10 @@ int main() {
         CHECK(P::segment_block_words(n) >= (nb + 1) * (2 + 3 * P::MAX_BATCH), "segments n %lld", n);
         const int64_t tiles = n > 0 ? (n + P::RS_TILE - 1) / P::RS_TILE : 1;
         CHECK(P::radix_hist_words(n) >= 256 * tiles + 256, "radix n %lld", n);
+        // four sets: the first pass's tile histogram, then tile counts + digit totals of up
+        // to three lookback passes (32-bit keys)
+        CHECK(P::radix_hist_alloc_words(n) == P::RADIX_HIST_SETS * P::radix_hist_words(n) && P::RADIX_HIST_SETS >= 4,
+              "radix sets n %lld", n);
     }
     // the launchers refuse undersized buffers before any device work (null pointers: nothing is
     // dereferenced on a refused launch)
@@ -91,8 +95,8 @@ int main() {
         CHECK(launch_segments(nullptr, n, nullptr, nullptr, nullptr, blk, 33, g, nullptr, nullptr) ==
                   hipErrorInvalidValue, "segments with 33 scans");
         uint32_t *kout = nullptr, *vout = nullptr;
-        CHECK(radix_sort(nullptr, nullptr, nullptr, nullptr, n, 32, nullptr, P::radix_hist_words(n) - 1, &kout, &vout,
-                         nullptr) == hipErrorInvalidValue, "radix histogram one word short");
+        CHECK(radix_sort(nullptr, nullptr, nullptr, nullptr, n, 32, nullptr, P::radix_hist_words(n) - 1, nullptr, &kout,
+                         &vout, nullptr) == hipErrorInvalidValue, "radix histogram one word short");
         const int64_t tw = P::tile_words(g.XY, g.Z);
         CHECK(launch_raycast(GVOM_F32, nullptr, nullptr, nullptr, 2, n, g, nullptr, nullptr, 2 * tw - 1, 256, 1, 0, 1,
                              -1, nullptr, 1, 0, 0, nullptr) == hipErrorInvalidValue, "tiled cast into a short scratch");
