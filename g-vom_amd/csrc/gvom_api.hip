@@ -800,10 +800,10 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     CKH(h, hipEventRecord(h->ev_fork, fs));
     CKH(h, hipStreamWaitEvent(h->side, h->ev_fork, 0));
     uint32_t *sk = W.k0, *sv = W.v0;
-    CKH(h, radix_sort(W.k0, W.v0, W.k1, W.v1, NT, g.VB + sbits, W.hist, W.hwords, h->st, &sk, &sv, h->side));
+    CKH(h, radix_sort(W.k0, W.v0, W.k1, W.v1, NT, g.VB + sbits, g.VB, W.hist, W.hwords, h->st, &sk, &sv, h->side));
     h->sorted_vals = sv;
-    CKH(h, launch_segments(sk, NT, W.seg_start, W.seg_key, W.blk, W.blk_words, S, g, h->st, h->side));
-    CKH(h, launch_mom_part(dtype, W.world, sk, sv, NT, sd, S, g, h->st, W.part, h->side));
+    CKH(h, launch_segments(dtype, W.world, sk, sv, NT, sd, S, g, h->st, W.hist, W.hwords, W.blk, W.blk_words,
+                           W.seg_start, W.seg_key, W.part, h->side));
     CKH(h, hipEventRecord(h->ev_join, h->side));
     // the tiled path's conversion writes every word of the slot tables; the direct cast counts
     // into cleared tables, cleared here, beside the sort chain (nothing before the cast reads them)

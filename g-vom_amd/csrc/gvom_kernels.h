@@ -35,21 +35,24 @@ hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, co
 // later passes' published tile counts and digit totals, zeroed by launch_prepare.  A tile
 // that gives up waiting sets st->sort_fault.  On return *kout/*vout point at whichever
 // buffer pair holds the result.
-hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits,
+// scan_shift: keys >> scan_shift is the scan of the batch (the first pass counts each scan's
+// valid keys into set 1).
+hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits, int scan_shift,
                       uint32_t *hist, int64_t hwords, DevState *st, uint32_t **kout, uint32_t **vout,
                       hipStream_t s);
 int64_t radix_hist_entries(int64_t n);
 // words of the segment kernels' per-block counts (launch_segments' blk) for n sorted keys
 int64_t segment_block_words(int64_t n);
-// Segments of equal keys in the sorted array, per-scan bounds and slot cell counts.
-hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start, uint32_t *seg_key,
-                           uint32_t *blk, int64_t blk_words, int S, const Geo &g, DevState *st, hipStream_t s);
-// Per-segment moments (chunk partials, then per-segment sums) + slot table occupancy
-// marks + hit/total/min-height records.  All scan-stage writes to slot tables and records go
-// to the buffers new_buf(flip, slot) of the slots the batch fills.
-hipError_t launch_mom_part(int dtype, const void *world, const uint32_t *skeys, const uint32_t *svals,
-                           int64_t seg_cap, const ScanDesc *sd, int S, const Geo &g, DevState *st, Moment *part,
+// Segments of equal keys in the sorted array, per-scan bounds and slot cell counts, and the
+// per-chunk moment partials, in one launch; hist / hwords as given to radix_sort (set 1
+// holds the scans' valid-key counts and the blocks' published head counts).
+hipError_t launch_segments(int dtype, const void *world, const uint32_t *skeys, const uint32_t *svals, int64_t n,
+                           const ScanDesc *sd, int S, const Geo &g, DevState *st, uint32_t *hist, int64_t hwords,
+                           uint32_t *blk, int64_t blk_words, int32_t *seg_start, uint32_t *seg_key, Moment *part,
                            hipStream_t s);
+// Per-segment moment sums of the chunk partials + slot table occupancy marks +
+// hit/total/min-height records.  All scan-stage writes to slot tables and records go to the
+// buffers new_buf(flip, slot) of the slots the batch fills.
 hipError_t launch_mom_seg(const int32_t *seg_start, const uint32_t *seg_key, int64_t seg_cap, const Geo &g,
                           DevState *st, int32_t *slot_tables, SlotRec *slot_recs, int64_t ncap, const Moment *part,
                           Moment *mom, hipStream_t s);

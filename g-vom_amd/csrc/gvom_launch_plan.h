@@ -21,11 +21,16 @@ constexpr int64_t blocks(int64_t n, int64_t per) {
     return (n + per - 1) / per > 0 ? (n + per - 1) / per : 1;
 }
 
-// ---- radix sort: RADIX_HIST_SETS sets of 256 words per tile + 256: the first pass's per-tile
-// histogram, then per later pass (at most three: keys have <= 32 bits) the tiles' published
-// digit counts and the pass's digit totals
+// ---- radix sort and segments: RADIX_HIST_SETS sets of radix_hist_words(n) words.  Set 0:
+// the first pass's per-tile histogram.  Set p >= 1: the tiles' published digit counts of
+// pass p (at most three later passes: keys have <= 32 bits), the pass's digit totals, then
+// 64 words (set 1: valid keys per scan, counted by the first pass) and one word per
+// segment block (set 1: the segment kernel's published head counts).
 constexpr int RADIX_HIST_SETS = 4;
-constexpr int64_t radix_hist_words(int64_t n) { return 256 * blocks(n, RS_TILE) + 256; }
+constexpr int64_t radix_totals_at(int64_t n) { return 256 * blocks(n, RS_TILE); }
+constexpr int64_t radix_scan_counts_at(int64_t n) { return radix_totals_at(n) + 256; }
+constexpr int64_t segment_flags_at(int64_t n) { return radix_scan_counts_at(n) + 64; }
+constexpr int64_t radix_hist_words(int64_t n) { return segment_flags_at(n) + blocks(n, SEG_TILE); }
 constexpr int64_t radix_hist_alloc_words(int64_t n) { return RADIX_HIST_SETS * radix_hist_words(n); }
 
 // ---- segment kernels: per block of SEG_TILE keys, W = 2 + 3 S counters (heads, valid keys,

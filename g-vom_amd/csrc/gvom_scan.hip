@@ -524,7 +524,7 @@ __global__ __launch_bounds__(256) void k_raycast(const P *__restrict__ world,
 
 // ------------------------------------------------------------------------------------
 // Radix sort: 8-bit digits, tiles of RS_TILE keys (defined with k_prepare).
-// Points per moment chunk (one wavefront, see k_mom_part).
+// Points per moment chunk (one wavefront, see mom_chunk).
 constexpr int MCH = 64;
 
 // Tiles of RS_TILE keys, RS_THREADS threads (2 keys each): more wavefronts per tile keep
@@ -623,18 +623,22 @@ __device__ __forceinline__ void rs_scatter_tile(uint32_t *__restrict__ kout, uin
 
 // First pass.  hist: its per-tile digit counts (block-major, from k_prepare).  tot_next
 // (zero on entry, or null in a one-pass sort): the next digit's totals, accumulated here.
+// scan_cnt (zero on entry): valid keys (!= GVOM_NOKEY) per scan, the scan being key >> VB.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *__restrict__ kin,
                                                            const uint32_t *__restrict__ vin,
                                                            uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                            int64_t n, int shift, const uint32_t *__restrict__ hist,
-                                                           int nblk, uint32_t *__restrict__ tot_next) {
+                                                           int nblk, uint32_t *__restrict__ tot_next, int VB,
+                                                           uint32_t *__restrict__ scan_cnt) {
     __shared__ uint32_t wcnt[RS_WAVES][256];
     __shared__ uint32_t ptot[4][256], pbef[4][256];
     __shared__ uint32_t gbase[256], nh[256];
     __shared__ uint32_t wtot[4];
+    __shared__ uint32_t scn[GVOM_MAX_BATCH];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int k = threadIdx.x; k < RS_WAVES * 256; k += RS_THREADS) (&wcnt[0][0])[k] = 0;
     if (threadIdx.x < 256) nh[threadIdx.x] = 0;
+    if (threadIdx.x < GVOM_MAX_BATCH) scn[threadIdx.x] = 0;
     {
         // global offset of digit d for this block: keys of smaller digits in all blocks plus
         // keys of digit d in earlier blocks.  Four threads per digit each add every fourth
@@ -662,11 +666,27 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *__res
     const int64_t base = (int64_t)blockIdx.x * RS_TILE + w * (RS_ROWS * 64);
     uint32_t kk[RS_ROWS], vv[RS_ROWS], lo[RS_ROWS];
     rs_rank_tile(kin, vin, n, shift, base, lane, w, wcnt, tot_next ? nh : nullptr, kk, vv, lo);
+    // valid keys per scan: one LDS add per scan present in a row (rows are runs of consecutive
+    // points, so usually of one scan)
+#pragma unroll
+    for (int r = 0; r < RS_ROWS; ++r) {
+        const bool valid = base + r * 64 + lane < n && kk[r] != GVOM_NOKEY;
+        const uint32_t sc = valid ? kk[r] >> VB : 0u;
+        uint64_t rem = __ballot(valid);
+        while (rem) {
+            const int leader = __builtin_ctzll(rem);
+            const uint32_t s0 = __shfl(sc, leader);
+            const uint64_t m = __ballot(valid && sc == s0);
+            if (lane == leader) atomicAdd(&scn[s0], (uint32_t)__popcll(m));
+            rem &= ~m;
+        }
+    }
     __syncthreads();
     if (threadIdx.x < 256) {
         rs_wave_offsets(wcnt, threadIdx.x);
         if (tot_next && nh[threadIdx.x]) atomicAdd(&tot_next[threadIdx.x], nh[threadIdx.x]);
     }
+    if (threadIdx.x < GVOM_MAX_BATCH && scn[threadIdx.x]) atomicAdd(&scan_cnt[threadIdx.x], scn[threadIdx.x]);
     __syncthreads();
     rs_scatter_tile(kout, vout, n, shift, base, lane, w, wcnt, gbase, kk, vv, lo);
 }
@@ -755,50 +775,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter_lb(const uint32_t *__
 
 // ------------------------------------------------------------------------------------
 // Segments of the sorted key array.
-__device__ __forceinline__ bool is_head(const uint32_t *__restrict__ k, int64_t i) {
-    uint32_t key = k[i];
-    return key != GVOM_NOKEY && (i == 0 || k[i - 1] != key);
-}
 
 // Segment kernels: blocks of 256 threads x 8 sorted keys, independent of the sort's tile.
 constexpr int SEG_TILE = gvom_plan::SEG_TILE;   // 256 threads x 8 keys
 static_assert(SEG_TILE == 256 * 8, "segment kernels: 256 threads x 8 keys per block");
-
-// Per block of SEG_TILE sorted keys: [heads, valid keys, then per scan s (in-map heads,
-// halo heads, valid keys)], W = 2 + 3 * S words.
-__global__ __launch_bounds__(256) void k_seg_count(const uint32_t *__restrict__ k, int64_t n, int S, Geo g,
-                                                   uint32_t *__restrict__ blk) {
-    __shared__ uint32_t c[2 + 3 * GVOM_MAX_BATCH];
-    const int W = 2 + 3 * S;
-    if (threadIdx.x < W) c[threadIdx.x] = 0;
-    __syncthreads();
-    int64_t base = (int64_t)blockIdx.x * SEG_TILE;
-    uint32_t heads = 0, valid = 0;
-    for (int j = 0; j < 8; ++j) {
-        int64_t i = base + threadIdx.x * 8 + j;
-        if (i >= n) break;
-        const uint32_t key = k[i];
-        if (key == GVOM_NOKEY) continue;
-        ++valid;
-        const int sc = (int)(key >> g.VB);
-        if (S > 1) atomicAdd(&c[4 + 3 * sc], 1u);
-        if (i == 0 || k[i - 1] != key) {
-            ++heads;
-            const int halo = (int)((key >> (g.VB - 1)) & 1u);
-            atomicAdd(&c[2 + 3 * sc + halo], 1u);
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        heads += __shfl_down(heads, off);
-        valid += __shfl_down(valid, off);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&c[0], heads);
-        atomicAdd(&c[1], valid);
-    }
-    __syncthreads();
-    if (threadIdx.x < W) blk[(int64_t)blockIdx.x * W + threadIdx.x] = c[threadIdx.x];
-}
 
 // Totals of the per-block counts -> nseg, nvalid, end sentinel of the last segment,
 // per-scan [first segment, end of in-map segments) and slot cell counts, by one block
@@ -837,54 +817,6 @@ __device__ void seg_bounds_body(const uint32_t *__restrict__ blk, int nblk, int 
     st->chunk_off[S] = ch;
 }
 
-// Segment heads in sorted order; the last block then also derives the totals (the per-block
-// counts were complete before this kernel started).
-__global__ __launch_bounds__(256) void k_seg_compact(const uint32_t *__restrict__ k, int64_t n,
-                                                     const uint32_t *__restrict__ blk, int S, Geo g,
-                                                     int32_t *__restrict__ seg_start,
-                                                     uint32_t *__restrict__ seg_key, DevState *__restrict__ st) {
-    const int W = 2 + 3 * S;
-    __shared__ uint32_t wsum[4];
-    __shared__ uint32_t wpre[4];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    // heads in earlier blocks
-    uint32_t pre = 0;
-    for (int b = t; b < (int)blockIdx.x; b += 256) pre += blk[(int64_t)b * W];
-    for (int off = 32; off > 0; off >>= 1) pre += __shfl_down(pre, off);
-    if (lane == 0) wpre[w] = pre;
-    int64_t base = (int64_t)blockIdx.x * SEG_TILE + t * 8;
-    uint32_t flags = 0, cnt = 0;
-    for (int j = 0; j < 8; ++j) {
-        int64_t i = base + j;
-        if (i < n && is_head(k, i)) {
-            flags |= 1u << j;
-            ++cnt;
-        }
-    }
-    uint32_t v = cnt;
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t u = __shfl_up(v, off);
-        if (lane >= off) v += u;
-    }
-    if (lane == 63) wsum[w] = v;
-    __syncthreads();
-    uint32_t r = wpre[0] + wpre[1] + wpre[2] + wpre[3];
-    for (int ww = 0; ww < w; ++ww) r += wsum[ww];
-    r += v - cnt;
-    for (int j = 0; j < 8; ++j) {
-        if (flags & (1u << j)) {
-            int64_t i = base + j;
-            seg_start[r] = (int32_t)i;
-            seg_key[r] = k[i];
-            ++r;
-        }
-    }
-    if (blockIdx.x == gridDim.x - 1) {
-        __syncthreads();
-        seg_bounds_body(blk, (int)gridDim.x, S, g, seg_start, st);
-    }
-}
-
 __global__ __launch_bounds__(64) void k_seg_bounds(const uint32_t *__restrict__ blk, int nblk, int S, Geo g,
                                                    int32_t *__restrict__ seg_start, DevState *__restrict__ st) {
     seg_bounds_body(blk, nblk, S, g, seg_start, st);
@@ -892,7 +824,7 @@ __global__ __launch_bounds__(64) void k_seg_bounds(const uint32_t *__restrict__ 
 
 // ------------------------------------------------------------------------------------
 // Per-segment moments about the segment's own voxel corner, in two deterministic steps:
-//   k_mom_part: one wavefront per chunk of MCH sorted points (a lane per point) reduces
+//   k_segments: one wavefront per chunk of MCH sorted points (a lane per point) reduces
 //               every run of equal keys inside the chunk and writes its partial Moment at
 //               the run's last position;
 //   k_mom_seg : one thread per segment adds the partials of the chunks it spans, in
@@ -941,22 +873,38 @@ __device__ __forceinline__ void store_moment(Moment *m, const MomAcc &a) {
     m->pad = __float_as_int(a.mh);
 }
 
+// Chunk partials of one wavefront: chunk t of the scan-relative chunking (each scan's
+// valid keys, from its first sorted position, in chunks of MCH), a lane per point.  A
+// segmented inclusive scan (fixed shuffle tree) leaves each run's sum in its last lane,
+// which stores it at that point's sorted position.  scan_cnt: valid keys per scan (the
+// sorted keys are ordered by scan), so chunks are cut relative to each scan's first sorted
+// position and a scan's sums do not depend on the other scans of its batch.
 template <typename P>
-__global__ __launch_bounds__(256) void k_mom_part(const P *__restrict__ world, const uint32_t *__restrict__ skeys,
-                                                  const uint32_t *__restrict__ svals,
-                                                  const ScanDesc *__restrict__ sd, int S, Geo g,
-                                                  const DevState *__restrict__ st, Moment *__restrict__ part) {
-    // One wavefront per chunk of MCH = 64 sorted points, one lane per point.  Chunks are cut
-    // relative to each scan's first sorted position, so a scan's sums do not depend on the
-    // other scans of its batch.  A segmented inclusive scan (fixed shuffle tree) leaves each
-    // run's sum in its last lane, which stores it at that point's sorted position.
+__device__ __forceinline__ void mom_chunk(int32_t t, const P *__restrict__ world, const uint32_t *__restrict__ skeys,
+                                          const uint32_t *__restrict__ svals, const ScanDesc *__restrict__ sd, int S,
+                                          const Geo &g, const uint32_t *__restrict__ scan_cnt,
+                                          Moment *__restrict__ part) {
     const int lane = threadIdx.x & 63;
-    const int32_t t = (int32_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    if (t >= st->chunk_off[S]) return;   // uniform across the wavefront
-    int sc = 0;
-    while (sc + 1 < S && st->chunk_off[sc + 1] <= t) ++sc;
-    const int64_t sbeg = st->scan_pos[sc], send = sbeg + st->scan_valid[sc];
-    const int64_t j = sbeg + (int64_t)(t - st->chunk_off[sc]) * MCH + lane;
+    // lane s < S: scan s's valid keys, first sorted position and first chunk
+    const int32_t cnt_s = lane < S ? (int32_t)scan_cnt[lane] : 0;
+    const int32_t nch_s = (cnt_s + MCH - 1) / MCH;
+    int32_t pos_s = cnt_s, off_s = nch_s;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t up = __shfl_up(pos_s, o), uc = __shfl_up(off_s, o);
+        if (lane >= o) {
+            pos_s += up;
+            off_s += uc;
+        }
+    }
+    pos_s -= cnt_s;
+    off_s -= nch_s;
+    const int32_t total = __shfl(off_s + nch_s, S - 1);
+    if (t >= total) return;   // uniform across the wavefront
+    // the chunk's scan: the last one whose first chunk is <= t
+    const uint64_t le = __ballot(lane < S && off_s <= t);
+    const int sc = 63 - __builtin_clzll(le);
+    const int64_t sbeg = __shfl(pos_s, sc), send = sbeg + __shfl(cnt_s, sc);
+    const int64_t j = sbeg + (int64_t)(t - __shfl(off_s, sc)) * MCH + lane;
     const bool in = j < send;
     const uint32_t key = in ? skeys[j] : GVOM_NOKEY;
     const uint32_t prev = __shfl_up(key, 1);
@@ -1012,6 +960,111 @@ __global__ __launch_bounds__(256) void k_mom_part(const P *__restrict__ world, c
         a.mh = mh;
         store_moment(part + j, a);
     }
+}
+
+// Segments of the sorted keys and the moment chunk partials, in one launch of 256-thread
+// blocks.  Every wavefront writes the partials of one chunk (mom_chunk).  Blocks b < nbs
+// also own sorted keys [b SEG_TILE, (b + 1) SEG_TILE): they count heads, valid keys and per
+// scan s in-map heads, halo heads and valid keys (W = 2 + 3 S words into blk[b]), publish
+// the head count (flags[b] = heads | RS_READY, zero on entry), and after their chunk add the
+// head counts of the blocks before (waiting for those not yet published; a block waits only
+// on lower ones, as in the radix sort) to write their segment starts and keys in sorted
+// order.  The last of them then waits for every block and derives the totals.
+template <typename P>
+__global__ __launch_bounds__(256) void k_segments(const uint32_t *__restrict__ k, const uint32_t *__restrict__ svals,
+                                                  int64_t n, int nbs, const P *__restrict__ world,
+                                                  const ScanDesc *__restrict__ sd, int S, Geo g,
+                                                  DevState *__restrict__ st, const uint32_t *__restrict__ scan_cnt,
+                                                  uint32_t *__restrict__ flags, uint32_t *__restrict__ blk,
+                                                  int32_t *__restrict__ seg_start, uint32_t *__restrict__ seg_key,
+                                                  Moment *__restrict__ part) {
+    __shared__ uint32_t c[2 + 3 * GVOM_MAX_BATCH];
+    __shared__ uint32_t wsum[4], wpre[4];
+    __shared__ int fault;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int b = (int)blockIdx.x;
+    const bool segb = b < nbs;   // uniform
+    const int W = 2 + 3 * S;
+    const int64_t base = (int64_t)b * SEG_TILE + t * 8;
+    uint32_t hflags = 0, cnt = 0;
+    if (segb) {
+        if (t < W) c[t] = 0;
+        if (t == 0) fault = 0;
+        __syncthreads();
+        uint32_t valid = 0;
+        for (int jj = 0; jj < 8; ++jj) {
+            const int64_t i = base + jj;
+            if (i >= n) break;
+            const uint32_t key = k[i];
+            if (key == GVOM_NOKEY) continue;
+            ++valid;
+            const int sc = (int)(key >> g.VB);
+            if (S > 1) atomicAdd(&c[4 + 3 * sc], 1u);
+            if (i == 0 || k[i - 1] != key) {
+                hflags |= 1u << jj;
+                ++cnt;
+                atomicAdd(&c[2 + 3 * sc + (int)((key >> (g.VB - 1)) & 1u)], 1u);
+            }
+        }
+        uint32_t hs = cnt, vs = valid;
+        for (int off = 32; off > 0; off >>= 1) {
+            hs += __shfl_down(hs, off);
+            vs += __shfl_down(vs, off);
+        }
+        if (lane == 0) {
+            atomicAdd(&c[0], hs);
+            atomicAdd(&c[1], vs);
+        }
+        __syncthreads();
+        if (t == 0) {
+            for (int q = 0; q < W; ++q) blk[(int64_t)b * W + q] = c[q];
+            __hip_atomic_store(&flags[b], c[0] | RS_READY, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    mom_chunk<P>((int32_t)(b * 4 + w), world, k, svals, sd, S, g, scan_cnt, part);
+    if (!segb) return;
+    // heads in the blocks before this one
+    uint32_t pre = 0;
+    for (int bb = t; bb < b; bb += 256) {
+        uint32_t v = __hip_atomic_load(&flags[bb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int spins = 0;
+        while (!(v & RS_READY) && spins < RS_SPIN_LIMIT) {
+            __builtin_amdgcn_s_sleep(1);
+            v = __hip_atomic_load(&flags[bb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ++spins;
+        }
+        if (v & RS_READY) pre += v & ~RS_READY;
+        else fault = 1;
+    }
+    for (int off = 32; off > 0; off >>= 1) pre += __shfl_down(pre, off);
+    if (lane == 0) wpre[w] = pre;
+    uint32_t v = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t u = __shfl_up(v, off);
+        if (lane >= off) v += u;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    uint32_t r = wpre[0] + wpre[1] + wpre[2] + wpre[3];
+    for (int ww = 0; ww < w; ++ww) r += wsum[ww];
+    r += v - cnt;
+    for (int jj = 0; jj < 8; ++jj) {
+        if (hflags & (1u << jj)) {
+            const int64_t i = base + jj;
+            seg_start[r] = (int32_t)i;
+            seg_key[r] = k[i];
+            ++r;
+        }
+    }
+    if (b == nbs - 1) {
+        // every block's counts were published before the head counts this block waited for
+        // (release / acquire): the totals
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        seg_bounds_body(blk, nbs, S, g, seg_start, st);
+    }
+    if (t == 0 && fault) st->sort_fault = 1;
 }
 
 __global__ __launch_bounds__(256) void k_mom_seg(const int32_t *__restrict__ seg_start,
@@ -1428,23 +1481,24 @@ int64_t radix_hist_entries(int64_t n) { return gvom_plan::radix_hist_words(n); }
 
 int64_t segment_block_words(int64_t n) { return gvom_plan::segment_block_words(n); }
 
-hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits,
+hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n, int bits, int scan_shift,
                       uint32_t *hist, int64_t hwords, DevState *st, uint32_t **kout, uint32_t **vout, hipStream_t s) {
     *kout = k0;
     *vout = v0;
     if (n <= 0) return hipSuccess;
-    if (bits > 32) return hipErrorInvalidValue;
+    if (bits > 32 || scan_shift < 1 || scan_shift > 31 || (bits - scan_shift) > 5) return hipErrorInvalidValue;
     if (gvom_plan::radix_hist_words(n) > hwords) return hipErrorInvalidValue;   // hist too small for n keys
     const int nblk = (int)nblocks(n, RS_TILE);
     const int passes = (bits + 7) / 8;
     // set p >= 1 of hist: pass p's published tile counts (256 per tile), then its digit totals
     auto status = [&](int p) { return hist + (int64_t)p * hwords; };
-    auto totals = [&](int p) { return hist + (int64_t)p * hwords + (int64_t)256 * nblk; };
+    auto totals = [&](int p) { return hist + (int64_t)p * hwords + gvom_plan::radix_totals_at(n); };
     uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
     for (int p = 0; p < passes; ++p) {
         uint32_t *tn = p + 1 < passes ? totals(p + 1) : nullptr;
         if (p == 0) {
-            k_rs_scatter<<<nblk, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, 0, hist, nblk, tn);
+            k_rs_scatter<<<nblk, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, 0, hist, nblk, tn, scan_shift,
+                                                      hist + hwords + gvom_plan::radix_scan_counts_at(n));
             if (passes == 1) {
                 hipError_t e = launch_fill_i32((int32_t *)hist, 0, hwords, s);
                 if (e != hipSuccess) return e;
@@ -1461,30 +1515,29 @@ hipError_t radix_sort(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, in
     return hipGetLastError();
 }
 
-hipError_t launch_segments(const uint32_t *skeys, int64_t n, int32_t *seg_start, uint32_t *seg_key,
-                           uint32_t *blk, int64_t blk_words, int S, const Geo &g, DevState *st, hipStream_t s) {
+hipError_t launch_segments(int dtype, const void *world, const uint32_t *skeys, const uint32_t *svals, int64_t n,
+                           const ScanDesc *sd, int S, const Geo &g, DevState *st, uint32_t *hist, int64_t hwords,
+                           uint32_t *blk, int64_t blk_words, int32_t *seg_start, uint32_t *seg_key, Moment *part,
+                           hipStream_t s) {
     if (S < 1 || S > GVOM_MAX_BATCH || gvom_plan::segment_block_footprint(n, S) > blk_words)
         return hipErrorInvalidValue;   // blk too small for n keys of S scans
-    int nblk = n > 0 ? (int)nblocks(n, SEG_TILE) : 0;
-    if (n > 0) {
-        k_seg_count<<<nblk, 256, 0, s>>>(skeys, n, S, g, blk);
-        k_seg_compact<<<nblk, 256, 0, s>>>(skeys, n, blk, S, g, seg_start, seg_key, st);
-    } else {
-        k_seg_bounds<<<1, 64, 0, s>>>(blk, nblk, S, g, seg_start, st);
+    if (n <= 0) {
+        k_seg_bounds<<<1, 64, 0, s>>>(blk, 0, S, g, seg_start, st);
+        return hipGetLastError();
     }
-    return hipGetLastError();
-}
-
-// Chunk partials (k_mom_part) need only the sorted keys and segments; the segment sums
-// (k_mom_seg) also read the pass counts the ray cast left in the slot tables.
-hipError_t launch_mom_part(int dtype, const void *world, const uint32_t *skeys, const uint32_t *svals,
-                           int64_t seg_cap, const ScanDesc *sd, int S, const Geo &g, DevState *st, Moment *part,
-                           hipStream_t s) {
-    unsigned nbp = nblocks(((seg_cap + MCH - 1) / MCH + S) * 64, 256);
+    if (gvom_plan::radix_hist_words(n) > hwords) return hipErrorInvalidValue;
+    const int nbs = (int)nblocks(n, SEG_TILE);
+    // one wavefront per chunk: at most ceil(n / MCH) + S scan-relative chunks, 4 per block
+    const int64_t nchunk = (n + MCH - 1) / MCH + S;
+    const int64_t nb = std::max<int64_t>(nbs, (nchunk + 3) / 4);
+    const uint32_t *scan_cnt = hist + hwords + gvom_plan::radix_scan_counts_at(n);
+    uint32_t *flags = hist + hwords + gvom_plan::segment_flags_at(n);
     if (dtype == GVOM_F32)
-        k_mom_part<float><<<nbp, 256, 0, s>>>((const float *)world, skeys, svals, sd, S, g, st, part);
+        k_segments<float><<<(unsigned)nb, 256, 0, s>>>(skeys, svals, n, nbs, (const float *)world, sd, S, g, st,
+                                                       scan_cnt, flags, blk, seg_start, seg_key, part);
     else
-        k_mom_part<double><<<nbp, 256, 0, s>>>((const double *)world, skeys, svals, sd, S, g, st, part);
+        k_segments<double><<<(unsigned)nb, 256, 0, s>>>(skeys, svals, n, nbs, (const double *)world, sd, S, g, st,
+                                                        scan_cnt, flags, blk, seg_start, seg_key, part);
     return hipGetLastError();
 }
 

@@ -90,13 +90,21 @@ int main() {
         g.VB = 24;
         const int64_t n = 131072;
         const int64_t blk = P::segment_block_footprint(n, 3);
-        CHECK(launch_segments(nullptr, n, nullptr, nullptr, nullptr, blk - 1, 3, g, nullptr, nullptr) ==
-                  hipErrorInvalidValue, "segments one word short");
-        CHECK(launch_segments(nullptr, n, nullptr, nullptr, nullptr, blk, 33, g, nullptr, nullptr) ==
-                  hipErrorInvalidValue, "segments with 33 scans");
+        const int64_t hw = P::radix_hist_words(n);
+        CHECK(launch_segments(GVOM_F32, nullptr, nullptr, nullptr, n, nullptr, 3, g, nullptr, nullptr, hw, nullptr,
+                              blk - 1, nullptr, nullptr, nullptr, nullptr) == hipErrorInvalidValue,
+              "segments one word short");
+        CHECK(launch_segments(GVOM_F32, nullptr, nullptr, nullptr, n, nullptr, 33, g, nullptr, nullptr, hw, nullptr, blk,
+                              nullptr, nullptr, nullptr, nullptr) == hipErrorInvalidValue,
+              "segments with 33 scans");
+        CHECK(launch_segments(GVOM_F32, nullptr, nullptr, nullptr, n, nullptr, 3, g, nullptr, nullptr, hw - 1, nullptr,
+                              blk, nullptr, nullptr, nullptr, nullptr) == hipErrorInvalidValue,
+              "segments with a histogram set one word short");
         uint32_t *kout = nullptr, *vout = nullptr;
-        CHECK(radix_sort(nullptr, nullptr, nullptr, nullptr, n, 32, nullptr, P::radix_hist_words(n) - 1, nullptr, &kout,
-                         &vout, nullptr) == hipErrorInvalidValue, "radix histogram one word short");
+        CHECK(radix_sort(nullptr, nullptr, nullptr, nullptr, n, 32, 27, nullptr, P::radix_hist_words(n) - 1, nullptr,
+                         &kout, &vout, nullptr) == hipErrorInvalidValue, "radix histogram one word short");
+        CHECK(radix_sort(nullptr, nullptr, nullptr, nullptr, n, 32, 24, nullptr, P::radix_hist_words(n), nullptr, &kout,
+                         &vout, nullptr) == hipErrorInvalidValue, "radix sort with more than 32 scans");
         const int64_t tw = P::tile_words(g.XY, g.Z);
         CHECK(launch_raycast(GVOM_F32, nullptr, nullptr, nullptr, 2, n, g, nullptr, nullptr, 2 * tw - 1, 256, 1, 0, 1,
                              -1, nullptr, 1, 0, 0, nullptr) == hipErrorInvalidValue, "tiled cast into a short scratch");
