@@ -1119,8 +1119,8 @@ extern "C" int gvom_sync(gvom_handle *h) {
 static int read_state(gvom_handle *h, DevState *out) {
     CKH(h, hipMemcpyAsync(out, h->st, sizeof(DevState), hipMemcpyDeviceToHost, h->stream));
     CKH(h, hipStreamSynchronize(h->stream));
-    if (out->sort_fault) {
-        h->err = "internal: a radix-sort tile gave up waiting for the counts of an earlier tile";
+    if (out->wait_fault) {
+        h->err = "internal: a block gave up waiting for the counts of an earlier block";
         return GVOM_ESTATE;
     }
     return GVOM_OK;

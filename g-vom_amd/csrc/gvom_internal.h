@@ -68,14 +68,35 @@ struct DevState {
     // whether the slot holds a scan; written with the ring indices at the end of every scan
     // stage for the combine that follows
     alignas(16) int32_t comb_shift[GVOM_MAX_BUFFER][4];
-    // set when a radix-sort tile gave up waiting for an earlier tile's counts (a defect; the
-    // state getters then report an error)
-    int32_t sort_fault;
+    // set when a block gave up waiting for counts an earlier block publishes (PUB_READY
+    // protocol; a defect): the state getters then report an error
+    int32_t wait_fault;
 };
 
 // Physical buffer of ring slot `slot`: the one holding its scan, and the one the scan stage in
 // flight writes.
 __host__ __device__ __forceinline__ int cur_buf(int32_t flip, int slot) { return 2 * slot + ((flip >> slot) & 1); }
+
+// Counts a block publishes for the later blocks of the same launch (radix-sort passes,
+// segments, cell ids): count | PUB_READY in a word that is zero before the launch.  A block
+// publishes before it waits and waits only for lower blocks, which were dispatched before it
+// or are dispatched as other work retires, so every wait ends; the poll bound (about 30 ms
+// with s_sleep 1) only guards against a defect: the waiter then goes on without the count
+// and flags the fault.
+constexpr uint32_t PUB_READY = 0x80000000u;
+constexpr int PUB_SPIN_LIMIT = 1 << 20;
+
+// The count published at p; fault set (and 0 returned) when the bound is reached.
+__device__ __forceinline__ uint32_t pub_wait(const uint32_t *p, int &fault) {
+    uint32_t v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int spins = 0; !(v & PUB_READY) && spins < PUB_SPIN_LIMIT; ++spins) {
+        __builtin_amdgcn_s_sleep(1);
+        v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (v & PUB_READY) return v & ~PUB_READY;
+    fault = 1;
+    return 0;
+}
 __host__ __device__ __forceinline__ int new_buf(int32_t flip, int slot) {
     return 2 * slot + 1 - ((flip >> slot) & 1);
 }

@@ -33,7 +33,7 @@ hipError_t launch_raycast(int dtype, const void *world, const uint8_t *valid, co
 // kernel per pass.  hist holds RADIX_HIST_SETS x hwords words: the first pass's per-tile
 // counts, filled by launch_prepare (the last pass clears them for the next sort), and the
 // later passes' published tile counts and digit totals, zeroed by launch_prepare.  A tile
-// that gives up waiting sets st->sort_fault.  On return *kout/*vout point at whichever
+// that gives up waiting sets st->wait_fault.  On return *kout/*vout point at whichever
 // buffer pair holds the result.
 // scan_shift: keys >> scan_shift is the scan of the batch (the first pass counts each scan's
 // valid keys into set 1).

@@ -533,12 +533,8 @@ constexpr int RS_THREADS = 1024;
 constexpr int RS_WAVES = RS_THREADS / 64;
 constexpr int RS_ROWS = RS_TILE / RS_THREADS;   // 64-key rows per wavefront
 
-// Published tile counts of the lookback passes: count | RS_READY once the tile has ranked.
-constexpr uint32_t RS_READY = 0x80000000u;
-// Polls of an unpublished count before a tile gives up (about 30 ms with s_sleep 1): every
-// tile publishes before it waits, and tiles wait only on lower ones, so only a defect can
-// reach the bound; the tile then uses what it has (positions stay in [0, n)) and flags it.
-constexpr int RS_SPIN_LIMIT = 1 << 20;
+// The lookback passes publish their tiles' digit counts (PUB_READY protocol); a tile that
+// gives up waiting uses what it has, so its positions stay in [0, n).
 
 // Ranks the tile's keys by the digit at `shift`, stably: rows in key order, waves in order.
 // wcnt[w][d] ends as the count of digit d in wavefront w; lo = rank within the wavefront.
@@ -692,7 +688,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t *__res
 }
 
 // Later passes, without a histogram kernel: each tile ranks its keys, publishes its digit
-// counts (count | RS_READY in status[tile][d], zero on entry), then adds the counts of the
+// counts (count | PUB_READY in status[tile][d], zero on entry), then adds the counts of the
 // tiles before it, waiting for those not yet published; `tot` (this digit's totals,
 // accumulated by the previous pass) gives the digits' starts.  A tile waits only on lower
 // tiles, which were dispatched before it or are dispatched as other work retires, so every
@@ -724,7 +720,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter_lb(const uint32_t *__
     __syncthreads();
     if (threadIdx.x < 256) {
         const uint32_t c = rs_wave_offsets(wcnt, threadIdx.x);
-        __hip_atomic_store(&status[(int64_t)me * 256 + threadIdx.x], c | RS_READY, __ATOMIC_RELAXED,
+        __hip_atomic_store(&status[(int64_t)me * 256 + threadIdx.x], c | PUB_READY, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
         if (tot_next && nh[threadIdx.x]) atomicAdd(&tot_next[threadIdx.x], nh[threadIdx.x]);
     }
@@ -740,19 +736,19 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter_lb(const uint32_t *__
                 const int b = b0 + 4 * u;
                 v[u] = b < me ? __hip_atomic_load(&status[(int64_t)b * 256 + d], __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT)
-                              : RS_READY;
+                              : PUB_READY;
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int b = b0 + 4 * u;
                 int spins = 0;
-                while (!(v[u] & RS_READY) && spins < RS_SPIN_LIMIT) {
+                while (!(v[u] & PUB_READY) && spins < PUB_SPIN_LIMIT) {
                     __builtin_amdgcn_s_sleep(1);
                     v[u] = __hip_atomic_load(&status[(int64_t)b * 256 + d], __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
                     ++spins;
                 }
-                if (v[u] & RS_READY) before += v[u] & ~RS_READY;
+                if (v[u] & PUB_READY) before += v[u] & ~PUB_READY;
                 else fault = 1;
             }
         }
@@ -770,7 +766,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter_lb(const uint32_t *__
     if (hclear)
         for (int64_t k = (int64_t)me * RS_THREADS + threadIdx.x; k < hwords; k += (int64_t)gridDim.x * RS_THREADS)
             hclear[k] = 0;
-    if (threadIdx.x == 0 && fault) st->sort_fault = 1;
+    if (threadIdx.x == 0 && fault) st->wait_fault = 1;
 }
 
 // ------------------------------------------------------------------------------------
@@ -966,7 +962,7 @@ __device__ __forceinline__ void mom_chunk(int32_t t, const P *__restrict__ world
 // blocks.  Every wavefront writes the partials of one chunk (mom_chunk).  Blocks b < nbs
 // also own sorted keys [b SEG_TILE, (b + 1) SEG_TILE): they count heads, valid keys and per
 // scan s in-map heads, halo heads and valid keys (W = 2 + 3 S words into blk[b]), publish
-// the head count (flags[b] = heads | RS_READY, zero on entry), and after their chunk add the
+// the head count (flags[b] = heads | PUB_READY, zero on entry), and after their chunk add the
 // head counts of the blocks before (waiting for those not yet published; a block waits only
 // on lower ones, as in the radix sort) to write their segment starts and keys in sorted
 // order.  The last of them then waits for every block and derives the totals.
@@ -1018,24 +1014,14 @@ __global__ __launch_bounds__(256) void k_segments(const uint32_t *__restrict__ k
         __syncthreads();
         if (t == 0) {
             for (int q = 0; q < W; ++q) blk[(int64_t)b * W + q] = c[q];
-            __hip_atomic_store(&flags[b], c[0] | RS_READY, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&flags[b], c[0] | PUB_READY, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     mom_chunk<P>((int32_t)(b * 4 + w), world, k, svals, sd, S, g, scan_cnt, part);
     if (!segb) return;
     // heads in the blocks before this one
     uint32_t pre = 0;
-    for (int bb = t; bb < b; bb += 256) {
-        uint32_t v = __hip_atomic_load(&flags[bb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int spins = 0;
-        while (!(v & RS_READY) && spins < RS_SPIN_LIMIT) {
-            __builtin_amdgcn_s_sleep(1);
-            v = __hip_atomic_load(&flags[bb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ++spins;
-        }
-        if (v & RS_READY) pre += v & ~RS_READY;
-        else fault = 1;
-    }
+    for (int bb = t; bb < b; bb += 256) pre += pub_wait(&flags[bb], fault);
     for (int off = 32; off > 0; off >>= 1) pre += __shfl_down(pre, off);
     if (lane == 0) wpre[w] = pre;
     uint32_t v = cnt;
@@ -1064,7 +1050,7 @@ __global__ __launch_bounds__(256) void k_segments(const uint32_t *__restrict__ k
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         seg_bounds_body(blk, nbs, S, g, seg_start, st);
     }
-    if (t == 0 && fault) st->sort_fault = 1;
+    if (t == 0 && fault) st->wait_fault = 1;
 }
 
 __global__ __launch_bounds__(256) void k_mom_seg(const int32_t *__restrict__ seg_start,
