@@ -200,14 +200,21 @@ __global__ __launch_bounds__(256) void k_comb_tables(Geo g, const DevState *__re
     }
 }
 
-// Same combine for xy_size % 4 == 0, with less scalar work per column.  The 4 columns of a
-// group lie in one x row; each source's shift (precomputed by the scan stage's commit) gives
+// Same combine for xy_size % 8 == 0, with less scalar work per column.  The RCPW columns of
+// a group lie in one x row; each source's shift (precomputed by the scan stage's commit) gives
 // one uniform column pointer, and a word that is off the map (column or z) is read from a
 // zero page instead of being masked after the load, so no per-word predicate outlives its
-// address.  Each wavefront handles GPW groups.  (The general kernel above issues ~1200
-// scalar instructions per wavefront, which made it scalar-issue bound.)
-constexpr int GPW = 1;   // measured: 1 group per wavefront 34.4 us, 2 groups 35.3 us, 4 groups 36.8 us
-constexpr int ZERO_PAGE_WORDS = 4 * 256 + 64;   // covers lane + j * z_size for z_size <= 256
+// address.  A source's shift and column base are scalar work done once per group, so a wider
+// group spends less of it per column: on 256 x 256 x 64 with 8 slots, 4 columns per wavefront
+// took 29.4 us, 8 columns with 4 sources loaded together 28.6 us (8 sources together: 129
+// VGPRs, 3 wavefronts per SIMD; 16 columns spill ~190 SGPRs).  Each wavefront handles GPW
+// groups.  (The general kernel above issues ~1200 scalar instructions per wavefront for 4
+// columns, which made it scalar-issue bound.)
+constexpr int RCPW = 8;
+constexpr int RSRC_GROUP = 4;
+constexpr int RCPW_MAX = 8;
+constexpr int GPW = 1;   // measured with 4 columns: 1 group per wavefront 34.4 us, 2 groups 35.3 us, 4 groups 36.8 us
+constexpr int ZERO_PAGE_WORDS = RCPW_MAX * 256 + 64;   // covers lane + j * z_size for z_size <= 256
 
 // combine rule of one voxel: union of the sources' occupancy and sum of their passes, then
 // the previous map (gvom.py:1010-1063)
@@ -236,7 +243,7 @@ __global__ __launch_bounds__(256) void k_comb_rows(Geo g, const DevState *__rest
                                                    double *__restrict__ inferred) {
     const int lane = threadIdx.x & 63;
     const int32_t XY = g.XY, Z = g.Z, X = (int32_t)g.X, V = (int32_t)g.V;
-    const int32_t cw = uni((int32_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (CPW * GPW));
+    const int32_t cw = uni((int32_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (RCPW * GPW));
     if (cw >= X) return;
     const int parity = uni(st->parity), cur = 1 - parity;
     const int have_last = uni(st->have_last);
@@ -248,30 +255,30 @@ __global__ __launch_bounds__(256) void k_comb_rows(Geo g, const DevState *__rest
     const int32_t *ltab = comb_tables + (int64_t)parity * V;
     const int32_t *zp = zero_page + lane;
     const int NW = (Z + 63) >> 6;
-    uint32_t count[GPW][CPW], ffree[GPW][CPW];
+    uint32_t count[GPW][RCPW], ffree[GPW][RCPW];
 #pragma unroll
     for (int q = 0; q < GPW; ++q)
 #pragma unroll
-        for (int j = 0; j < CPW; ++j) {
+        for (int j = 0; j < RCPW; ++j) {
             count[q][j] = 0;
             ffree[q][j] = 0xFFFFFFFFu;
         }
     for (int z0 = 0; z0 < Z; z0 += 64) {
         const int z = z0 + lane;
         const bool zin = z < Z;
-        uint32_t mask[GPW][CPW], P[GPW][CPW];
+        uint32_t mask[GPW][RCPW], P[GPW][RCPW];
 #pragma unroll
         for (int q = 0; q < GPW; ++q)
 #pragma unroll
-            for (int j = 0; j < CPW; ++j) mask[q][j] = P[q][j] = 0;
-        for (int s0 = 0; s0 < g.B; s0 += SRC_GROUP) {
+            for (int j = 0; j < RCPW; ++j) mask[q][j] = P[q][j] = 0;
+        for (int s0 = 0; s0 < g.B; s0 += RSRC_GROUP) {
 #pragma unroll
             for (int q = 0; q < GPW; ++q) {
-                const int32_t c0 = cw + q * CPW;
+                const int32_t c0 = cw + q * RCPW;
                 const int32_t x0 = c0 / XY, y0 = c0 - (c0 / XY) * XY;
-                int32_t v[SRC_GROUP][CPW];
+                int32_t v[RSRC_GROUP][RCPW];
 #pragma unroll
-                for (int k = 0; k < SRC_GROUP; ++k) {
+                for (int k = 0; k < RSRC_GROUP; ++k) {
                     // shifts precomputed by the scan stage's commit (one 16-byte scalar load)
                     const int s = s0 + k < g.B ? s0 + k : 0;
                     const int4 sh = *reinterpret_cast<const int4 *>(st->comb_shift[s]);
@@ -285,16 +292,16 @@ __global__ __launch_bounds__(256) void k_comb_rows(Geo g, const DevState *__rest
                     const int32_t *cp = slot_tables + (int64_t)cur_buf(flip, s) * V +
                                         (int32_t)(((uint32_t)ux * (uint32_t)XY + (uint32_t)uy0) * (uint32_t)Z);
 #pragma unroll
-                    for (int j = 0; j < CPW; ++j) {
+                    for (int j = 0; j < RCPW; ++j) {
                         const bool cok = rowok & ((uint32_t)(uy0 + j) < (uint32_t)XY);   // uniform
                         const int32_t *a = (cok & zok) ? cp + j * Z + zs : zp + j * Z;
                         v[k][j] = *a;
                     }
                 }
 #pragma unroll
-                for (int k = 0; k < SRC_GROUP; ++k)
+                for (int k = 0; k < RSRC_GROUP; ++k)
 #pragma unroll
-                    for (int j = 0; j < CPW; ++j) {
+                    for (int j = 0; j < RCPW; ++j) {
                         if (v[k][j] < 0) mask[q][j] |= 1u << (s0 + k);
                         else P[q][j] += (uint32_t)v[k][j];
                     }
@@ -304,22 +311,22 @@ __global__ __launch_bounds__(256) void k_comb_rows(Geo g, const DevState *__rest
         const bool zokl = (have_last != 0) & zin & ((uint32_t)zl < (uint32_t)Z);
 #pragma unroll
         for (int q = 0; q < GPW; ++q) {
-            const int32_t c0 = cw + q * CPW;
+            const int32_t c0 = cw + q * RCPW;
             if (c0 >= X) break;
             const int32_t x0 = c0 / XY, y0 = c0 - (c0 / XY) * XY;
             const int32_t ux = x0 + ldx, uy0 = y0 + ldy;
             const bool rowok = (uint32_t)ux < (uint32_t)XY;
             const int32_t *cp = ltab + (int32_t)(((uint32_t)ux * (uint32_t)XY + (uint32_t)uy0) * (uint32_t)Z);
-            int32_t vl[CPW];
-            bool okl[CPW];
+            int32_t vl[RCPW];
+            bool okl[RCPW];
 #pragma unroll
-            for (int j = 0; j < CPW; ++j) {
+            for (int j = 0; j < RCPW; ++j) {
                 okl[j] = rowok & zokl & ((uint32_t)(uy0 + j) < (uint32_t)XY);
                 vl[j] = *(okl[j] ? cp + j * Z + zl : zp + j * Z);
             }
             int32_t *out = comb_tables + (int64_t)cur * V + (int64_t)c0 * Z;
 #pragma unroll
-            for (int j = 0; j < CPW; ++j) {
+            for (int j = 0; j < RCPW; ++j) {
                 bool occ, freev;
                 const int32_t o = comb_word(mask[q][j], P[q][j], okl[j], vl[j], occ, freev);
                 if (zin) out[j * Z + z] = o;
@@ -332,15 +339,15 @@ __global__ __launch_bounds__(256) void k_comb_rows(Geo g, const DevState *__rest
             }
         }
     }
-    if (lane < GPW * CPW) {
+    if (lane < GPW * RCPW) {
         const int32_t c = cw + lane;
         if (c < X) {
             uint32_t cnt = 0, ff = 0xFFFFFFFFu;
 #pragma unroll
             for (int q = 0; q < GPW; ++q)
 #pragma unroll
-                for (int j = 0; j < CPW; ++j)
-                    if (lane == q * CPW + j) {
+                for (int j = 0; j < RCPW; ++j)
+                    if (lane == q * RCPW + j) {
                         cnt = count[q][j];
                         ff = ffree[q][j];
                     }
@@ -936,8 +943,9 @@ int64_t zero_page_words() { return ZERO_PAGE_WORDS; }
 hipError_t launch_comb_tables(const Geo &g, DevState *st, const int32_t *slot_tables, const int32_t *zero_page,
                               int32_t *comb_tables, int32_t *col_count, uint64_t *occ_bits, double *height,
                               double *inferred, hipStream_t s) {
-    if (g.XY % CPW == 0 && zero_page)
-        k_comb_rows<<<nblocks((g.X + CPW * GPW - 1) / (CPW * GPW), 4), 256, 0, s>>>(g, st, slot_tables, zero_page,
+    static_assert(RCPW <= RCPW_MAX, "the zero page covers RCPW_MAX columns");
+    if (g.XY % RCPW == 0 && zero_page)
+        k_comb_rows<<<nblocks((g.X + RCPW * GPW - 1) / (RCPW * GPW), 4), 256, 0, s>>>(g, st, slot_tables, zero_page,
                                                                                  comb_tables, col_count, occ_bits,
                                                                                  height, inferred);
     else
