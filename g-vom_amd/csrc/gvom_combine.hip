@@ -649,7 +649,7 @@ __global__ __launch_bounds__(MT) void k_maps2d(Geo g, const DevState *__restrict
                                                 double *__restrict__ slope_x, double *__restrict__ slope_y,
                                                 double *__restrict__ rough, double *__restrict__ delta,
                                                 int32_t *__restrict__ pos, int32_t *__restrict__ neg,
-                                                int32_t *__restrict__ vis) {
+                                                int32_t *__restrict__ vis, uint8_t *__restrict__ host_maps) {
     __shared__ double H[LT][LT];
     __shared__ uint64_t rowm[LT];   // bit ly set when H[lx][ly] is observed (rows: fixed x)
     __shared__ uint64_t colm[LT];   // bit lx set when H[lx][ly] is observed (columns: fixed y)
@@ -859,9 +859,19 @@ __global__ __launch_bounds__(MT) void k_maps2d(Geo g, const DevState *__restrict
     const int32_t pv = (sqrt(sx * sx + sy * sy) >= g.slope_thr) ? 100 : winpv[cell];
     // __make_negative_obstacle_map (506-512); __make_visability_map (445-452)
     if (act) {
+        const int32_t nv = (dh_out > g.neg_thr) ? 100 : 0, vv = (h0 > -1000) ? 1 : 0;
         pos[c0] = pv;
-        neg[c0] = (dh_out > g.neg_thr) ? 100 : 0;
-        vis[c0] = (h0 > -1000) ? 1 : 0;
+        neg[c0] = nv;
+        vis[c0] = vv;
+        if (host_maps) {
+            // the same four maps, [pos | neg | vis | rough], into pinned host memory
+            const int64_t X = g.X;
+            int32_t *hm = reinterpret_cast<int32_t *>(host_maps);
+            hm[c0] = pv;
+            hm[X + c0] = nv;
+            hm[2 * X + c0] = vv;
+            reinterpret_cast<double *>(hm + 3 * X)[c0] = rg;
+        }
     }
 }
 
@@ -980,10 +990,10 @@ hipError_t launch_comb_records(const Geo &g, DevState *st, const int32_t *slot_t
 hipError_t launch_maps2d(const Geo &g, const DevState *st, const double *height, const double *inferred,
                          const int32_t *comb_tables, const CombRec *comb_recs, int64_t ccap, double *slope_x,
                          double *slope_y, double *rough, double *delta, int32_t *pos, int32_t *neg, int32_t *vis,
-                         hipStream_t s) {
+                         uint8_t *host_maps, hipStream_t s) {
     dim3 grid((unsigned)((g.XY + TILE - 1) / TILE), (unsigned)((g.XY + TILE - 1) / TILE));
     k_maps2d<<<grid, MT, 0, s>>>(g, st, height, inferred, comb_tables, comb_recs, ccap, slope_x, slope_y, rough,
-                                  delta, pos, neg, vis);
+                                  delta, pos, neg, vis, host_maps);
     return hipGetLastError();
 }
 

@@ -86,7 +86,7 @@ hipError_t launch_comb_records(const Geo &g, DevState *st, const int32_t *slot_t
 hipError_t launch_maps2d(const Geo &g, const DevState *st, const double *height, const double *inferred,
                          const int32_t *comb_tables, const CombRec *comb_recs, int64_t ccap, double *slope_x,
                          double *slope_y, double *rough, double *delta, int32_t *pos, int32_t *neg, int32_t *vis,
-                         hipStream_t s);
+                         uint8_t *host_maps, hipStream_t s);
 
 
 // ---- export (gvom_combine.hip) -----------------------------------------------------------
