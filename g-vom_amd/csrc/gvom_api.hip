@@ -198,16 +198,16 @@ struct gvom_handle {
     double dg_wh = 0.0, dk_wh = 0.0;
     // A host-array scan call (gvom_process_pointcloud with host points) that repeats the
     // previous one's point count and dtype replays a captured launch sequence: one launch
-    // instead of a dozen, and no gaps between the kernels.  The sequence reads its points and
-    // its descriptor from hp_in, which one transfer from the pinned staging buffer fills.  It
-    // is replayed only under the buffer generation and launch settings (cfg_gen, bumped by
+    // instead of a dozen, no gaps between the kernels, and no transfer: the kernels read the
+    // points and the scan's descriptor from the pinned staging buffer itself (one sequence per
+    // buffer).  A sequence is replayed only under the buffer generation, the staging buffers
+    // (pin_gen, bumped when one is reallocated) and the launch settings (cfg_gen, bumped by
     // every setting that changes what a call launches) it was captured under.
-    hipGraph_t hp_graph = nullptr;
-    hipGraphExec_t hp_exec = nullptr;
-    uint8_t *hp_in = nullptr;
-    size_t hp_in_bytes = 0;
-    int64_t hp_n = -1, hp_gen = -1, hp_cfg = -1;
+    hipGraph_t hp_graph[2] = {nullptr, nullptr};
+    hipGraphExec_t hp_exec[2] = {nullptr, nullptr};
+    int64_t hp_n = -1, hp_gen = -1, hp_cfg = -1, hp_pin_gen = -1;
     int hp_dtype = -1;
+    int64_t pin_gen = 0;
     int64_t hk_n = -1;   // the previous host-array scan call's point count and dtype
     int hk_dtype = -1;
     int64_t cfg_gen = 0;
@@ -271,9 +271,10 @@ static void free_all(gvom_handle *h) {
     if (h->dg_exec) (void)hipGraphExecDestroy(h->dg_exec);
     if (h->dg_graph) (void)hipGraphDestroy(h->dg_graph);
     if (h->dg_sd) (void)hipFree(h->dg_sd);
-    if (h->hp_exec) (void)hipGraphExecDestroy(h->hp_exec);
-    if (h->hp_graph) (void)hipGraphDestroy(h->hp_graph);
-    if (h->hp_in) (void)hipFree(h->hp_in);
+    for (int k = 0; k < 2; ++k) {
+        if (h->hp_exec[k]) (void)hipGraphExecDestroy(h->hp_exec[k]);
+        if (h->hp_graph[k]) (void)hipGraphDestroy(h->hp_graph[k]);
+    }
     for (auto &e : h->ev_pool)
         if (e) (void)hipEventDestroy(e);
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
@@ -795,6 +796,7 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
                 if (h->pin_pts[k]) CKH(h, hipHostFree(h->pin_pts[k]));
                 h->pin_pts[k] = nullptr;
                 h->pin_pts_bytes[k] = 0;
+                ++h->pin_gen;
                 CKH(h, hipHostMalloc(&h->pin_pts[k], bytes, hipHostMallocDefault));
                 h->pin_pts_bytes[k] = bytes;
             }
@@ -873,33 +875,42 @@ static void drop_graph(hipGraphExec_t &exec, hipGraph_t &graph) {
     graph = nullptr;
 }
 
-// Where a replayed scan's descriptor follows its points in hp_in and in the pinned staging buffer.
+// Where a replayed scan's descriptor follows its points in the pinned staging buffer.
 static size_t host_desc_offset(int64_t n, int dtype) {
     const size_t bytes = (size_t)n * 3 * (dtype == GVOM_F32 ? 4 : 8);
     return (bytes + 255) & ~(size_t)255;
 }
 
-// Captures the scan stage of one host-array scan of n points into h->hp_exec, reading the
-// points from hp_in and the descriptor from hp_in + host_desc_offset(n, dtype).  Nothing runs
-// and the host mirror is left as it was.  The previous call had the same count and dtype, so
-// every work buffer already holds n points; a capture that reallocated a buffer anyway, or
-// failed, keeps no sequence and the call runs the eager path.
+// Captures the scan stage of one host-array scan of n points twice, into h->hp_exec[k] for
+// k = 0, 1: sequence k reads the points from pinned staging buffer k and the descriptor from
+// the same buffer at host_desc_offset(n, dtype).  Nothing runs and the host mirror is left as
+// it was.  The previous call had the same count and dtype, so every work buffer already holds
+// n points; a capture that reallocated a buffer anyway, or failed, keeps no sequence and the
+// call runs the eager path.
 static void capture_host_process(gvom_handle *h, int64_t n, int dtype, const double ego[3], const void *T,
                                  int t_dtype) {
-    drop_graph(h->hp_exec, h->hp_graph);
+    for (int k = 0; k < 2; ++k) drop_graph(h->hp_exec[k], h->hp_graph[k]);
     h->hp_n = -1;
     const size_t off = host_desc_offset(n, dtype), need = off + sizeof(ScanDesc);
-    if (hipStreamSynchronize(h->stream) != hipSuccess) return;   // an earlier replay may read hp_in
-    if (h->hp_in_bytes < need) {
-        if (h->hp_in) (void)hipFree(h->hp_in);
-        h->hp_in = nullptr;
-        h->hp_in_bytes = 0;
-        if (hipMalloc((void **)&h->hp_in, need) != hipSuccess) {
-            h->hp_in = nullptr;
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return;   // nothing in flight reads the staging buffers
+    void *dev[2] = {nullptr, nullptr};
+    for (int k = 0; k < 2; ++k) {
+        if (h->pin_pts_bytes[k] < need) {
+            if (h->pin_pts[k]) (void)hipHostFree(h->pin_pts[k]);
+            h->pin_pts[k] = nullptr;
+            h->pin_pts_bytes[k] = 0;
+            ++h->pin_gen;
+            if (hipHostMalloc(&h->pin_pts[k], need, hipHostMallocDefault) != hipSuccess) {
+                h->pin_pts[k] = nullptr;
+                (void)hipGetLastError();
+                return;
+            }
+            h->pin_pts_bytes[k] = need;
+        }
+        if (hipHostGetDevicePointer(&dev[k], h->pin_pts[k], 0) != hipSuccess || !dev[k]) {
             (void)hipGetLastError();
             return;
         }
-        h->hp_in_bytes = need;
     }
     const int m_next = h->next_slot, m_last = h->last_slot, m_filled = h->filled_mask;
     double m_origin[GVOM_MAX_BUFFER][3], m_ego[3];
@@ -907,46 +918,57 @@ static void capture_host_process(gvom_handle *h, int64_t n, int dtype, const dou
     std::memcpy(m_ego, h->latest_ego, sizeof(m_ego));
     const std::string m_err = h->err;
     const int64_t gen = h->buf_gen;
-    if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        (void)hipGetLastError();
+    bool ok = true;
+    for (int k = 0; k < 2 && ok; ++k) {
+        if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            ok = false;
+            break;
+        }
+        h->capturing = true;
+        h->cap_sd_dev = reinterpret_cast<ScanDesc *>((uint8_t *)dev[k] + off);
+        int64_t offs[2] = {0, n};
+        const int rc = process_impl(h, dev[k], offs, 1, dtype, ego, T, t_dtype, true, false);
+        hipGraph_t graph = nullptr;
+        const hipError_t ee = hipStreamEndCapture(h->stream, &graph);
+        h->capturing = false;
+        h->cap_sd_dev = nullptr;
+        (void)mark_back(h);   // ev_back was recorded inside the capture
+        h->next_slot = m_next;
+        h->last_slot = m_last;
+        h->filled_mask = m_filled;
+        std::memcpy(h->slot_origin, m_origin, sizeof(m_origin));
+        std::memcpy(h->latest_ego, m_ego, sizeof(m_ego));
+        h->err = m_err;
+        hipGraphExec_t exec = nullptr;
+        if (rc || ee != hipSuccess || !graph || h->buf_gen != gen ||
+            hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+            ok = false;
+            break;
+        }
+        h->hp_graph[k] = graph;
+        h->hp_exec[k] = exec;
+    }
+    if (!ok) {
+        for (int k = 0; k < 2; ++k) drop_graph(h->hp_exec[k], h->hp_graph[k]);
         return;
     }
-    h->capturing = true;
-    h->cap_sd_dev = reinterpret_cast<ScanDesc *>(h->hp_in + off);
-    int64_t offs[2] = {0, n};
-    const int rc = process_impl(h, h->hp_in, offs, 1, dtype, ego, T, t_dtype, true, false);
-    hipGraph_t graph = nullptr;
-    const hipError_t ee = hipStreamEndCapture(h->stream, &graph);
-    h->capturing = false;
-    h->cap_sd_dev = nullptr;
-    (void)mark_back(h);   // ev_back was recorded inside the capture
-    h->next_slot = m_next;
-    h->last_slot = m_last;
-    h->filled_mask = m_filled;
-    std::memcpy(h->slot_origin, m_origin, sizeof(m_origin));
-    std::memcpy(h->latest_ego, m_ego, sizeof(m_ego));
-    h->err = m_err;
-    hipGraphExec_t exec = nullptr;
-    if (rc || ee != hipSuccess || !graph || h->buf_gen != gen ||
-        hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        (void)hipGetLastError();
-        return;
-    }
-    h->hp_graph = graph;
-    h->hp_exec = exec;
     h->hp_n = n;
     h->hp_dtype = dtype;
     h->hp_gen = gen;
     h->hp_cfg = h->cfg_gen;
+    h->hp_pin_gen = h->pin_gen;
 }
 
 constexpr int NOT_REPLAYED = 100;
 
-// A host-array scan call through the captured scan stage: the points and the descriptor go
-// through one pinned staging buffer (the two buffers alternate, so the call waits only for the
-// transfer two calls back) in one transfer, then one graph launch.  Returns NOT_REPLAYED when
-// the call is not taken; the eager path then runs it and reports any error in its arguments.
+// A host-array scan call through a captured scan stage: the points and the descriptor are
+// written into one of the two pinned staging buffers (used in turn, so the call waits only for
+// the sequence two calls back, the last reader of that buffer) and the sequence bound to that
+// buffer is launched.  Returns NOT_REPLAYED when the call is not taken; the eager path then
+// runs it and reports any error in its arguments.
 static int process_host_replay(gvom_handle *h, const void *xyz, int64_t n, int dtype, const double ego[3],
                                const void *T, int t_dtype) {
     const Geo &g = h->g;
@@ -958,8 +980,8 @@ static int process_host_replay(gvom_handle *h, const void *xyz, int64_t n, int d
         if (!std::isfinite(ego[k]) || std::fabs(ego[k] / res) >= 4194304.0) return NOT_REPLAYED;
     }
     auto valid = [&]() {
-        return h->hp_exec && h->hp_n == n && h->hp_dtype == dtype && h->hp_gen == h->buf_gen &&
-               h->hp_cfg == h->cfg_gen;
+        return h->hp_exec[0] && h->hp_exec[1] && h->hp_n == n && h->hp_dtype == dtype && h->hp_gen == h->buf_gen &&
+               h->hp_cfg == h->cfg_gen && h->hp_pin_gen == h->pin_gen;
     };
     if (!valid()) {
         // captured on the second call in a row with this count and dtype
@@ -979,23 +1001,15 @@ static int process_host_replay(gvom_handle *h, const void *xyz, int64_t n, int d
     d.offset = 0;
     d.count = n;
     const size_t pbytes = (size_t)n * 3 * (dtype == GVOM_F32 ? 4 : 8);
-    const size_t off = host_desc_offset(n, dtype), bytes = off + sizeof(ScanDesc);
+    const size_t off = host_desc_offset(n, dtype);
     const int k = h->pin_next;
     CKH(h, hipEventSynchronize(h->ev_pin[k]));
-    if (h->pin_pts_bytes[k] < bytes) {
-        if (h->pin_pts[k]) CKH(h, hipHostFree(h->pin_pts[k]));
-        h->pin_pts[k] = nullptr;
-        h->pin_pts_bytes[k] = 0;
-        CKH(h, hipHostMalloc(&h->pin_pts[k], bytes, hipHostMallocDefault));
-        h->pin_pts_bytes[k] = bytes;
-    }
     std::memcpy(h->pin_pts[k], xyz, pbytes);
     std::memcpy((uint8_t *)h->pin_pts[k] + off, &d, sizeof(d));
-    CKH(h, hipMemcpyAsync(h->hp_in, h->pin_pts[k], bytes, hipMemcpyHostToDevice, h->stream));
+    h->diff_n = -1;   // the sequence reuses the work buffers a pending gvom_diff_backward reads
+    CKH(h, hipGraphLaunch(h->hp_exec[k], h->stream));
     CKH(h, hipEventRecord(h->ev_pin[k], h->stream));
     h->pin_next = 1 - k;
-    h->diff_n = -1;   // the sequence reuses the work buffers a pending gvom_diff_backward reads
-    CKH(h, hipGraphLaunch(h->hp_exec, h->stream));
     CKH(h, mark_back(h));
     // host mirror of the ring, as the eager call leaves it
     const int slot = h->next_slot;
