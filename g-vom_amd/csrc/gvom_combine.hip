@@ -17,7 +17,9 @@
 //   maps2d    16x16 column tiles with a 15-cell halo in LDS: slope and roughness
 //             (718-805), guessed height delta (593-714), positive/negative obstacle
 //             and visibility maps (445-555).
-//   commit    ping-pong parity of the combined buffers.
+//   commit    ping-pong parity of the combined buffers, flipped by the column scan's
+//             first thread (every later kernel of the combine reads the new parity), and
+//             the new map's cell count, written by the record merge.
 #include <hip/hip_runtime.h>
 
 #include "gvom_kernels.h"
@@ -381,12 +383,17 @@ __global__ __launch_bounds__(1024) void k_col_scan(int32_t *__restrict__ cnt, in
     __shared__ int32_t wsum[16];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (blockIdx.x == 0 && t == 0) {
-        // origin of the combined map being built = newest slot's origin (gvom.py:229)
+        // origin of the combined map being built = newest slot's origin (gvom.py:229), and
+        // the ping-pong flip (gvom.py:276-316): no other thread of this kernel reads the
+        // parity, the combine's earlier kernel (the table pass) has finished, and its later
+        // kernels read the new value
         const int cur = 1 - st->parity;
         const double *co = st->slot_origin[st->last_slot];
         st->comb_origin[cur][0] = co[0];
         st->comb_origin[cur][1] = co[1];
         st->comb_origin[cur][2] = co[2];
+        st->parity = cur;
+        st->have_last = 1;
     }
     const int64_t i = (int64_t)blockIdx.x * COLSCAN + 4 * t;
     // four counts per thread; past the last column they read as 0 and are not written
@@ -500,7 +507,7 @@ __global__ __launch_bounds__(256) void k_comb_ids(Geo g, const DevState *__restr
     const int32_t X = (int32_t)g.X, Z = g.Z;
     const int32_t cb = uni((int32_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64);
     if (cb >= X) return;
-    const int cur = 1 - st->parity;
+    const int cur = st->parity;   // flipped by the column scan
     const int NW = (Z + 63) >> 6;
     int32_t *ctab = comb_tables + (int64_t)cur * g.V;
     // earlier chunks' totals (a wavefront's 64 columns lie in one chunk: 64 divides COLSCAN)
@@ -537,7 +544,7 @@ __global__ __launch_bounds__(256) void k_comb_ids(Geo g, const DevState *__restr
 // (min_h + z + oz) * z_res (gvom.py:572-576).
 constexpr int MERGE_GROUP = 4;
 
-__global__ __launch_bounds__(256) void k_comb_merge(Geo g, const DevState *__restrict__ st,
+__global__ __launch_bounds__(256) void k_comb_merge(Geo g, DevState *__restrict__ st,
                                                     const int32_t *__restrict__ slot_tables,
                                                     const SlotRec *__restrict__ slot_recs, int64_t ncap,
                                                     const int32_t *__restrict__ comb_tables,
@@ -545,12 +552,17 @@ __global__ __launch_bounds__(256) void k_comb_merge(Geo g, const DevState *__res
                                                     const uint2 *__restrict__ occ_list,
                                                     const int32_t *__restrict__ chunk_tot, int64_t nchunk,
                                                     double *__restrict__ height) {
-    const int parity = st->parity, cur = 1 - parity, flip = st->flip;
+    const int cur = st->parity, parity = 1 - cur, flip = st->flip;   // flipped by the column scan
     const double *co = st->comb_origin[cur];
     const int lane = threadIdx.x & 63;
     uint32_t C = 0;
     for (int64_t k = lane; k < nchunk; k += 64) C += (uint32_t)chunk_tot[k];
     for (int off = 32; off > 0; off >>= 1) C += __shfl_xor(C, off);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the new map's cell count (the commit of gvom.py:276-316)
+        st->comb_cells[cur] = (int32_t)C;
+        st->comb_total = (int32_t)C;
+    }
     if ((int64_t)C > ccap) C = (uint32_t)ccap;   // the compact list and records hold ccap cells
     CombRec *crec = comb_recs + (int64_t)cur * ccap;
     const CombRec *lrec = comb_recs + (int64_t)parity * ccap;
@@ -603,22 +615,6 @@ __global__ __launch_bounds__(256) void k_comb_merge(Geo g, const DevState *__res
         crec[id] = out;
         if (lowest) height[c] = (((double)mh + (double)z) + co[2]) * g.z_res;
     }
-}
-
-// Commit of the combine (gvom.py:276-316): cell count of the new map (sum of the column-scan
-// chunk totals) and the ping-pong flip, which k_maps2d and later readers see.  One wavefront.
-__global__ void k_commit_combine(DevState *st, const int32_t *__restrict__ chunk_tot, int64_t nchunk) {
-    if (blockIdx.x != 0) return;
-    const int lane = threadIdx.x & 63;
-    uint32_t tot = 0;
-    for (int64_t k = lane; k < nchunk; k += 64) tot += (uint32_t)chunk_tot[k];
-    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-    if (threadIdx.x != 0) return;
-    const int cur = 1 - st->parity;
-    st->comb_cells[cur] = (int32_t)tot;
-    st->comb_total = (int32_t)tot;
-    st->parity = cur;
-    st->have_last = 1;
 }
 
 // ------------------------------------------------------------------------------------
@@ -695,7 +691,7 @@ __global__ __launch_bounds__(MT) void k_maps2d(Geo g, const DevState *__restrict
         int32_t pvw = 0;
         if (act) {
             const double h0 = HAT(x0, y0);
-            const int cur = st->parity;   // committed by k_commit_combine
+            const int cur = st->parity;   // flipped by the column scan
             const double oz = st->comb_origin[cur][2];
             const double flo = floor(((h0 + g.pos_thr) / g.z_res) - oz);
             const double fhi = floor(((h0 + g.robot_h) / g.z_res) - oz);
@@ -993,7 +989,6 @@ hipError_t launch_comb_records(const Geo &g, DevState *st, const int32_t *slot_t
                                                                  occ_list);
     k_comb_merge<<<2048, 256, 0, s>>>(g, st, slot_tables, slot_recs, ncap, comb_tables, comb_recs, ccap, occ_list,
                                       chunk_tot, col_chunks(g.X), height);
-    k_commit_combine<<<1, 64, 0, s>>>(st, chunk_tot, col_chunks(g.X));
     return hipGetLastError();
 }
 
