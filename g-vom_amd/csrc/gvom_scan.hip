@@ -291,8 +291,13 @@ __device__ __forceinline__ bool ray_advance(Ray &r, const Geo &g, int32_t &ix, i
     return true;
 }
 
+// ((ix * XY + iy) * Z + iz) as ix * (XY * Z) + iy * Z + iz: on the map ix, iy < XY <= 2^13 and
+// XY * Z <= 2^21, so both products are 24-bit multiplies (full-rate v_mad_u32_u24 instead of a
+// 64-bit multiply-add and a 32-bit multiply); a lane off the map gets a masked product, and its
+// key is discarded.
 __device__ __forceinline__ uint32_t lin_index(int32_t ix, int32_t iy, int32_t iz, const Geo &g) {
-    return ((uint32_t)ix * (uint32_t)g.XY + (uint32_t)iy) * (uint32_t)g.Z + (uint32_t)iz;
+    return __umul24((uint32_t)ix, (uint32_t)g.XY * (uint32_t)g.Z) + __umul24((uint32_t)iy, (uint32_t)g.Z) +
+           (uint32_t)iz;
 }
 
 // ray_advance for every lane at once, without branches (so a step costs no exec-mask
@@ -345,7 +350,9 @@ __host__ __device__ __forceinline__ TileGeo tile_geo(const Geo &g) {
     return t;
 }
 __device__ __forceinline__ uint32_t tiled_index(int32_t ix, int32_t iy, int32_t iz, const Geo &g, int32_t ty) {
-    return ((((uint32_t)(ix >> 2) * (uint32_t)ty + (uint32_t)(iy >> 2)) * (uint32_t)g.Z + (uint32_t)iz) << 4) +
+    // as lin_index: 24-bit products (ty * Z <= 2^19)
+    return ((__umul24((uint32_t)(ix >> 2), (uint32_t)ty * (uint32_t)g.Z) + __umul24((uint32_t)(iy >> 2), (uint32_t)g.Z) +
+             (uint32_t)iz) << 4) +
            ((uint32_t)(ix & 3) << 2) + (uint32_t)(iy & 3);
 }
 
