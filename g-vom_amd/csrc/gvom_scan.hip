@@ -267,6 +267,15 @@ __device__ __forceinline__ void ray_init(Ray &r, const ScanDesc &d, const Geo &g
     r.alive = r.length < r.rl1;
 }
 
+// (int32_t)floorf(x) in one instruction: v_cvt_flr_i32_f32 converts rounding toward -inf with
+// the same saturation and NaN rules as v_cvt_i32_f32, which the compiler otherwise emits after
+// a separate v_floor_f32.
+__device__ __forceinline__ int32_t floor_to_i32(float x) {
+    int32_t r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 // One step of an alive ray (gvom.py:1209-1231): returns false when the voxel it enters is
 // off the map (the ray is then dead), else its coordinates; advances length.
 __device__ __forceinline__ bool ray_advance(Ray &r, const Geo &g, int32_t &ix, int32_t &iy, int32_t &iz) {
@@ -363,9 +372,9 @@ __device__ __forceinline__ uint32_t ray_step_key(Ray &r, const Geo &g, uint32_t 
     r.p1 += r.inc1;
     r.p2 += r.inc2;
     // floor(pt - origin) in f64 (gvom.py:1215-1223) = floor(pt) - origin, as in ray_advance
-    const int32_t fx = (int32_t)floorf(r.p0) - r.o0;
-    const int32_t fy = (int32_t)floorf(r.p1) - r.o1;
-    const int32_t fz = (int32_t)floorf(r.p2) - r.o2;
+    const int32_t fx = floor_to_i32(r.p0) - r.o0;
+    const int32_t fy = floor_to_i32(r.p1) - r.o1;
+    const int32_t fz = floor_to_i32(r.p2) - r.o2;
     const bool in = ((uint32_t)fx < (uint32_t)g.XY) & ((uint32_t)fy < (uint32_t)g.XY) & ((uint32_t)fz < (uint32_t)g.Z);
     const bool step = was & in;
     r.length += r.dlen;   // read only while the ray is alive
