@@ -196,20 +196,25 @@ struct gvom_handle {
     const void *dg_pts = nullptr, *dk_pts = nullptr;
     int64_t dg_n = -1, dk_n = -1, dg_gen = -1;
     double dg_wh = 0.0, dk_wh = 0.0;
-    // A host-array scan call (gvom_process_pointcloud with host points) that repeats the
-    // previous one's point count and dtype replays a captured launch sequence: one launch
-    // instead of a dozen, no gaps between the kernels, and no transfer: the kernels read the
-    // points and the scan's descriptor from the pinned staging buffer itself (one sequence per
-    // buffer).  A sequence is replayed only under the buffer generation, the staging buffers
-    // (pin_gen, bumped when one is reallocated) and the launch settings (cfg_gen, bumped by
-    // every setting that changes what a call launches) it was captured under.
+    // A scan call that repeats the previous one's point count, dtype and source replays a
+    // captured launch sequence: one launch instead of a dozen and no gaps between the kernels.
+    // Host-array calls (gvom_process_pointcloud) need no transfer: the kernels read the points
+    // and the scan's descriptor from the pinned staging buffer itself.  Device-pointer calls
+    // without overlap (gvom_process_pointcloud_device) read the points from the caller's buffer
+    // and the descriptor from the staging buffer.  One sequence per staging buffer.  A
+    // sequence is replayed only for the source it was captured with (hp_src: the caller's
+    // device buffer, or null for host arrays) and under the buffer generation, the staging
+    // buffers (pin_gen, bumped when one is reallocated) and the launch settings (cfg_gen,
+    // bumped by every setting that changes what a call launches) it was captured under.
     hipGraph_t hp_graph[2] = {nullptr, nullptr};
     hipGraphExec_t hp_exec[2] = {nullptr, nullptr};
     int64_t hp_n = -1, hp_gen = -1, hp_cfg = -1, hp_pin_gen = -1;
     int hp_dtype = -1;
+    const void *hp_src = nullptr;
     int64_t pin_gen = 0;
-    int64_t hk_n = -1;   // the previous host-array scan call's point count and dtype
+    int64_t hk_n = -1;   // the previous eager scan call's point count, dtype and source
     int hk_dtype = -1;
+    const void *hk_src = nullptr;
     int64_t cfg_gen = 0;
 
     bool timing = false;
@@ -875,23 +880,26 @@ static void drop_graph(hipGraphExec_t &exec, hipGraph_t &graph) {
     graph = nullptr;
 }
 
-// Where a replayed scan's descriptor follows its points in the pinned staging buffer.
-static size_t host_desc_offset(int64_t n, int dtype) {
+// Where a replayed scan's descriptor lies in the pinned staging buffer: after the points of a
+// host-array call, at the start for a device-pointer call (src set).
+static size_t host_desc_offset(int64_t n, int dtype, const void *src) {
+    if (src) return 0;
     const size_t bytes = (size_t)n * 3 * (dtype == GVOM_F32 ? 4 : 8);
     return (bytes + 255) & ~(size_t)255;
 }
 
-// Captures the scan stage of one host-array scan of n points twice, into h->hp_exec[k] for
-// k = 0, 1: sequence k reads the points from pinned staging buffer k and the descriptor from
-// the same buffer at host_desc_offset(n, dtype).  Nothing runs and the host mirror is left as
-// it was.  The previous call had the same count and dtype, so every work buffer already holds
-// n points; a capture that reallocated a buffer anyway, or failed, keeps no sequence and the
-// call runs the eager path.
+// Captures the scan stage of one scan of n points twice, into h->hp_exec[k] for k = 0, 1:
+// sequence k reads the points from the device buffer src (a device-pointer call) or from
+// pinned staging buffer k (src null: a host-array call), and the descriptor from staging
+// buffer k at host_desc_offset(n, dtype, src).  Nothing runs and the host mirror is left as it
+// was.  The previous call had the same count, dtype and source, so every work buffer already
+// holds n points; a capture that reallocated a buffer anyway, or failed, keeps no sequence
+// and the call runs the eager path.
 static void capture_host_process(gvom_handle *h, int64_t n, int dtype, const double ego[3], const void *T,
-                                 int t_dtype) {
+                                 int t_dtype, const void *src) {
     for (int k = 0; k < 2; ++k) drop_graph(h->hp_exec[k], h->hp_graph[k]);
     h->hp_n = -1;
-    const size_t off = host_desc_offset(n, dtype), need = off + sizeof(ScanDesc);
+    const size_t off = host_desc_offset(n, dtype, src), need = off + sizeof(ScanDesc);
     if (hipStreamSynchronize(h->stream) != hipSuccess) return;   // nothing in flight reads the staging buffers
     void *dev[2] = {nullptr, nullptr};
     for (int k = 0; k < 2; ++k) {
@@ -928,7 +936,7 @@ static void capture_host_process(gvom_handle *h, int64_t n, int dtype, const dou
         h->capturing = true;
         h->cap_sd_dev = reinterpret_cast<ScanDesc *>((uint8_t *)dev[k] + off);
         int64_t offs[2] = {0, n};
-        const int rc = process_impl(h, dev[k], offs, 1, dtype, ego, T, t_dtype, true, false);
+        const int rc = process_impl(h, src ? src : dev[k], offs, 1, dtype, ego, T, t_dtype, true, false);
         hipGraph_t graph = nullptr;
         const hipError_t ee = hipStreamEndCapture(h->stream, &graph);
         h->capturing = false;
@@ -957,6 +965,7 @@ static void capture_host_process(gvom_handle *h, int64_t n, int dtype, const dou
     }
     h->hp_n = n;
     h->hp_dtype = dtype;
+    h->hp_src = src;
     h->hp_gen = gen;
     h->hp_cfg = h->cfg_gen;
     h->hp_pin_gen = h->pin_gen;
@@ -964,13 +973,14 @@ static void capture_host_process(gvom_handle *h, int64_t n, int dtype, const dou
 
 constexpr int NOT_REPLAYED = 100;
 
-// A host-array scan call through a captured scan stage: the points and the descriptor are
-// written into one of the two pinned staging buffers (used in turn, so the call waits only for
-// the sequence two calls back, the last reader of that buffer) and the sequence bound to that
-// buffer is launched.  Returns NOT_REPLAYED when the call is not taken; the eager path then
-// runs it and reports any error in its arguments.
+// A scan call through a captured scan stage: the descriptor (and a host-array call's points)
+// are written into one of the two pinned staging buffers (used in turn, so the call waits only
+// for the sequence two calls back, the last reader of that buffer) and the sequence bound to
+// that buffer is launched.  on_device: xyz is a device buffer the sequence reads in place.
+// Returns NOT_REPLAYED when the call is not taken; the eager path then runs it and reports
+// any error in its arguments.
 static int process_host_replay(gvom_handle *h, const void *xyz, int64_t n, int dtype, const double ego[3],
-                               const void *T, int t_dtype) {
+                               const void *T, int t_dtype, bool on_device) {
     const Geo &g = h->g;
     if (h->capturing || h->timing || n <= 0 || !xyz || (dtype != GVOM_F32 && dtype != GVOM_F64))
         return NOT_REPLAYED;
@@ -979,14 +989,15 @@ static int process_host_replay(gvom_handle *h, const void *xyz, int64_t n, int d
         const double res = k < 2 ? g.xy_res : g.z_res;
         if (!std::isfinite(ego[k]) || std::fabs(ego[k] / res) >= 4194304.0) return NOT_REPLAYED;
     }
+    const void *src = on_device ? xyz : nullptr;
     auto valid = [&]() {
-        return h->hp_exec[0] && h->hp_exec[1] && h->hp_n == n && h->hp_dtype == dtype && h->hp_gen == h->buf_gen &&
-               h->hp_cfg == h->cfg_gen && h->hp_pin_gen == h->pin_gen;
+        return h->hp_exec[0] && h->hp_exec[1] && h->hp_n == n && h->hp_dtype == dtype && h->hp_src == src &&
+               h->hp_gen == h->buf_gen && h->hp_cfg == h->cfg_gen && h->hp_pin_gen == h->pin_gen;
     };
     if (!valid()) {
-        // captured on the second call in a row with this count and dtype
-        if (n != h->hk_n || dtype != h->hk_dtype) return NOT_REPLAYED;
-        capture_host_process(h, n, dtype, ego, T, t_dtype);
+        // captured on the second call in a row with this count, dtype and source
+        if (n != h->hk_n || dtype != h->hk_dtype || src != h->hk_src) return NOT_REPLAYED;
+        capture_host_process(h, n, dtype, ego, T, t_dtype, src);
         if (!valid()) return NOT_REPLAYED;
     }
     ScanDesc d;
@@ -1001,10 +1012,10 @@ static int process_host_replay(gvom_handle *h, const void *xyz, int64_t n, int d
     d.offset = 0;
     d.count = n;
     const size_t pbytes = (size_t)n * 3 * (dtype == GVOM_F32 ? 4 : 8);
-    const size_t off = host_desc_offset(n, dtype);
+    const size_t off = host_desc_offset(n, dtype, src);
     const int k = h->pin_next;
     CKH(h, hipEventSynchronize(h->ev_pin[k]));
-    std::memcpy(h->pin_pts[k], xyz, pbytes);
+    if (!on_device) std::memcpy(h->pin_pts[k], xyz, pbytes);
     std::memcpy((uint8_t *)h->pin_pts[k] + off, &d, sizeof(d));
     h->diff_n = -1;   // the sequence reuses the work buffers a pending gvom_diff_backward reads
     CKH(h, hipGraphLaunch(h->hp_exec[k], h->stream));
@@ -1025,13 +1036,14 @@ extern "C" int gvom_process_pointcloud(gvom_handle *h, const void *xyz, int64_t 
                                        const double ego[3], const void *T, int32_t t_dtype) {
     if (!h || !ego || n < 0) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
-    const int rr = process_host_replay(h, xyz, n, dtype, ego, T, t_dtype);
+    const int rr = process_host_replay(h, xyz, n, dtype, ego, T, t_dtype, false);
     if (rr != NOT_REPLAYED) return rr;
     int64_t off[2] = {0, n};
     const int rc = process_impl(h, xyz, off, 1, dtype, ego, T, t_dtype, false, false);
     if (rc == GVOM_OK) {
         h->hk_n = n;
         h->hk_dtype = dtype;
+        h->hk_src = nullptr;
     }
     return rc;
 }
@@ -1040,8 +1052,21 @@ extern "C" int gvom_process_pointcloud_device(gvom_handle *h, const void *xyz, i
                                               const double ego[3], const void *T, int32_t t_dtype) {
     if (!h || !ego || n < 0) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    // with overlap on, the stage forks from the previous commit and runs beside the combine
+    // enqueued before it, which a sequence launched on the main stream could not
+    const bool replayable = !h->overlap;
+    if (replayable) {
+        const int rr = process_host_replay(h, xyz, n, dtype, ego, T, t_dtype, true);
+        if (rr != NOT_REPLAYED) return rr;
+    }
     int64_t off[2] = {0, n};
-    return process_impl(h, xyz, off, 1, dtype, ego, T, t_dtype, true, true);
+    const int rc = process_impl(h, xyz, off, 1, dtype, ego, T, t_dtype, true, true);
+    if (rc == GVOM_OK) {
+        h->hk_n = replayable ? n : -1;
+        h->hk_dtype = dtype;
+        h->hk_src = xyz;
+    }
+    return rc;
 }
 
 extern "C" int gvom_process_pointcloud_device_noalloc(gvom_handle *h, const void *xyz, int64_t n, int32_t dtype,

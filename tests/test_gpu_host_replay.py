@@ -1,8 +1,9 @@
 """Host-array calls replayed from captured launch sequences (runs on an MI355X).
 
-From the second host-array scan call in a row with the same point count and dtype, a handle
-replays a captured launch sequence instead of launching each kernel.  Handle A takes that
-path; handle B has stage timing on, which keeps every call on the eager path.  Both see the same calls and must give the same maps, bit for bit, after every
+From the second scan call in a row with the same point count, dtype and source (host arrays,
+or one device buffer with overlap off), a handle replays a captured launch sequence instead of
+launching each kernel.  Handle A takes that path; handle B has stage timing on, which keeps
+every call on the eager path.  Both see the same calls and must give the same maps, bit for bit, after every
 combine, along a sequence that changes each thing a replay depends on: the point count, the
 point dtype, the transform's presence and dtype, the launch settings, the buffers (reserve),
 several scans before one combine, and a rejected call in between.  The last maps are also
@@ -98,6 +99,72 @@ def test_host_replay_matches_eager():
     # the replayed handle against the oracle
     compare_combined(A.combined_state(), o.combined, "replayed handle combined")
     compare_maps(A.maps_state(), o.combined, "replayed handle")
+    assert_equal("origin_m", out_a[0], out_o[0])
+    for name, a, b in zip(("pos", "neg", "vis"), (out_a[1], out_a[2], out_a[4]), (out_o[1], out_o[2], out_o[4])):
+        assert_equal(name, a, b)
+
+
+def test_device_replay_matches_eager():
+    import gvom
+    import gvom_native as nat
+    wl = synth.workload(2, frames=2)
+    base = wl.steps[0][0]
+    other = wl.steps[1][0]
+    A = gvom.Gvom(*wl.map_args)
+    B = gvom.Gvom(*wl.map_args)
+    B.set_timing(True)
+    o = OracleGvom(*wl.map_args)
+    d_base = nat.DeviceArray(base.points)
+    d_other = nat.DeviceArray(other.points)
+    d_copy = nat.DeviceArray(base.points)
+
+    def dproc(buf, n, pts, ego, T):
+        for g in (A, B):
+            g.process_pointcloud_device(buf, n, ego, T)
+        o.Process_pointcloud(pts, ego, T)
+
+    def hproc(pts, ego, T):
+        for g in (A, B):
+            g.Process_pointcloud(pts, ego, T)
+        o.Process_pointcloud(pts, ego, T)
+
+    def comb(label):
+        out_a = A.combine_maps()
+        _same(out_a, B.combine_maps(), label)
+        return out_a, o.combine_maps()
+
+    def ego(k):
+        return base.ego + np.array([0.5 * k, 0.3 * k, -0.1 * (k % 2)])
+
+    n = base.points.shape[0]
+    # one device buffer: captured on the second call, then replayed
+    for k in range(3):
+        dproc(d_base, n, base.points, ego(k), base.transform)
+        comb(f"device frame {k}")
+    # another buffer with the same points, then back: a new source each time
+    dproc(d_copy, n, base.points, ego(3), base.transform)
+    dproc(d_copy, n, base.points, ego(4), base.transform)
+    comb("second device buffer")
+    dproc(d_base, n, base.points, ego(5), base.transform)
+    comb("first buffer again")
+    # fewer points from the same buffer (another count)
+    dproc(d_base, n // 3, base.points[: n // 3], ego(6), base.transform)
+    dproc(d_base, n // 3, base.points[: n // 3], ego(7), None)
+    comb("a third of the points")
+    # host arrays in between, then the device buffer again
+    hproc(base.points, ego(8), base.transform)
+    hproc(base.points, ego(9), base.transform)
+    dproc(d_other, other.points.shape[0], other.points, other.ego, other.transform)
+    dproc(d_other, other.points.shape[0], other.points, other.ego + 0.25, other.transform)
+    comb("host arrays, then another scan's buffer")
+    # overlap on: eager calls (the stage forks from the previous commit), same maps
+    for g in (A, B):
+        g.set_overlap(True)
+    dproc(d_base, n, base.points, ego(10), base.transform)
+    dproc(d_base, n, base.points, ego(11), base.transform)
+    out_a, out_o = comb("overlap on")
+    compare_combined(A.combined_state(), o.combined, "device replay combined")
+    compare_maps(A.maps_state(), o.combined, "device replay")
     assert_equal("origin_m", out_a[0], out_o[0])
     for name, a, b in zip(("pos", "neg", "vis"), (out_a[1], out_a[2], out_a[4]), (out_o[1], out_o[2], out_o[4])):
         assert_equal(name, a, b)
