@@ -243,6 +243,13 @@ struct gvom_handle {
     int overlap = 0;
     hipStream_t front = nullptr;
     hipEvent_t ev_front = nullptr, ev_back = nullptr;
+    // An overlapped stage's commit waits only up to its segment sums (ev_front), which write
+    // what the combine's table pass reads; its neighbourhood metrics (slot records' moments)
+    // may still run on `front`, ending at ev_metrics.  metrics_pending: the main stream has
+    // not waited for them yet; the combine waits just before its record merge, every other
+    // call that touches the device at its start (join_metrics).
+    hipEvent_t ev_metrics = nullptr;
+    bool metrics_pending = false;
     // An overlapped stage inside a capture forks `front` from the main stream and `side` from
     // `front`.  hipStreamEndCapture has crashed on that nested fork under an older HIP runtime
     // than the one this library is built against (torch bundles one, and when torch is imported
@@ -263,6 +270,14 @@ struct gvom_handle {
             return GVOM_EHIP;                                                        \
         }                                                                            \
     } while (0)
+
+// Orders the main stream after the neighbourhood metrics an overlapped scan stage left
+// running on `front`.
+static hipError_t join_metrics(gvom_handle *h) {
+    if (!h->metrics_pending) return hipSuccess;
+    h->metrics_pending = false;
+    return hipStreamWaitEvent(h->stream, h->ev_metrics, 0);
+}
 
 static void free_all(gvom_handle *h) {
     void *ptrs[] = {h->st, h->slot_tables, h->slot_recs, h->comb_tables, h->comb_recs, h->height, h->inferred,
@@ -293,6 +308,7 @@ static void free_all(gvom_handle *h) {
     if (h->ev_out) (void)hipEventDestroy(h->ev_out);
     if (h->ev_front) (void)hipEventDestroy(h->ev_front);
     if (h->ev_back) (void)hipEventDestroy(h->ev_back);
+    if (h->ev_metrics) (void)hipEventDestroy(h->ev_metrics);
     if (h->side) (void)hipStreamDestroy(h->side);
     if (h->front) (void)hipStreamDestroy(h->front);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -381,6 +397,7 @@ extern "C" int gvom_create(const gvom_params *params, gvom_handle **out) {
     CKC(hipStreamCreateWithFlags(&h->front, hipStreamNonBlocking));
     CKC(hipEventCreateWithFlags(&h->ev_front, hipEventDisableTiming));
     CKC(hipEventCreateWithFlags(&h->ev_back, hipEventDisableTiming));
+    CKC(hipEventCreateWithFlags(&h->ev_metrics, hipEventDisableTiming));
     CKC(hipEventRecord(h->ev_back, h->stream));
     CKC(hipEventRecord(h->ev_pin[0], h->stream));
     CKC(hipEventRecord(h->ev_pin[1], h->stream));
@@ -436,7 +453,13 @@ extern "C" const char *gvom_last_error(const gvom_handle *h) {
     return h ? h->err.c_str() : g_create_error.c_str();
 }
 
-extern "C" void *gvom_stream(gvom_handle *h) { return h ? (void *)h->stream : nullptr; }
+extern "C" void *gvom_stream(gvom_handle *h) {
+    if (!h) return nullptr;
+    // work the caller orders after the stream also follows pending metrics
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (join_metrics(h) != hipSuccess) h->err = "hipStreamWaitEvent on the metrics event failed";
+    return (void *)h->stream;
+}
 
 // Grow per-batch work buffers (points) and per-slot record capacity (cells per scan).
 static int ensure_capacity(gvom_handle *h, int64_t max_scan, int64_t batch_pts) {
@@ -521,6 +544,7 @@ static bool tiled_fits(const Geo &g);
 extern "C" int gvom_reserve(gvom_handle *h, int64_t max_points, int64_t max_batch_points) {
     if (!h) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     if (max_points < 0 || max_batch_points < 0) {
         h->err = "negative size";
         return GVOM_EINVAL;
@@ -564,6 +588,7 @@ static void stage_event(gvom_handle *h, int k, hipStream_t on = nullptr) {
 // Everything enqueued on the main stream so far may use the scan stage's work buffers: the
 // next overlapped scan stage waits for it.
 static hipError_t mark_back(gvom_handle *h) { return hipEventRecord(h->ev_back, h->stream); }
+
 
 // Maps whose tiled scratch keys would not fit in 31 bits use the direct cast instead.
 static bool tiled_fits(const Geo &g) { return gvom_plan::tiled_cast_fits(g.XY, g.Z, g.B); }
@@ -768,6 +793,9 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     // state, buffer bits) goes on st, after that combine.
     const bool ovl = h->overlap && on_device && may_overlap && (h->nested_capture || !h->capturing);
     hipStream_t fs = ovl ? h->front : st;
+    // a stage on the main stream reuses the work buffers pending metrics still read; an
+    // overlapped one runs on `front` after them
+    if (!ovl) CKH(h, join_metrics(h));
     if (ovl) CKH(h, hipStreamWaitEvent(fs, h->ev_back, 0));
     const size_t esz = dtype == GVOM_F32 ? 4 : 8;
     const void *pts = xyz;
@@ -849,11 +877,18 @@ static int process_impl(gvom_handle *h, const void *xyz, const int64_t *offsets,
     stage_event(h, 3, fs);
     CKH(h, launch_mom_seg(W.seg_start, W.seg_key, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, W.part,
                           W.mom, fs));
-    CKH(h, launch_metrics(W.seg_key, W.mom, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, S, fs));
     if (ovl) {
-        // back on the main stream, after any combine enqueued before this call
+        // back on the main stream, after any combine enqueued before this call: the commit
+        // needs the tables' occupied words and the records' counts, which the segment sums
+        // wrote; the metrics run on beside the commit and the next combine's table pass
         CKH(h, hipEventRecord(h->ev_front, fs));
         CKH(h, hipStreamWaitEvent(st, h->ev_front, 0));
+        h->metrics_pending = false;   // `front` ran any earlier stage's metrics before ev_front
+    }
+    CKH(h, launch_metrics(W.seg_key, W.mom, NT, g, h->st, h->slot_tables, h->slot_recs, h->ncap, S, fs));
+    if (ovl) {
+        CKH(h, hipEventRecord(h->ev_metrics, fs));
+        h->metrics_pending = true;
     }
     CKH(h, launch_commit_process(h->st, sd, g, S, st));
     stage_event(h, 4);
@@ -989,6 +1024,8 @@ static int process_host_replay(gvom_handle *h, const void *xyz, int64_t n, int d
         const double res = k < 2 ? g.xy_res : g.z_res;
         if (!std::isfinite(ego[k]) || std::fabs(ego[k] / res) >= 4194304.0) return NOT_REPLAYED;
     }
+    // the sequence runs on the main stream and reuses the work buffers pending metrics read
+    CKH(h, join_metrics(h));
     const void *src = on_device ? xyz : nullptr;
     auto valid = [&]() {
         return h->hp_exec[0] && h->hp_exec[1] && h->hp_n == n && h->hp_dtype == dtype && h->hp_src == src &&
@@ -1104,6 +1141,7 @@ extern "C" int gvom_process_pointcloud2(gvom_handle *h, const void *data, const 
                                         const double ego[3], const void *T, int32_t t_dtype, int32_t on_device) {
     if (!h || !L || !ego) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     static const int fsize[9] = {0, 1, 1, 2, 2, 4, 4, 4, 8};
     if (L->height < 0 || L->width < 0 || L->point_step <= 0 || L->row_step < 0) {
         h->err = "PointCloud2 layout: negative size or non-positive point_step";
@@ -1168,6 +1206,7 @@ static int combine_impl(gvom_handle *h, uint8_t *host_maps = nullptr) {
                               h->height, h->inferred, st));
     stage_event(h, 6);
     CKH(h, launch_scan_columns(h->col_count, h->col_count_words, g.X, h->chunk_tot, h->chunk_tot_words, h->st, st));
+    CKH(h, join_metrics(h));   // the record merge reads the newest stage's metrics
     CKH(h, launch_comb_records(g, h->st, h->slot_tables, h->slot_recs, h->ncap, h->comb_tables, h->comb_recs,
                                h->ccap, h->col_count, h->chunk_tot, h->occ_bits, h->occ_list, h->height, st));
     stage_event(h, 7);
@@ -1227,6 +1266,7 @@ extern "C" int gvom_occupancy_grids(gvom_handle *h, double density_threshold, do
                                     double max_roughness, int8_t *out, int32_t on_device, double origin_m[3]) {
     if (!h || !out) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     bool has = h->have_last != 0;
     if (h->capturing) {
         if (!on_device) {
@@ -1252,6 +1292,7 @@ extern "C" int gvom_occupancy_grids(gvom_handle *h, double density_threshold, do
 extern "C" int gvom_expand_binary(gvom_handle *h, const int32_t *in, int32_t *out, int32_t xy_size, double r) {
     if (!h || !in || !out || xy_size < 1) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     CKH(h, launch_expand_binary(in, out, xy_size, r, h->stream));
     return GVOM_OK;
 }
@@ -1260,6 +1301,7 @@ extern "C" int gvom_lowpass_binary(gvom_handle *h, const int32_t *in, int32_t *o
                                    int32_t filter_size, double filter_fraction) {
     if (!h || !in || !out || xy_size < 1 || filter_size < 0) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     CKH(h, launch_lowpass_binary(in, out, xy_size, filter_size, filter_fraction, h->stream));
     return GVOM_OK;
 }
@@ -1271,6 +1313,7 @@ extern "C" int gvom_convolve(gvom_handle *h, const double *in, const double *ker
         return GVOM_EINVAL;
     }
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     CKH(h, launch_convolve(in, kernel, out, xy_size, kernel_size, h->stream));
     return GVOM_OK;
 }
@@ -1278,6 +1321,7 @@ extern "C" int gvom_convolve(gvom_handle *h, const double *in, const double *ker
 extern "C" int gvom_probe_atomics(gvom_handle *h, int32_t pattern, int64_t n_atomics, float *ms) {
     if (!h || !ms || pattern < 0 || pattern > 5 || n_atomics <= 0) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     if (h->capturing) {
         h->err = "cannot probe while capturing";
         return GVOM_ESTATE;
@@ -1321,6 +1365,7 @@ extern "C" int gvom_device_maps(gvom_handle *h, int32_t **pos, int32_t **neg, do
 extern "C" int gvom_sync(gvom_handle *h) {
     if (!h) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     CKH(h, hipStreamSynchronize(h->stream));
     return GVOM_OK;
 }
@@ -1347,6 +1392,7 @@ extern "C" int gvom_get_slot(gvom_handle *h, int32_t slot, int32_t *index_map, i
                              int32_t *total, float *min_h, double *metrics, double origin[3], int64_t *cells) {
     if (!h) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     if (slot < 0 || slot >= h->g.B) {
         h->err = "slot out of range";
         return GVOM_EINVAL;
@@ -1388,6 +1434,7 @@ extern "C" int gvom_get_combined(gvom_handle *h, int32_t *index_map, int64_t cap
                                  float *min_h, float *metrics, float *eigen, double origin[3], int64_t *cells) {
     if (!h) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     if (!h->have_last) return GVOM_NO_DATA;
     DevState ds;
     int rc = read_state(h, &ds);
@@ -1427,6 +1474,7 @@ extern "C" int gvom_get_maps(gvom_handle *h, double *height, double *inferred, d
                              double *rough, double *delta, int32_t *pos, int32_t *neg, int32_t *vis) {
     if (!h) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     if (!h->have_last) return GVOM_NO_DATA;
     const size_t X = (size_t)h->g.X;
     hipStream_t st = h->stream;
@@ -1456,6 +1504,7 @@ extern "C" int gvom_ring_state(const gvom_handle *h, int32_t *buffer_index, int3
 extern "C" int gvom_debug_voxel_map(gvom_handle *h, float *out, int64_t cap, int64_t *rows) {
     if (!h) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     if (!h->have_last) return GVOM_NO_DATA;
     DevState ds;
     int rc = read_state(h, &ds);
@@ -1482,6 +1531,7 @@ extern "C" int gvom_debug_voxel_map(gvom_handle *h, float *out, int64_t cap, int
 static int debug_height_common(gvom_handle *h, float *out7, float *out3) {
     if (!h) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     if (!h->have_last) return GVOM_NO_DATA;
     if (!out7 && !out3) return GVOM_OK;
     const int64_t X = h->g.X;
@@ -1508,6 +1558,7 @@ extern "C" int gvom_debug_inferred_height_map(gvom_handle *h, float *out) {
 extern "C" int gvom_set_timing(gvom_handle *h, int32_t enabled) {
     if (!h) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     h->timing = enabled != 0;
     h->set_used.clear();
     h->cur_set = -1;
@@ -1517,6 +1568,7 @@ extern "C" int gvom_set_timing(gvom_handle *h, int32_t enabled) {
 extern "C" int gvom_stage_times(gvom_handle *h, float *ms, int32_t n) {
     if (!h || !ms) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     CKH(h, hipStreamSynchronize(h->stream));
     static const int pairs[9][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {5, 6}, {6, 7}, {7, 8}, {1, 9}, {9, 2}};
     for (int k = 0; k < n; ++k) {
@@ -1736,6 +1788,7 @@ extern "C" int gvom_diff_forward(gvom_handle *h, const void *points, int64_t n, 
                                  const double ego[3], double w_height, double out[3]) {
     if (!h || !T || !ego || n < 0) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     return diff_forward_impl(h, points, n, T, ego, w_height, out, nullptr, false, nullptr);
 }
 
@@ -1743,6 +1796,7 @@ extern "C" int gvom_diff_forward_async(gvom_handle *h, const void *points, int64
                                        const double ego[3], double w_height, double *out_dev, void *stream) {
     if (!h || !T || !ego || n < 0) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     return diff_forward_impl(h, points, n, T, ego, w_height, nullptr, out_dev, true, (hipStream_t)stream);
 }
 
@@ -1786,6 +1840,7 @@ extern "C" int gvom_diff_backward(gvom_handle *h, double dL, double *grad_points
                                   double grad_xi[6]) {
     if (!h) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     return diff_backward_impl(h, dL, nullptr, grad_points, grad_world, grad_xi, nullptr, false, nullptr);
 }
 
@@ -1793,6 +1848,7 @@ extern "C" int gvom_diff_backward_async(gvom_handle *h, const double *dL_dev, do
                                         double *grad_world, double *grad_xi_dev, void *stream) {
     if (!h || !dL_dev) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     return diff_backward_impl(h, 0.0, dL_dev, grad_points, grad_world, nullptr, grad_xi_dev, true,
                               (hipStream_t)stream);
 }
@@ -1809,6 +1865,7 @@ struct gvom_graph {
 extern "C" int gvom_graph_begin(gvom_handle *h, int32_t max_scans) {
     if (!h || max_scans < 1) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     if (h->capturing) {
         h->err = "a capture is already in progress";
         return GVOM_ESTATE;
@@ -1845,6 +1902,7 @@ extern "C" int gvom_graph_begin(gvom_handle *h, int32_t max_scans) {
 extern "C" int gvom_graph_end(gvom_handle *h, gvom_graph **out) {
     if (!h || !out) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     *out = nullptr;
     if (!h->capturing) {
         h->err = "no capture in progress";
@@ -1883,6 +1941,7 @@ extern "C" int gvom_graph_scans(const gvom_graph *gr) {
 extern "C" int gvom_graph_launch(gvom_handle *h, gvom_graph *gr, const double *ego, const void *T, int32_t t_dtype) {
     if (!h || !gr || (!ego && !gr->sd.empty())) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     if (h->capturing) {
         h->err = "cannot launch while capturing";
         return GVOM_ESTATE;
@@ -1955,6 +2014,7 @@ extern "C" int gvom_autotune(gvom_handle *h, const void *points, int64_t n, int3
                              const void *T, int32_t t_dtype, int32_t chosen[3], float *best_ms) {
     if (!h || !ego || n <= 0 || (dtype != GVOM_F32 && dtype != GVOM_F64)) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     if (h->capturing) {
         h->err = "cannot autotune while capturing";
         return GVOM_ESTATE;
@@ -2086,6 +2146,7 @@ extern "C" int gvom_hip_versions(int32_t *built, int32_t *running) {
 extern "C" int gvom_set_overlap(gvom_handle *h, int32_t enabled) {
     if (!h) return GVOM_EINVAL;
     std::lock_guard<std::mutex> lk(h->mu);
+    CKH(h, join_metrics(h));
     if (h->capturing) {
         h->err = "cannot change overlap while capturing";
         return GVOM_ESTATE;

@@ -59,6 +59,9 @@ struct DevState {
     int32_t scan_pos[GVOM_MAX_BATCH];        // first sorted position of each scan's valid keys
     int32_t scan_valid[GVOM_MAX_BATCH];      // valid keys per scan
     int32_t chunk_off[GVOM_MAX_BATCH + 1];   // moment chunks before each scan (scan-relative chunking)
+    // physical buffer each scan of the batch in flight writes, fixed before its commit: the
+    // neighbourhood metrics read it while the commit may already rewrite flip and next_slot
+    int32_t stage_buf[GVOM_MAX_BATCH];
     double slot_origin[GVOM_MAX_BUFFER][3];
     double comb_origin[2][3];
     double latest_ego[3];

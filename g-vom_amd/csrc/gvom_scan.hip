@@ -819,6 +819,7 @@ __device__ void seg_bounds_body(const uint32_t *__restrict__ blk, int nblk, int 
         st->first_seg[s] = f;
         st->inb_end[s] = f + in;
         st->scan_cells[s] = in;
+        st->stage_buf[s] = new_buf(st->flip, (st->next_slot + s) % g.B);
         st->scan_pos[s] = pos;
         st->scan_valid[s] = nv;
         st->chunk_off[s] = ch;
@@ -1162,7 +1163,7 @@ __global__ __launch_bounds__(256) void k_metrics(const uint32_t *__restrict__ se
         const uint32_t vk = key & ((1u << g.VB) - 1u);
         if ((vk >> (g.VB - 1)) & 1u) continue;   // halo segment: no record (uniform in the group)
         const int s = (int)(key >> g.VB);
-        const int buf = new_buf(st->flip, (st->next_slot + s) % g.B);
+        const int buf = st->stage_buf[s];   // the commit may already run beside this kernel
         const int32_t fs = st->first_seg[s], ie = st->inb_end[s];
         const int32_t *table = tables + (int64_t)buf * g.V;
         const uint32_t Z = (uint32_t)g.Z, XY = (uint32_t)g.XY;

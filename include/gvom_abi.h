@@ -265,6 +265,10 @@ void *gvom_stream(gvom_handle *h);
  * on_device) runs on a second stream as soon as the previous scan stage has finished,
  * concurrently with a gvom_combine_maps_device enqueued in between; only its commit (ring
  * state, buffer selection) and everything else stay ordered on the handle's stream.  The
+ * commit waits for the stage up to its segment sums; the stage's neighbourhood metrics (the
+ * slot records' moments) may still run beside it and beside the next combine, whose record
+ * merge waits for them, as does every other call on the handle when it starts (and
+ * gvom_stream, so that work ordered after the stream it returns follows them).  The
  * caller's points must therefore be complete when the process call is made (not merely
  * enqueued on the handle's stream after the preceding combine).  Also applies inside a
  * hipGraph capture, unless the HIP runtime in use is older than the one the library was built
