@@ -3,7 +3,8 @@
 From the second scan call in a row with the same point count, dtype and source (host arrays,
 or one device buffer with overlap off), a handle replays a captured launch sequence instead of
 launching each kernel.  Handle A takes that path; handle B has stage timing on, which keeps
-every call on the eager path.  Both see the same calls and must give the same maps, bit for bit, after every
+every call on the eager path.  A third test covers overlap mode's pending metrics: slot exports
+right after a scan call wait for them.  Both see the same calls and must give the same maps, bit for bit, after every
 combine, along a sequence that changes each thing a replay depends on: the point count, the
 point dtype, the transform's presence and dtype, the launch settings, the buffers (reserve),
 several scans before one combine, and a rejected call in between.  The last maps are also
@@ -168,3 +169,31 @@ def test_device_replay_matches_eager():
     assert_equal("origin_m", out_a[0], out_o[0])
     for name, a, b in zip(("pos", "neg", "vis"), (out_a[1], out_a[2], out_a[4]), (out_o[1], out_o[2], out_o[4])):
         assert_equal(name, a, b)
+
+
+def test_overlap_getters_wait_for_metrics():
+    """With overlap on, a scan stage's neighbourhood metrics may still run after its call
+    returns; a slot export right after it (no combine in between) must see them finished, as a
+    serial handle does, and so must the maps of the combines that follow."""
+    import gvom
+    import gvom_native as nat
+    wl = synth.workload(2, frames=2)
+    scans = [st[0] for st in wl.steps]
+    dev = [nat.DeviceArray(s.points) for s in scans]
+    serial = gvom.Gvom(*wl.map_args)
+    ovl = gvom.Gvom(*wl.map_args)
+    ovl.set_overlap(True)
+    for k in range(5):
+        s = scans[k % 2]
+        ego = s.ego + np.array([0.35 * k, -0.2 * k, 0.05 * k])
+        for g in (serial, ovl):
+            g.process_pointcloud_device(dev[k % 2], s.points.shape[0], ego, s.transform)
+        for slot in range(serial.buffer_size):
+            a, b = serial.slot_state(slot), ovl.slot_state(slot)
+            assert (a is None) == (b is None), f"scan {k} slot {slot}"
+            if a is not None:
+                for name in a:
+                    assert np.array_equal(a[name], b[name]), f"scan {k} slot {slot} {name}"
+        if k % 2:
+            _same(serial.combine_maps(), ovl.combine_maps(), f"combine after scan {k}")
+    print("overlapped slot exports and combines equal the serial handle's", flush=True)
